@@ -1,0 +1,132 @@
+"""Benchmark: ray.surface intersections per second, FP64, one MI355X.
+
+Workload (one step): the 12-surface echelle spectrograph (rt_amd/schemes.py,
+echelle()) at 4883 rays x 2048 wavelengths = 10,000,384 rays: beam generated
+in-kernel (Philox, seed per step), traced through every surface, integer
+hit-count frame and QE-weighted FP64 flux frame accumulated on the 4096^2
+CCD.  All buffers are resident in HBM before the timed region.
+
+value = sum over non-auxiliary surfaces of the rays alive on entering that
+surface's intersection stage (read from the kernel's own counters), divided
+by the timed wall time of the K steps.  The CPU baseline is the oracle's
+fused trace on a bounded sample of the same rays on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n-per-lambda", type=int, default=4883)
+    ap.add_argument("--nlambda", type=int, default=2048)
+    ap.add_argument("--cpu-sample", type=int, default=200000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine is measured on one GPU; --gpus > 1 is not supported "
+                                   "(rays shard by range, see DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+    from rt_amd import flops, lib, schemes
+    from rt_amd import scheme as S
+
+    lib.init(0)
+    sch = schemes.echelle(args.n_per_lambda, args.nlambda)
+    n = sch.beam.n_rays
+    ds = lib.DeviceScheme(sch)
+    det = sch.detector()
+    npix = det.ccd["ydim"] * det.ccd["xdim"]
+    rays = torch.empty((7, n), dtype=torch.float64, device="cuda")
+    flag = torch.empty(n, dtype=torch.int8, device="cuda")
+    hist = torch.zeros(npix, dtype=torch.int32, device="cuda")
+    flux = torch.zeros(npix, dtype=torch.float64, device="cuda")
+    stats = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
+    flags = S.RT_TRACE_GENERATE | S.RT_TRACE_STORE | S.RT_TRACE_HIST | S.RT_TRACE_FLUX
+    stream = torch.cuda.current_stream()
+
+    def step(k):
+        ds.trace(rays, flag, seed=k, flags=flags, hist=hist, flux=flux, stats=stats,
+                 stream=stream.cuda_stream)
+
+    for k in range(args.warmup):
+        step(1000 + k)
+    torch.cuda.synchronize()
+    stats.zero_()
+    hist.zero_()
+    flux.zero_()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for k in range(args.steps):
+        step(k)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    st = stats.cpu().numpy().view(np.uint64)
+    ray_surf = int(sum(int(st[S.ST_SURF + 4 * s]) for s in range(sch.n_surf)))
+    ms_per_step = wall * 1e3 / args.steps
+    value = ray_surf / wall
+    kernel_ms = gpu_ms / args.steps
+    fl = flops.trace_flops(sch, st) / args.steps
+    achieved_tflops = fl / (kernel_ms * 1e-3) / 1e12
+
+    out = {
+        "metric": "ray-surface hits/s FP64 on 1 MI355X (12-surface echelle, 1e7 rays)",
+        "value": value,
+        "unit": "ray*surface/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Philox-generated beam, seed = step index)",
+        "config": {"workload": "echelle-12surf", "rays": n, "n_per_lambda": args.n_per_lambda,
+                   "nlambda": args.nlambda, "surfaces": sch.n_surf, "ccd": "4096x4096",
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": None},
+        "detail": {"kernel_ms": kernel_ms, "ray_surface_per_step": ray_surf // args.steps,
+                   "ccd_hits_per_step": int(st[S.ST_CCD_HITS]) // args.steps,
+                   "flop_per_step": fl},
+    }
+
+    if not args.no_cpu:
+        from oracle import oracle as O
+        m = min(args.cpu_sample, n)
+        threads = len(os.sched_getaffinity(0))
+        t0 = time.perf_counter()
+        r = O.trace(sch, seed=0, n=m, nthreads=threads)
+        cpu_s = time.perf_counter() - t0
+        cst = r.stats
+        cpu_rs = int(sum(int(cst[S.ST_SURF + 4 * s]) for s in range(sch.n_surf)))
+        out["cpu_baseline"] = {"value": cpu_rs / cpu_s, "unit": "ray*surface/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"first {m} rays of the same echelle beam, oracle fused trace"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,182 @@
+/*
+ * rt_hip.h -- C ABI of the MI355X (gfx950) per-ray tracing engine.
+ *
+ * This is the thin extern "C" boundary underneath the rt engine plugin API
+ * (common-lib/rt_engine_func.h:25-104 of amirych/rt).  The 17 C++ functions
+ * of that header are implemented in librt_engine_hip.so on top of the
+ * entry points below; callers that keep rays resident in HBM (a fused
+ * Scheme::run, the Python tests, bench.py) call these directly.
+ *
+ * Conventions
+ *   - Every function returns an int status that mirrors RT_engine_error
+ *     (common-lib/rt_engine_errors.h:4-5): RT_OK .. RT_ERR_BAD_VALUE.
+ *   - Buffers named *_dev are device (HBM) pointers owned by the caller.
+ *     Pointers without the suffix are host pointers.
+ *   - `stream` is a hipStream_t passed as void*; NULL is the null stream.
+ *   - All floating-point data is FP64.  A ray is alive while flag != 0;
+ *     the engine only ever writes flag = 0 (rt_engine_func.h semantics).
+ *   - No function throws; no function keeps per-call global state, so
+ *     distinct schemes may be traced concurrently on distinct streams.
+ *
+ * Packed scheme table
+ *   A scheme is `n_surf` records of RT_SURF_STRIDE doubles (layout: enum
+ *   rt_surf_field) plus a spline pool.  A spline stored at pool offset o is
+ *   [K, x[0..K-1], y[0..K-1], y2[0..K-1]] (natural cubic spline; y2 from
+ *   the Numerical Recipes recurrence).  A refractive index / QE / spectrum
+ *   reference is a (kind, value) pair: kind RT_FN_CONST -> the constant
+ *   `value`; kind RT_FN_SPLINE -> spline at pool offset `value`.
+ */
+#ifndef RT_HIP_H
+#define RT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes: same values as enum RT_engine_error (rt_engine_errors.h:4-5) */
+enum rt_status {
+    RT_OK = 0,
+    RT_ERR_INIT_FAILED = 1,
+    RT_ERR_BAD_ALLOC = 2,
+    RT_ERR_FAILED = 3,
+    RT_ERR_BAD_VALUE = 4
+};
+
+#define RT_MAX_SURF 32        /* surfaces per scheme */
+#define RT_MAX_POLY 10        /* MAX_POLY_COEFFS, common-lib/surface.h:10 */
+#define RT_MAX_SPLINE_KNOTS 1024
+
+/* Surface::SurfaceClass / SurfaceType / SurfaceShape (common-lib/surface.h:25-27) */
+enum rt_surf_class { RT_PLANE = 0, RT_CONIC = 1, RT_TORIC = 2, RT_AUXCLASS = 3 };
+enum rt_surf_type  { RT_MIRROR = 0, RT_LENS = 1, RT_GRATING = 2, RT_STOP = 3,
+                     RT_SHADE = 4, RT_DETECTOR = 5, RT_AUXTYPE = 6 };
+enum rt_surf_shape { RT_CIRCLE = 0, RT_RECTANGLE = 1 };
+/* Grating::GratingType (common-lib/surface.h:136) */
+enum rt_grating_type { RT_REFLECTIVE = 0, RT_TRANSPARENT = 1 };
+/* order rule of a grating: the reference's single order per grating, or the
+   echelle rule m = clamp(floor(K/lambda + 0.5), m_min, m_max), K = 2 d sin(blaze) */
+enum rt_order_mode { RT_ORDER_FIXED = 0, RT_ORDER_BLAZE = 1 };
+/* (kind, value) function references */
+enum rt_fn_kind { RT_FN_CONST = 0, RT_FN_SPLINE = 1 };
+
+/* one packed surface record, RT_SURF_STRIDE doubles */
+enum rt_surf_field {
+    RT_S_CLASS = 0, RT_S_TYPE = 1, RT_S_SHAPE = 2,
+    RT_S_C = 3,          /* vertex curvature 1/R                          */
+    RT_S_K = 4,          /* conic constant k                              */
+    RT_S_NPOLY = 5,      /* N aspheric coefficients                       */
+    RT_S_A = 6,          /* a1..a10 at RT_S_A + 0..9 (r^2 .. r^20 terms)  */
+    RT_S_CR = 16,        /* toric 1/Rr                                    */
+    RT_S_SIZE0 = 17,     /* circle radius | rectangle width               */
+    RT_S_SIZE1 = 18,     /* rectangle height                              */
+    RT_S_CEN0 = 19, RT_S_CEN1 = 20,
+    RT_S_DIST = 21,      /* 21..23 translation dx, dy, dz                 */
+    RT_S_ANG = 24,       /* 24..26 rotation about Y, X, Z in radians      */
+    RT_S_SIN = 27,       /* 27..29 sin of the three angles (host libm)    */
+    RT_S_COS = 30,       /* 30..32 cos of the three angles (host libm)    */
+    RT_S_GRTYPE = 33, RT_S_ORDER = 34, RT_S_GRCONST = 35,   /* d in um   */
+    RT_S_ORDMODE = 36, RT_S_BLAZEK = 37, RT_S_ORDMIN = 38, RT_S_ORDMAX = 39,
+    RT_S_N1KIND = 40, RT_S_N1VAL = 41, RT_S_N2KIND = 42, RT_S_N2VAL = 43,
+    /* detector frame (used on the surface of type RT_DETECTOR when XDIM > 0) */
+    RT_S_CCD_XDIM = 44, RT_S_CCD_YDIM = 45, RT_S_CCD_XPIX = 46, RT_S_CCD_YPIX = 47,
+    RT_S_CCD_XSIZE = 48, RT_S_CCD_YSIZE = 49,
+    RT_S_QEKIND = 50, RT_S_QEVAL = 51, RT_S_SPKIND = 52, RT_S_SPVAL = 53,
+    RT_SURF_STRIDE = 56
+};
+
+/* beam description, RT_BEAM_STRIDE doubles (Beam::SetParams/Create, beam.cpp:67-97,257-485) */
+enum rt_beam_type { RT_BEAM_PARALLEL = 0, RT_BEAM_CONIC = 1, RT_BEAM_GAUSS = 2 };
+enum rt_range_distr { RT_RANGE_RANDOM = 0, RT_RANGE_UNIFORM = 1, RT_RANGE_MONO = 2 };
+enum rt_beam_field {
+    RT_B_TYPE = 0, RT_B_SHAPE = 1, RT_B_RANGE_DISTR = 2,
+    RT_B_NPER = 3,       /* params[0]: rays (per wavelength for composite beams) */
+    RT_B_P1 = 4,         /* params[1]: inner radius | inner width           */
+    RT_B_P2 = 5,         /* params[2]: outer radius | outer width           */
+    RT_B_P3 = 6,         /* params[3] (rectangle): width/height ratio       */
+    RT_B_ANGPAR = 7,     /* cone angle (deg) or Gaussian FWHM (arcsec)      */
+    RT_B_CEN = 8,        /* 8..10 beam center                               */
+    RT_B_RANGE = 11,     /* 11..12 wavelength range (um)                    */
+    RT_B_NLAMBDA = 13,   /* wavelengths of a composite beam                 */
+    RT_B_COMPOSITE = 14, /* 1: lambda[r] = grid[r mod NLAMBDA]              */
+    RT_BEAM_STRIDE = 16
+};
+
+/* trace flags */
+enum rt_trace_flag {
+    RT_TRACE_GENERATE = 1,   /* generate the beam in-kernel from (beam_desc, seed) */
+    RT_TRACE_STORE = 2,      /* write the final ray state to X..lambda, flag       */
+    RT_TRACE_HIST = 4,       /* accumulate integer hits into ccd_hist_dev          */
+    RT_TRACE_FLUX = 8        /* accumulate photon flux into ccd_flux_dev           */
+};
+
+/* statistics block: RT_STATS_LEN uint64 counters */
+enum rt_stat_field {
+    RT_ST_RAYS = 0,          /* rays traced                                       */
+    RT_ST_GOOD = 1,          /* rays alive after the last surface                 */
+    RT_ST_CCD_HITS = 2,      /* rays counted into the CCD frame                   */
+    RT_ST_CCD_OFFCHIP = 3,   /* alive rays whose pixel falls outside the frame    */
+    RT_ST_CCD_EDGE = 4,      /* counted rays within 1e-9 px of a pixel edge       */
+    RT_ST_SPLINE_OOB = 5,    /* rays killed by a spline argument out of range     */
+    RT_ST_SURF = 16,         /* per surface s: 4 counters at RT_ST_SURF + 4*s:    */
+    /*   +0 rays alive entering the intersection stage (the ray.surface count)     */
+    /*   +1 Newton iterations summed over rays                                    */
+    /*   +2 rays lost at intersection, +3 rays lost at mask or action             */
+    RT_STATS_LEN = RT_ST_SURF + 4 * RT_MAX_SURF
+};
+
+/* ---- device / context ------------------------------------------------- */
+
+/* Select `device` for the calling thread and cache its properties. */
+int rt_hip_init(int device);
+
+/* Human-readable engine/device description, one item per line
+   (replaces rt_engine_info, cuda_engine/rt_cuda_engine_init.cu:14-56). */
+int rt_hip_info(char *buf, size_t len);
+
+/* Text of the last error recorded on the calling thread. */
+int rt_hip_last_error(char *buf, size_t len);
+
+/* ---- scheme ----------------------------------------------------------- */
+
+typedef struct rt_hip_scheme rt_hip_scheme;
+
+/* Validate and upload a packed scheme (host arrays) to device memory. */
+int rt_hip_scheme_create(const double *surf_table, int n_surf,
+                         const double *spline_pool, size_t pool_len,
+                         rt_hip_scheme **out);
+int rt_hip_scheme_destroy(rt_hip_scheme *scheme);
+
+/* ---- fused trace (replaces the per-surface loop of Scheme::run,
+        common-lib/scheme.cpp:983-1046, one launch for all surfaces) ---- */
+
+/*
+ * Trace rays [ray_begin, ray_begin + n_rays) through every surface.
+ * With RT_TRACE_GENERATE the beam is generated in-kernel (Philox4x32-10,
+ * key = seed, counter = global ray index) and X..flag are outputs;
+ * otherwise X..lambda, flag are the input rays and are updated in place
+ * when RT_TRACE_STORE is set.  ccd_hist_dev (uint32, YDIM*XDIM) and
+ * ccd_flux_dev (FP64, YDIM*XDIM) are accumulated, never cleared.
+ * stats_dev (RT_STATS_LEN uint64) is accumulated, never cleared.
+ * dump_dev, when not NULL, receives the state after every surface:
+ * [n_surf][7][n_rays] doubles (X,Y,Z,cX,cY,cZ,lambda) and
+ * dump_flag_dev [n_surf][n_rays].
+ */
+int rt_hip_trace(const rt_hip_scheme *scheme, const double *beam_desc, uint64_t seed,
+                 size_t ray_begin, size_t n_rays,
+                 double *X_dev, double *Y_dev, double *Z_dev,
+                 double *cX_dev, double *cY_dev, double *cZ_dev,
+                 double *lambda_dev, signed char *flag_dev,
+                 int flags,
+                 unsigned int *ccd_hist_dev, double *ccd_flux_dev,
+                 unsigned long long *stats_dev,
+                 double *dump_dev, signed char *dump_flag_dev,
+                 void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RT_HIP_H */

@@ -1,0 +1,537 @@
+// rt_device.h -- per-ray stage arithmetic of the rt tracing path for gfx950.
+//
+// Each function restates one engine kernel of amirych/rt (file:line in its
+// comment) as a register-resident per-ray step, so the fused trace, the
+// per-stage kernels behind rt_engine_func.h and the batched trace share one
+// implementation.  FP64 throughout.  Build decisions D2-D20 are listed in
+// rt_trace.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rt_hip.h"
+
+namespace rt {
+
+constexpr double kPi = 3.14159265359;            // common-lib/num_defs.h:20
+constexpr double kDeg2Rad = kPi / 180.0;         // num_defs.h:31
+constexpr double kPi2Beam = 6.28318531;          // rt_cuda_engine_beam.cu:41
+constexpr double kTwoPi = 6.283185307179586;     // Box-Muller angle scale
+constexpr double kNewtonEps = 1.0e-7;            // rt_cuda_engine_surface.cu:667
+constexpr unsigned kNewtonMaxIter = 100000u;     // rt_cuda_engine_surface.cu:668
+constexpr double kHC = 1.98644560233e-12;        // rt_cuda_engine_ccd.cu:19
+
+struct Ray {
+    double X, Y, Z, cX, cY, cZ, lam;
+};
+
+// ---------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// (0,1] from 64 random bits: (top 53 bits + 1) * 2^-53
+__device__ __forceinline__ double u01(uint32_t hi, uint32_t lo) {
+    uint64_t v = ((((uint64_t)hi) << 32) | lo) >> 11;
+    return (double)(v + 1) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ void philox_stream(uint64_t seed, uint64_t idx, uint32_t stream,
+                                              uint32_t out[4]) {
+    philox4x32_10((uint32_t)idx, (uint32_t)(idx >> 32), stream, 0u,
+                  (uint32_t)seed, (uint32_t)(seed >> 32), out);
+}
+
+// ----------------------------------------------------------- beam generation
+// circle: random_circle_kernel (rt_cuda_engine_beam.cu:126-152);
+// rectangle: uniform over the outer rectangle minus the inner one
+// (beam.cpp:232-244 semantics; the reference kernel :154-163 is a stub).
+__device__ __forceinline__ void gen_position(const double* B, uint64_t seed, uint64_t r, Ray& ray) {
+    uint32_t o[4];
+    philox_stream(seed, r, 0u, o);
+    double u1 = u01(o[0], o[1]);
+    double u2 = u01(o[2], o[3]);
+    if ((int)B[RT_B_SHAPE] == RT_CIRCLE) {
+        double phi = kPi2Beam * u2;
+        double R = (B[RT_B_P2] - B[RT_B_P1]) * sqrt(u1) + B[RT_B_P1];
+        double s, c;
+        sincos(phi, &s, &c);
+        ray.X = R * c + B[RT_B_CEN + 0];
+        ray.Y = R * s + B[RT_B_CEN + 1];
+    } else {
+        double W2 = B[RT_B_P2], H2 = B[RT_B_P2] / B[RT_B_P3];
+        double W1 = B[RT_B_P1], H1 = B[RT_B_P1] / B[RT_B_P3];
+        double x = (u1 - 0.5) * W2, y = (u2 - 0.5) * H2;
+        for (uint32_t k = 1; k < 16 && W1 > 0.0 && fabs(x) < 0.5 * W1 && fabs(y) < 0.5 * H1; ++k) {
+            philox_stream(seed, r, 3u + k, o);
+            x = (u01(o[0], o[1]) - 0.5) * W2;
+            y = (u01(o[2], o[3]) - 0.5) * H2;
+        }
+        ray.X = x + B[RT_B_CEN + 0];
+        ray.Y = y + B[RT_B_CEN + 1];
+    }
+    ray.Z = B[RT_B_CEN + 2];
+}
+
+// Uniform (per-beam) values of the direction generators, evaluated once on
+// the host as the reference does: the cone half-angle sin/cos of
+// conic_cosins_kernel (beam.cu:592-598) and tan(sigma) of gauss_cosins_cycle
+// (beam.cu:1034).
+struct BeamUniform {
+    double cone_cos, cone_sin, gauss_tg;
+};
+
+template <int TYPE>
+__device__ __forceinline__ void gen_direction(const BeamUniform& U, uint64_t seed, uint64_t r, Ray& ray) {
+    if (TYPE == RT_BEAM_PARALLEL) {                    // beam.cpp:363-377
+        ray.cX = 0.0; ray.cY = 0.0; ray.cZ = 1.0;
+    } else if (TYPE == RT_BEAM_CONIC) {                // conic_cosins_kernel, beam.cu:586-611
+        double t = atan2(ray.Y, ray.X);
+        double st, ct;
+        sincos(t, &st, &ct);
+        ray.cZ = U.cone_cos;
+        ray.cX = U.cone_sin * ct;
+        ray.cY = U.cone_sin * st;
+    } else {                                           // gauss_cosins_kernel, beam.cu:952-991
+        const double tg = U.gauss_tg;
+        uint32_t o[4];
+        philox_stream(seed, r, 1u, o);
+        double u3 = u01(o[0], o[1]);
+        double u4 = u01(o[2], o[3]);
+        double rad = sqrt(-2.0 * log(u3));
+        double sb, cb;
+        sincos(kTwoPi * u4, &sb, &cb);
+        double tX = tg * (rad * cb);
+        double tY = tg * (rad * sb);
+        double cX, cY, cZ = 1.0, cs, sn, sig;
+        cs = 1.0 / sqrt(1.0 + tX * tX);
+        sig = signbit(tX) ? -1.0 : 1.0;
+        sn = sig * (1.0 / sqrt(1.0 + 1.0 / tX / tX));
+        cY = -sn * cZ;
+        cZ = cs * cZ;
+        cs = 1.0 / sqrt(1.0 + tY * tY);
+        sig = signbit(tY) ? -1.0 : 1.0;
+        sn = sig * (1.0 / sqrt(1.0 + 1.0 / tY / tY));
+        cX = -sn * cZ;
+        cZ = cs * cZ;
+        ray.cX = cX; ray.cY = cY; ray.cZ = cZ;
+    }
+}
+
+// wavelengths: beam.cpp:425-484; uniform_range_beam_kernel (beam.cu:701-710,
+// global index), random_range_beam_kernel (beam.cu:801-812)
+__device__ __forceinline__ double gen_lambda(const double* B, uint64_t seed, uint64_t r) {
+    int distr = (int)B[RT_B_RANGE_DISTR];
+    double r0 = B[RT_B_RANGE + 0], r1 = B[RT_B_RANGE + 1];
+    uint64_t nper = (uint64_t)B[RT_B_NPER];
+    bool composite = B[RT_B_COMPOSITE] != 0.0;
+    uint64_t nl = composite ? (uint64_t)B[RT_B_NLAMBDA] : nper;
+    uint64_t j = composite ? (r % nl) : r;
+    if (distr == RT_RANGE_MONO) return (r1 + r0) / 2.0;
+    if (distr == RT_RANGE_UNIFORM) {
+        if (nl <= 1) return r0;
+        double d_l = (r1 - r0) / (double)(nl - 1);
+        return r0 + (double)j * d_l;
+    }
+    uint32_t o[4];
+    philox_stream(seed, j, 2u, o);
+    return (r1 - r0) * u01(o[0], o[1]) + r0;
+}
+
+template <int TYPE>
+__device__ __forceinline__ void gen_ray(const double* B, const BeamUniform& U, uint64_t seed, uint64_t r,
+                                        Ray& ray) {
+    gen_position(B, seed, r, ray);
+    gen_direction<TYPE>(U, seed, r, ray);
+    ray.lam = gen_lambda(B, seed, r);
+}
+
+// ------------------------------------------------------------------ splines
+// TabulatedFunction::operator[] (tabulated_function.cpp:173-204) over a
+// spline record [K, x[K], y[K], y2[K]] (normally staged in LDS).
+// err: 1 argument out of range, 2 zero interval.
+__device__ __forceinline__ double spline_eval(const double* s, double xv, int& err) {
+    int K = (int)s[0];
+    const double* x = s + 1;
+    const double* y = x + K;
+    const double* y2 = y + K;
+    if ((xv < x[0]) || (xv > x[K - 1])) { err = 1; return 0.0; }
+    int il = 0, ir = K;
+    while ((ir - il) > 1) {
+        int i = (il + ir) >> 1;
+        double xi = x[i];
+        if (xi == xv) return y[i];
+        if (xi > xv) ir = i; else il = i;
+    }
+    double h = x[ir] - x[il];
+    if (h == 0.0) { err = 2; return 0.0; }
+    double a = (x[ir] - xv) / h;
+    double b = (xv - x[il]) / h;
+    return a * y[il] + b * y[ir] + ((a * a * a - a) * y2[il] + (b * b * b - b) * y2[ir]) * (h * h) / 6.0;
+}
+
+__device__ __forceinline__ double fn_eval(const double* pool, double kind, double val, double xv, int& err) {
+    if (kind == (double)RT_FN_CONST) return val;
+    return spline_eval(pool + (size_t)val, xv, err);
+}
+
+// ------------------------------------------------------------------- stages
+
+// beam_transform (beam.cu:416-579): translation when any d != 0, then the
+// rotations about Y, X, Z that have a non-zero angle.  sin/cos come from the
+// table (evaluated on the host, as beam.cu:520-521 does).
+__device__ __forceinline__ void rot2(double sA, double cA, double& v1, double& v2) {
+    double e = v1 * cA - v2 * sA;
+    v2 = v1 * sA + v2 * cA;
+    v1 = e;
+}
+
+__device__ __forceinline__ void transform_ray(const double* p, Ray& r) {
+    const double d0 = p[RT_S_DIST + 0], d1 = p[RT_S_DIST + 1], d2 = p[RT_S_DIST + 2];
+    if ((d2 != 0.0) || (d0 != 0.0) || (d1 != 0.0)) { r.X -= d0; r.Y -= d1; r.Z -= d2; }
+    if (p[RT_S_ANG + 0] != 0.0) {
+        double s = p[RT_S_SIN + 0], c = p[RT_S_COS + 0];
+        rot2(s, c, r.X, r.Z); rot2(s, c, r.cX, r.cZ);
+    }
+    if (p[RT_S_ANG + 1] != 0.0) {
+        double s = p[RT_S_SIN + 1], c = p[RT_S_COS + 1];
+        rot2(s, c, r.Y, r.Z); rot2(s, c, r.cY, r.cZ);
+    }
+    if (p[RT_S_ANG + 2] != 0.0) {
+        double s = p[RT_S_SIN + 2], c = p[RT_S_COS + 2];
+        rot2(s, c, r.X, r.Y); rot2(s, c, r.cX, r.cY);
+    }
+}
+
+// Beam::Normalize (beam.cpp:541-569): norm scaled by the largest component
+__device__ __forceinline__ void normalize_ray(double& cX, double& cY, double& cZ) {
+    double acX = fabs(cX), acY = fabs(cY), acZ = fabs(cZ);
+    double m, a, b;
+    if (acX >= acY) {
+        if (acX >= acZ) { m = acX; a = acY; b = acZ; } else { m = acZ; a = acX; b = acY; }
+    } else {
+        if (acY >= acZ) { m = acY; a = acX; b = acZ; } else { m = acZ; a = acX; b = acY; }
+    }
+    double norm = m * sqrt(1.0 + a * a / m / m + b * b / m / m);
+    cX /= norm; cY /= norm; cZ /= norm;
+}
+
+// conic_derivs (surface.cu:45-61)
+__device__ __forceinline__ void conic_derivs(const double* p, double X, double Y, double& dFdX, double& dFdY) {
+    const double c = p[RT_S_C], k1 = p[RT_S_K] + 1;
+    const int N = (int)p[RT_S_NPOLY];
+    double r2 = X * X + Y * Y;
+    double E = c / sqrt(1.0 - k1 * c * c * r2);
+    double sum = 0.0, rr = 1.0;
+    for (int i = 1; i <= N; ++i, rr *= r2) sum += (double)i * p[RT_S_A + i - 1] * rr;
+    E += 2.0 * sum;
+    dFdX = -E * X;
+    dFdY = -E * Y;
+}
+
+// toric_derivs (surface.cu:65-94), asphere derivative with its factor Y (D18)
+__device__ __forceinline__ void toric_derivs(const double* p, double X, double Y, double Z,
+                                             double& dFdX, double& dFdY, double& dFdZ) {
+    const double c = p[RT_S_C], k1 = p[RT_S_K] + 1, cR = p[RT_S_CR];
+    const int N = (int)p[RT_S_NPOLY];
+    double Y2 = Y * Y;
+    double Fy = c * Y2 / (1.0 + sqrt(1.0 - k1 * c * c * Y2));
+    double sum = 0.0, rr = Y2;
+    for (int i = 0; i < N; ++i, rr *= Y2) sum += p[RT_S_A + i] * rr;
+    Fy += sum;
+    double dFy = c * Y / sqrt(1.0 - k1 * c * c * Y2);
+    sum = 0.0; rr = 1.0;
+    for (int i = 1; i <= N; ++i, rr *= Y2) sum += (double)i * p[RT_S_A + i - 1] * rr;
+    dFy += 2.0 * sum * Y;
+    dFdX = -cR * X;
+    dFdY = (cR * Fy - 1.0) * dFy;
+    dFdZ = 1.0 - cR * Z;
+}
+
+// Intersection with the surface (engine_intersection_cycle dispatch,
+// surface.cu:593-636).  Returns false for a lost ray; `iters` counts Newton
+// iterations.  A lost Newton ray keeps its last iterate, as the reference
+// writes Xi,Yi,Zi before flagging (surface.cu:319-326).
+__device__ __forceinline__ bool intersect_ray(const double* p, Ray& r, unsigned& iters) {
+    const int cls = (int)p[RT_S_CLASS];
+    const int N = (int)p[RT_S_NPOLY];
+    const double c = p[RT_S_C];
+    iters = 0;
+    if (cls == RT_PLANE || (cls == RT_CONIC && N == 0 && c == 0.0)) {
+        // plane_intersection (surface.cu:154-178)
+        double s0 = -r.Z / r.cZ;
+        if (s0 < 0.0) return false;
+        r.X += r.cX * s0;
+        r.Y += r.cY * s0;
+        r.Z = 0.0;
+        return true;
+    }
+    if (cls == RT_CONIC && N == 0) {
+        // conic_intersection (surface.cu:216-253), stable root (D7), root by sign(c) (D8)
+        const double k1 = p[RT_S_K] + 1;
+        double A = k1 * r.cZ * r.cZ + r.cX * r.cX + r.cY * r.cY;
+        double B = 2.0 * (k1 * r.Z * r.cZ + r.X * r.cX + r.Y * r.cY - r.cZ / c);
+        double C = k1 * r.Z * r.Z + r.X * r.X + r.Y * r.Y - 2.0 * r.Z / c;
+        double D = B * B - 4.0 * A * C;
+        if (D < 0.0) return false;
+        double sq = sqrt(D);
+        double sgnB = (B < 0.0) ? -1.0 : 1.0;
+        double sigma = (c > 0) ? -1.0 : 1.0;
+        double q = -0.5 * (B + sgnB * sq);
+        double s;
+        if (sigma == sgnB) s = (q != 0.0) ? C / q : (-B + sigma * sq) / 2.0 / A;
+        else s = q / A;
+        r.X += s * r.cX;
+        r.Y += s * r.cY;
+        r.Z += s * r.cZ;
+        return true;
+    }
+    if (cls == RT_CONIC) {
+        // iterative_conic_intersection (surface.cu:257-333), counters per ray (D6)
+        const double k1 = p[RT_S_K] + 1;
+        const double X0 = r.X, Y0 = r.Y, cX = r.cX, cY = r.cY, cZ = r.cZ;
+        double s0 = -r.Z / cZ;
+        if (!(s0 >= 0)) return false;
+        bool alive = true;
+        unsigned it = 0;
+        double s1, s2 = 0.0, Xi = 0, Yi = 0, Zi = 0;
+        do {
+            s1 = s2;
+            Xi = X0 + (s1 + s0) * cX;
+            Yi = Y0 + (s1 + s0) * cY;
+            Zi = s1 * cZ;
+            double r2 = Xi * Xi + Yi * Yi;
+            double root = sqrt(1.0 - k1 * c * c * r2);
+            double F = Zi - c * r2 / (1.0 + root);
+            double sum = 0.0, rr = r2;
+            for (int i = 0; i < N; ++i, rr *= r2) sum += p[RT_S_A + i] * rr;
+            F -= sum;
+            double E = c / root;
+            sum = 0.0; rr = 1.0;
+            for (int i = 1; i <= N; ++i, rr *= r2) sum += (double)i * p[RT_S_A + i - 1] * rr;
+            E += 2.0 * sum;
+            double dF = -E * Xi * cX - E * Yi * cY + cZ;
+            if (fabs(dF) < kNewtonEps) { alive = false; break; }
+            s2 = s1 - F / dF;
+            ++it;
+        } while ((fabs(s2 - s1) >= kNewtonEps) && (it <= kNewtonMaxIter));
+        r.X = Xi; r.Y = Yi; r.Z = Zi;
+        iters = it;
+        if (it > kNewtonMaxIter) alive = false;
+        return alive;
+    }
+    if (cls == RT_TORIC) {
+        // iterative_toric_intersection (surface.cu:414-492) with the documented
+        // F of surface.cpp:56-61 (D4) and the asphere derivative factor (D18)
+        const double k1 = p[RT_S_K] + 1, cR = p[RT_S_CR];
+        const double X0 = r.X, Y0 = r.Y, cX = r.cX, cY = r.cY, cZ = r.cZ;
+        double s0 = -r.Z / cZ;
+        if (!(s0 >= 0)) return false;
+        bool alive = true;
+        unsigned it = 0;
+        double s1, s2 = 0.0, Xi = 0, Yi = 0, Zi = 0;
+        do {
+            s1 = s2;
+            Xi = X0 + (s1 + s0) * cX;
+            Yi = Y0 + (s1 + s0) * cY;
+            Zi = s1 * cZ;
+            double Y2 = Yi * Yi;
+            double root = sqrt(1.0 - k1 * c * c * Y2);
+            double Fy = c * Y2 / (1.0 + root);
+            double sum = 0.0, rr = Y2;
+            for (int i = 0; i < N; ++i, rr *= Y2) sum += p[RT_S_A + i] * rr;
+            Fy += sum;
+            double F = Zi - Fy - 0.5 * cR * (Xi * Xi + Zi * Zi - Fy * Fy);
+            double dFy = c * Yi / root;
+            sum = 0.0; rr = 1.0;
+            for (int i = 1; i <= N; ++i, rr *= Y2) sum += (double)i * p[RT_S_A + i - 1] * rr;
+            dFy += 2.0 * sum * Yi;
+            double dF = -cR * Xi * cX + (cR * Fy - 1.0) * dFy * cY + (1.0 - cR * Zi) * cZ;
+            if (fabs(dF) < kNewtonEps) { alive = false; break; }
+            s2 = s1 - F / dF;
+            ++it;
+        } while ((fabs(s2 - s1) >= kNewtonEps) && (it <= kNewtonMaxIter));
+        r.X = Xi; r.Y = Yi; r.Z = Zi;
+        iters = it;
+        if (it > kNewtonMaxIter) alive = false;
+        return alive;
+    }
+    return true;   // AuxClass
+}
+
+// Surface::ApplyConstrains (surface.cpp:264-321), Shade::ApplyConstrains (:652-699)
+__device__ __forceinline__ bool mask_ray(const double* p, double X, double Y) {
+    const int type = (int)p[RT_S_TYPE], shape = (int)p[RT_S_SHAPE];
+    const double s0 = p[RT_S_SIZE0], s1 = p[RT_S_SIZE1], c0 = p[RT_S_CEN0], c1 = p[RT_S_CEN1];
+    double xr = X - c0, yr = Y - c1;
+    if (type == RT_SHADE) {
+        if (shape == RT_CIRCLE) return !(xr * xr + yr * yr <= s0 * s0);
+        return !((xr >= -s0 / 2.0) && (xr <= s0 / 2.0) && (yr >= -s1 / 2.0) && (yr <= s1 / 2.0));
+    }
+    if (shape == RT_CIRCLE) return !(xr * xr + yr * yr > s0 * s0);
+    return !((X < (c0 - s0 / 2.0)) || (X > (c0 + s0 / 2.0)) || (Y < (c1 - s1 / 2.0)) || (Y > (c1 + s1 / 2.0)));
+}
+
+// surface_reflection kernels (surface.cu:771-827)
+__device__ __forceinline__ void reflect_ray(const double* p, Ray& r) {
+    const int cls = (int)p[RT_S_CLASS];
+    if (cls == RT_PLANE) { r.cZ = -r.cZ; return; }
+    if (cls == RT_CONIC) {
+        double dFdX, dFdY;
+        conic_derivs(p, r.X, r.Y, dFdX, dFdY);
+        double a2 = 2.0 * (r.cX * dFdX + r.cY * dFdY + r.cZ) / (dFdX * dFdX + dFdY * dFdY + 1.0);
+        r.cX -= a2 * dFdX;
+        r.cY -= a2 * dFdY;
+        r.cZ -= a2;
+    } else if (cls == RT_TORIC) {
+        double dFdX, dFdY, dFdZ;
+        toric_derivs(p, r.X, r.Y, r.Z, dFdX, dFdY, dFdZ);
+        double a2 = 2.0 * (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ) / (dFdX * dFdX + dFdY * dFdY + dFdZ * dFdZ);
+        r.cX -= a2 * dFdX;
+        r.cY -= a2 * dFdY;
+        r.cZ -= a2 * dFdZ;
+    }
+}
+
+// refraction kernels (surface.cu:956-1066); false on total internal reflection
+__device__ __forceinline__ bool refract_ray(const double* p, Ray& r, double mu) {
+    const int cls = (int)p[RT_S_CLASS];
+    double a, b, a2, G;
+    if (cls == RT_PLANE) {
+        a = mu * r.cZ;
+        b = mu * mu - 1.0;
+        a2 = a * a;
+        if (b > a2) return false;
+        b = a2 - b;
+        G = -a + sqrt(b);
+        r.cX *= mu;
+        r.cY *= mu;
+        r.cZ = mu * r.cZ + G;
+        return true;
+    }
+    double dFdX, dFdY, dFdZ = 1.0;
+    if (cls == RT_CONIC) conic_derivs(p, r.X, r.Y, dFdX, dFdY);
+    else if (cls == RT_TORIC) toric_derivs(p, r.X, r.Y, r.Z, dFdX, dFdY, dFdZ);
+    else return true;
+    G = dFdX * dFdX + dFdY * dFdY + dFdZ * dFdZ;
+    a = mu * (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ) / G;
+    b = (mu * mu - 1.0) / G;
+    a2 = a * a;
+    if (b > a2) return false;
+    b = a2 - b;
+    G = -a + sqrt(b);
+    r.cX = mu * r.cX + G * dFdX;
+    r.cY = mu * r.cY + G * dFdY;
+    r.cZ = mu * r.cZ + G * dFdZ;
+    return true;
+}
+
+// grating kernels (surface.cu:1247-1520); false for an evanescent order
+__device__ __forceinline__ bool diffract_ray(const double* p, Ray& r, double mu, double L) {
+    const int cls = (int)p[RT_S_CLASS];
+    const bool refl = ((int)p[RT_S_GRTYPE] == RT_REFLECTIVE);
+    double a, b, a2, G;
+    if (cls == RT_PLANE) {
+        if (!refl) {                                   // :1247-1282
+            a = mu * r.cZ;
+            b = mu * mu - 1.0 + L * L - 2.0 * mu * L * r.cX;
+            a2 = a * a;
+            if (b > a2) return false;
+            b = a2 - b;
+            G = -a + sqrt(b);
+            r.cX = mu * r.cX - L;
+            r.cY *= mu;
+            r.cZ = mu * r.cZ + G;
+        } else {                                       // :1284-1318
+            a = r.cZ;
+            b = L * L - 2.0 * L * r.cX;
+            a2 = a * a;
+            if (b > a2) return false;
+            b = a2 - b;
+            G = -a - sqrt(b);
+            r.cX -= L;
+            r.cZ += G;
+        }
+        return true;
+    }
+    double dFdX, dFdY, dFdZ = 1.0, u, v, w;
+    if (cls == RT_CONIC) {
+        conic_derivs(p, r.X, r.Y, dFdX, dFdY);
+        G = dFdX / (dFdY * dFdY + 1.0);
+        u = 1.0 / sqrt(1.0 + dFdX * G);
+        v = -dFdY * u * G;
+        w = -u * G;
+    } else if (cls == RT_TORIC) {
+        toric_derivs(p, r.X, r.Y, r.Z, dFdX, dFdY, dFdZ);
+        G = dFdX / (dFdY * dFdY + dFdZ * dFdZ);       // D5
+        u = 1.0 / sqrt(1.0 + dFdX * G);
+        v = -dFdY * u * G;
+        w = -dFdZ * u * G;
+    } else {
+        return true;
+    }
+    G = dFdX * dFdX + dFdY * dFdY + dFdZ * dFdZ;
+    if (!refl) {                                       // :1320-1368, :1423-1470
+        a = mu * (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ) / G;
+        b = (mu * mu - 1.0 + L * L - 2.0 * L * mu * (r.cX * u + r.cY * v + r.cZ * w)) / G;
+        a2 = a * a;
+        if (b > a2) return false;
+        b = a2 - b;
+        G = -a + sqrt(b);
+        r.cX = mu * r.cX - L * u + G * dFdX;
+        r.cY = mu * r.cY - L * v + G * dFdY;
+        r.cZ = mu * r.cZ - L * w + G * dFdZ;
+    } else {                                           // :1370-1421, :1472-1520
+        a = (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ) / G;
+        b = (L * L - 2.0 * L * (r.cX * u + r.cY * v + r.cZ * w)) / G;
+        a2 = a * a;
+        if (b > a2) return false;
+        b = a2 - b;
+        G = -a - sqrt(b);
+        if (cls == RT_CONIC) {
+            r.cX = r.cX - L * u + G * dFdX;
+            r.cY = r.cY - L * v + G * dFdY;
+            r.cZ = r.cZ - L * w + G * dFdZ;
+        } else {
+            r.cX += -L * u + G * dFdX;
+            r.cY += -L * v + G * dFdY;
+            r.cZ += -L * w + G * dFdZ;
+        }
+    }
+    return true;
+}
+
+// grating order of a ray: Order[0] (surface.cpp:570) or the echelle blaze rule
+__device__ __forceinline__ double grating_order(const double* p, double lam) {
+    if ((int)p[RT_S_ORDMODE] == RT_ORDER_FIXED) return p[RT_S_ORDER];
+    double m = floor(p[RT_S_BLAZEK] / lam + 0.5);
+    m = fmax(m, p[RT_S_ORDMIN]);
+    m = fmin(m, p[RT_S_ORDMAX]);
+    return m;
+}
+
+// truncation toward zero to uint32 (ccd_coordinates_kernel, ccd.cu:30-45):
+// negative and NaN -> 0, >= 2^32 -> 0xffffffff
+__device__ __forceinline__ uint32_t trunc_u32(double v) {
+    if (!(v >= 0.0)) return 0u;
+    if (v >= 4294967296.0) return 0xffffffffu;
+    return (uint32_t)v;
+}
+
+__device__ __forceinline__ bool near_edge(double v) {
+    double f = v - floor(v);
+    return (f < 1e-9) || (1.0 - f < 1e-9);
+}
+
+}  // namespace rt

@@ -1,0 +1,126 @@
+"""ctypes binding of librt_hip.so (the C ABI of include/rt_hip.h).
+
+Device memory and streams come from PyTorch-ROCm (plumbing only): buffers are
+torch tensors on a HIP device whose data_ptr() is handed to the engine.  The
+engine library is required: loading fails loudly when it is not built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import scheme as S
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librt_hip.so")
+_LIB = None
+
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load librt_hip.so; raise if it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.rt_hip_init.argtypes = [C.c_int]
+        L.rt_hip_info.argtypes = [C.c_char_p, C.c_size_t]
+        L.rt_hip_last_error.argtypes = [C.c_char_p, C.c_size_t]
+        L.rt_hip_validate_scheme.argtypes = [_dp, C.c_int, _dp, C.c_size_t]
+        L.rt_hip_scheme_create.argtypes = [_dp, C.c_int, _dp, C.c_size_t, C.POINTER(_vp)]
+        L.rt_hip_scheme_destroy.argtypes = [_vp]
+        L.rt_hip_trace.argtypes = ([_vp, _dp, C.c_uint64, C.c_size_t, C.c_size_t] + [_vp] * 8 +
+                                   [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp])
+        _LIB = L
+    return _LIB
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().rt_hip_last_error(buf, len(buf))
+    return buf.value.decode()
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        raise EngineError(f"{what} failed with status {rc}: {last_error()}")
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _tp(t) -> Optional[int]:
+    """device address of a torch tensor (or None)."""
+    return None if t is None else t.data_ptr()
+
+
+def info() -> str:
+    buf = C.create_string_buffer(4096)
+    check(lib().rt_hip_info(buf, len(buf)), "rt_hip_info")
+    return buf.value.decode()
+
+
+def validate(scheme: "S.Scheme") -> int:
+    t = np.ascontiguousarray(scheme.table)
+    p = np.ascontiguousarray(scheme.pool)
+    return lib().rt_hip_validate_scheme(_dptr(t), scheme.n_surf, _dptr(p), len(p))
+
+
+class DeviceScheme:
+    """A scheme uploaded to the current HIP device (rt_hip_scheme)."""
+
+    def __init__(self, scheme: "S.Scheme"):
+        self.scheme = scheme
+        self._t = np.ascontiguousarray(scheme.table, dtype=np.float64)
+        self._p = np.ascontiguousarray(scheme.pool, dtype=np.float64)
+        h = _vp()
+        check(lib().rt_hip_scheme_create(_dptr(self._t), scheme.n_surf, _dptr(self._p), len(self._p),
+                                          C.byref(h)), "rt_hip_scheme_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().rt_hip_scheme_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def trace(self, rays=None, flag=None, *, seed=0, begin=0, n=None, flags=0, beam=None,
+              hist=None, flux=None, stats=None, dump=None, dump_flag=None, stream=None):
+        """Launch rt_hip_trace.  `rays` is a (7, n) float64 CUDA tensor or None;
+        with rays given and RT_TRACE_GENERATE in flags they are outputs."""
+        import torch
+        b = np.ascontiguousarray((beam if beam is not None else self.scheme.beam).pack())
+        if n is None:
+            n = rays.shape[1] if rays is not None else self.scheme.beam.n_rays
+        ptrs = [None] * 7
+        if rays is not None:
+            assert rays.dtype == torch.float64 and rays.is_cuda and rays.is_contiguous()
+            assert rays.shape[0] == 7 and rays.shape[1] >= n
+            ptrs = [rays[k].data_ptr() for k in range(7)]
+        if flag is not None:
+            assert flag.dtype == torch.int8 and flag.is_cuda and flag.numel() >= n
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        rc = lib().rt_hip_trace(self.handle, _dptr(b), seed, begin, n, *ptrs, _tp(flag), flags,
+                                _tp(hist), _tp(flux), _tp(stats), _tp(dump), _tp(dump_flag), stream)
+        check(rc, "rt_hip_trace")
+
+
+def init(device: int = 0):
+    check(lib().rt_hip_init(device), "rt_hip_init")
