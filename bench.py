@@ -35,9 +35,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n-per-lambda", type=int, default=4883)
     ap.add_argument("--nlambda", type=int, default=2048)
-    ap.add_argument("--cpu-sample", type=int, default=200000)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="rays traced by the CPU baseline (0: the whole workload)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(args.copy_calibration * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": x.numel() * 8, "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine is measured on one GPU; --gpus > 1 is not supported "
                                    "(rays shard by range, see DESIGN.md)"}))
@@ -115,7 +126,7 @@ def main():
 
     if not args.no_cpu:
         from oracle import oracle as O
-        m = min(args.cpu_sample, n)
+        m = min(args.cpu_sample, n) if args.cpu_sample > 0 else n
         threads = len(os.sched_getaffinity(0))
         t0 = time.perf_counter()
         r = O.trace(sch, seed=0, n=m, nthreads=threads)
@@ -124,7 +135,7 @@ def main():
         cpu_rs = int(sum(int(cst[S.ST_SURF + 4 * s]) for s in range(sch.n_surf)))
         out["cpu_baseline"] = {"value": cpu_rs / cpu_s, "unit": "ray*surface/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"first {m} rays of the same echelle beam, oracle fused trace"}
+                               "sample": (f"the whole workload ({m} rays)" if m == n else f"first {m} rays of the same beam") + ", oracle fused trace, one pass"}
     print(json.dumps(out))
 
 

@@ -26,6 +26,64 @@ struct Ray {
     double X, Y, Z, cX, cY, cZ, lam;
 };
 
+// Device-side record: the public RT_SURF_STRIDE fields plus per-surface
+// constants derived on the host when the scheme is uploaded.
+enum DevField {
+    D_INV_C = RT_SURF_STRIDE + 0,      // 1/c (0 when c == 0)
+    D_INV_D = RT_SURF_STRIDE + 1,      // 1/grating constant
+    D_HALF_X = RT_SURF_STRIDE + 2,     // CCD xsize/2
+    D_HALF_Y = RT_SURF_STRIDE + 3,     // CCD ysize/2
+    DEV_STRIDE = RT_SURF_STRIDE + 8
+};
+
+// ------------------------------------------------------------- FP64 math
+// The hardware reciprocal / reciprocal square root estimates refined by two
+// Newton-Raphson steps each (quadratic convergence: any estimate better than
+// 2^-14 reaches the FP64 rounding floor), about 1 ulp from the IEEE result
+// at a third of the instructions of the IEEE division sequence.  Zero,
+// infinity and NaN inputs return the IEEE special values; denormal
+// arguments are outside the domain of every caller.
+
+__device__ __forceinline__ bool finite_nonzero(double r) {
+    return __builtin_isfinite(r) && r != 0.0;
+}
+
+__device__ __forceinline__ double rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    double q = fma(r, e, r);
+    e = fma(-x, q, 1.0);
+    q = fma(q, e, q);
+    return finite_nonzero(r) ? q : r;
+}
+
+__device__ __forceinline__ double fdiv(double a, double b) {
+    double r = rcp(b);
+    double q = a * r;
+    double c = fma(fma(-b, q, a), r, q);
+    return __builtin_isfinite(q) ? c : q;
+}
+
+__device__ __forceinline__ double rsqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double h = 0.5 * x;
+    double e = fma(-h * y, y, 0.5);
+    double z = fma(y, e, y);
+    e = fma(-h * z, z, 0.5);
+    z = fma(z, e, z);
+    return finite_nonzero(y) ? z : y;
+}
+
+// sqrt(x) from x * rsqrt(x) with one residual correction
+__device__ __forceinline__ double fsqrt_with(double x, double ry) {
+    double s = x * ry;
+    double t = fma(-s, s, x);
+    s = fma(t, 0.5 * ry, s);
+    return (x == 0.0 || !__builtin_isfinite(x)) ? x + x * 0.0 : s;
+}
+
+__device__ __forceinline__ double fsqrt(double x) { return fsqrt_with(x, rsqrt(x)); }
+
 // ---------------------------------------------------------------- Philox
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -64,7 +122,7 @@ __device__ __forceinline__ void gen_position(const double* B, uint64_t seed, uin
     double u2 = u01(o[2], o[3]);
     if ((int)B[RT_B_SHAPE] == RT_CIRCLE) {
         double phi = kPi2Beam * u2;
-        double R = (B[RT_B_P2] - B[RT_B_P1]) * sqrt(u1) + B[RT_B_P1];
+        double R = (B[RT_B_P2] - B[RT_B_P1]) * fsqrt(u1) + B[RT_B_P1];
         double s, c;
         sincos(phi, &s, &c);
         ray.X = R * c + B[RT_B_CEN + 0];
@@ -109,20 +167,19 @@ __device__ __forceinline__ void gen_direction(const BeamUniform& U, uint64_t see
         philox_stream(seed, r, 1u, o);
         double u3 = u01(o[0], o[1]);
         double u4 = u01(o[2], o[3]);
-        double rad = sqrt(-2.0 * log(u3));
+        double rad = fsqrt(-2.0 * log(u3));
         double sb, cb;
         sincos(kTwoPi * u4, &sb, &cb);
         double tX = tg * (rad * cb);
         double tY = tg * (rad * sb);
-        double cX, cY, cZ = 1.0, cs, sn, sig;
-        cs = 1.0 / sqrt(1.0 + tX * tX);
-        sig = signbit(tX) ? -1.0 : 1.0;
-        sn = sig * (1.0 / sqrt(1.0 + 1.0 / tX / tX));
+        // cos = 1/sqrt(1+t^2); sin = sign(t)/sqrt(1+1/t^2) = t * cos
+        double cX, cY, cZ = 1.0, cs, sn;
+        cs = rsqrt(1.0 + tX * tX);
+        sn = tX * cs;
         cY = -sn * cZ;
         cZ = cs * cZ;
-        cs = 1.0 / sqrt(1.0 + tY * tY);
-        sig = signbit(tY) ? -1.0 : 1.0;
-        sn = sig * (1.0 / sqrt(1.0 + 1.0 / tY / tY));
+        cs = rsqrt(1.0 + tY * tY);
+        sn = tY * cs;
         cX = -sn * cZ;
         cZ = cs * cZ;
         ray.cX = cX; ray.cY = cY; ray.cZ = cZ;
@@ -158,27 +215,36 @@ __device__ __forceinline__ void gen_ray(const double* B, const BeamUniform& U, u
 }
 
 // ------------------------------------------------------------------ splines
-// TabulatedFunction::operator[] (tabulated_function.cpp:173-204) over a
-// spline record [K, x[K], y[K], y2[K]] (normally staged in LDS).
+// TabulatedFunction::operator[] (tabulated_function.cpp:173-204).  Device
+// spline record: [K, x0, (K-1)/(x[K-1]-x0), x[K], y[K], y2[K]] (staged in
+// LDS).  The interval is guessed from the knot density and verified; a
+// wrong guess (non-uniform knots) falls back to the reference's bisection.
+// Any interval with x[i] <= xv <= x[i+1] yields the same value as the
+// bisection, exact knots included (a or b is then exactly 0 and 1).
 // err: 1 argument out of range, 2 zero interval.
 __device__ __forceinline__ double spline_eval(const double* s, double xv, int& err) {
-    int K = (int)s[0];
-    const double* x = s + 1;
+    const int K = (int)s[0];
+    const double* x = s + 3;
     const double* y = x + K;
     const double* y2 = y + K;
     if ((xv < x[0]) || (xv > x[K - 1])) { err = 1; return 0.0; }
-    int il = 0, ir = K;
-    while ((ir - il) > 1) {
-        int i = (il + ir) >> 1;
-        double xi = x[i];
-        if (xi == xv) return y[i];
-        if (xi > xv) ir = i; else il = i;
+    int il = (int)((xv - s[1]) * s[2]);
+    il = il < 0 ? 0 : (il > K - 2 ? K - 2 : il);
+    if (!(x[il] <= xv && xv <= x[il + 1])) {
+        int lo = 0, hi = K;
+        while ((hi - lo) > 1) {
+            int i = (lo + hi) >> 1;
+            if (x[i] > xv) hi = i; else lo = i;
+        }
+        il = lo < K - 1 ? lo : K - 2;
     }
+    const int ir = il + 1;
     double h = x[ir] - x[il];
     if (h == 0.0) { err = 2; return 0.0; }
-    double a = (x[ir] - xv) / h;
-    double b = (xv - x[il]) / h;
-    return a * y[il] + b * y[ir] + ((a * a * a - a) * y2[il] + (b * b * b - b) * y2[ir]) * (h * h) / 6.0;
+    double rh = rcp(h);
+    double a = (x[ir] - xv) * rh;
+    double b = (xv - x[il]) * rh;
+    return a * y[il] + b * y[ir] + ((a * a * a - a) * y2[il] + (b * b * b - b) * y2[ir]) * (h * h) * (1.0 / 6.0);
 }
 
 __device__ __forceinline__ double fn_eval(const double* pool, double kind, double val, double xv, int& err) {
@@ -214,17 +280,13 @@ __device__ __forceinline__ void transform_ray(const double* p, Ray& r) {
     }
 }
 
-// Beam::Normalize (beam.cpp:541-569): norm scaled by the largest component
+// Beam::Normalize (beam.cpp:541-569).  The reference divides by a norm
+// scaled by the largest component (7 divisions and a square root); direction
+// cosines are O(1), so the unscaled norm cannot overflow and one reciprocal
+// square root gives the same vector to about 1 ulp.
 __device__ __forceinline__ void normalize_ray(double& cX, double& cY, double& cZ) {
-    double acX = fabs(cX), acY = fabs(cY), acZ = fabs(cZ);
-    double m, a, b;
-    if (acX >= acY) {
-        if (acX >= acZ) { m = acX; a = acY; b = acZ; } else { m = acZ; a = acX; b = acY; }
-    } else {
-        if (acY >= acZ) { m = acY; a = acX; b = acZ; } else { m = acZ; a = acX; b = acY; }
-    }
-    double norm = m * sqrt(1.0 + a * a / m / m + b * b / m / m);
-    cX /= norm; cY /= norm; cZ /= norm;
+    double inv = rsqrt(cX * cX + cY * cY + cZ * cZ);
+    cX *= inv; cY *= inv; cZ *= inv;
 }
 
 // conic_derivs (surface.cu:45-61)
@@ -232,7 +294,7 @@ __device__ __forceinline__ void conic_derivs(const double* p, double X, double Y
     const double c = p[RT_S_C], k1 = p[RT_S_K] + 1;
     const int N = (int)p[RT_S_NPOLY];
     double r2 = X * X + Y * Y;
-    double E = c / sqrt(1.0 - k1 * c * c * r2);
+    double E = c * rsqrt(1.0 - k1 * c * c * r2);
     double sum = 0.0, rr = 1.0;
     for (int i = 1; i <= N; ++i, rr *= r2) sum += (double)i * p[RT_S_A + i - 1] * rr;
     E += 2.0 * sum;
@@ -246,11 +308,13 @@ __device__ __forceinline__ void toric_derivs(const double* p, double X, double Y
     const double c = p[RT_S_C], k1 = p[RT_S_K] + 1, cR = p[RT_S_CR];
     const int N = (int)p[RT_S_NPOLY];
     double Y2 = Y * Y;
-    double Fy = c * Y2 / (1.0 + sqrt(1.0 - k1 * c * c * Y2));
+    double t = 1.0 - k1 * c * c * Y2;
+    double irt = rsqrt(t);
+    double Fy = fdiv(c * Y2, 1.0 + fsqrt_with(t, irt));
     double sum = 0.0, rr = Y2;
     for (int i = 0; i < N; ++i, rr *= Y2) sum += p[RT_S_A + i] * rr;
     Fy += sum;
-    double dFy = c * Y / sqrt(1.0 - k1 * c * c * Y2);
+    double dFy = c * Y * irt;
     sum = 0.0; rr = 1.0;
     for (int i = 1; i <= N; ++i, rr *= Y2) sum += (double)i * p[RT_S_A + i - 1] * rr;
     dFy += 2.0 * sum * Y;
@@ -270,7 +334,7 @@ __device__ __forceinline__ bool intersect_ray(const double* p, Ray& r, unsigned&
     iters = 0;
     if (cls == RT_PLANE || (cls == RT_CONIC && N == 0 && c == 0.0)) {
         // plane_intersection (surface.cu:154-178)
-        double s0 = -r.Z / r.cZ;
+        double s0 = fdiv(-r.Z, r.cZ);
         if (s0 < 0.0) return false;
         r.X += r.cX * s0;
         r.Y += r.cY * s0;
@@ -281,17 +345,18 @@ __device__ __forceinline__ bool intersect_ray(const double* p, Ray& r, unsigned&
         // conic_intersection (surface.cu:216-253), stable root (D7), root by sign(c) (D8)
         const double k1 = p[RT_S_K] + 1;
         double A = k1 * r.cZ * r.cZ + r.cX * r.cX + r.cY * r.cY;
-        double B = 2.0 * (k1 * r.Z * r.cZ + r.X * r.cX + r.Y * r.cY - r.cZ / c);
-        double C = k1 * r.Z * r.Z + r.X * r.X + r.Y * r.Y - 2.0 * r.Z / c;
+        const double ic = p[D_INV_C];
+        double B = 2.0 * (k1 * r.Z * r.cZ + r.X * r.cX + r.Y * r.cY - r.cZ * ic);
+        double C = k1 * r.Z * r.Z + r.X * r.X + r.Y * r.Y - 2.0 * r.Z * ic;
         double D = B * B - 4.0 * A * C;
         if (D < 0.0) return false;
-        double sq = sqrt(D);
+        double sq = fsqrt(D);
         double sgnB = (B < 0.0) ? -1.0 : 1.0;
         double sigma = (c > 0) ? -1.0 : 1.0;
         double q = -0.5 * (B + sgnB * sq);
         double s;
-        if (sigma == sgnB) s = (q != 0.0) ? C / q : (-B + sigma * sq) / 2.0 / A;
-        else s = q / A;
+        if (sigma == sgnB) s = (q != 0.0) ? fdiv(C, q) : fdiv(-B + sigma * sq, 2.0 * A);
+        else s = fdiv(q, A);
         r.X += s * r.cX;
         r.Y += s * r.cY;
         r.Z += s * r.cZ;
@@ -301,7 +366,7 @@ __device__ __forceinline__ bool intersect_ray(const double* p, Ray& r, unsigned&
         // iterative_conic_intersection (surface.cu:257-333), counters per ray (D6)
         const double k1 = p[RT_S_K] + 1;
         const double X0 = r.X, Y0 = r.Y, cX = r.cX, cY = r.cY, cZ = r.cZ;
-        double s0 = -r.Z / cZ;
+        double s0 = fdiv(-r.Z, cZ);
         if (!(s0 >= 0)) return false;
         bool alive = true;
         unsigned it = 0;
@@ -312,18 +377,19 @@ __device__ __forceinline__ bool intersect_ray(const double* p, Ray& r, unsigned&
             Yi = Y0 + (s1 + s0) * cY;
             Zi = s1 * cZ;
             double r2 = Xi * Xi + Yi * Yi;
-            double root = sqrt(1.0 - k1 * c * c * r2);
-            double F = Zi - c * r2 / (1.0 + root);
+            double t = 1.0 - k1 * c * c * r2;
+            double irt = rsqrt(t);
+            double F = Zi - fdiv(c * r2, 1.0 + fsqrt_with(t, irt));
             double sum = 0.0, rr = r2;
             for (int i = 0; i < N; ++i, rr *= r2) sum += p[RT_S_A + i] * rr;
             F -= sum;
-            double E = c / root;
+            double E = c * irt;
             sum = 0.0; rr = 1.0;
             for (int i = 1; i <= N; ++i, rr *= r2) sum += (double)i * p[RT_S_A + i - 1] * rr;
             E += 2.0 * sum;
             double dF = -E * Xi * cX - E * Yi * cY + cZ;
             if (fabs(dF) < kNewtonEps) { alive = false; break; }
-            s2 = s1 - F / dF;
+            s2 = s1 - fdiv(F, dF);
             ++it;
         } while ((fabs(s2 - s1) >= kNewtonEps) && (it <= kNewtonMaxIter));
         r.X = Xi; r.Y = Yi; r.Z = Zi;
@@ -336,7 +402,7 @@ __device__ __forceinline__ bool intersect_ray(const double* p, Ray& r, unsigned&
         // F of surface.cpp:56-61 (D4) and the asphere derivative factor (D18)
         const double k1 = p[RT_S_K] + 1, cR = p[RT_S_CR];
         const double X0 = r.X, Y0 = r.Y, cX = r.cX, cY = r.cY, cZ = r.cZ;
-        double s0 = -r.Z / cZ;
+        double s0 = fdiv(-r.Z, cZ);
         if (!(s0 >= 0)) return false;
         bool alive = true;
         unsigned it = 0;
@@ -347,19 +413,20 @@ __device__ __forceinline__ bool intersect_ray(const double* p, Ray& r, unsigned&
             Yi = Y0 + (s1 + s0) * cY;
             Zi = s1 * cZ;
             double Y2 = Yi * Yi;
-            double root = sqrt(1.0 - k1 * c * c * Y2);
-            double Fy = c * Y2 / (1.0 + root);
+            double t = 1.0 - k1 * c * c * Y2;
+            double irt = rsqrt(t);
+            double Fy = fdiv(c * Y2, 1.0 + fsqrt_with(t, irt));
             double sum = 0.0, rr = Y2;
             for (int i = 0; i < N; ++i, rr *= Y2) sum += p[RT_S_A + i] * rr;
             Fy += sum;
             double F = Zi - Fy - 0.5 * cR * (Xi * Xi + Zi * Zi - Fy * Fy);
-            double dFy = c * Yi / root;
+            double dFy = c * Yi * irt;
             sum = 0.0; rr = 1.0;
             for (int i = 1; i <= N; ++i, rr *= Y2) sum += (double)i * p[RT_S_A + i - 1] * rr;
             dFy += 2.0 * sum * Yi;
             double dF = -cR * Xi * cX + (cR * Fy - 1.0) * dFy * cY + (1.0 - cR * Zi) * cZ;
             if (fabs(dF) < kNewtonEps) { alive = false; break; }
-            s2 = s1 - F / dF;
+            s2 = s1 - fdiv(F, dF);
             ++it;
         } while ((fabs(s2 - s1) >= kNewtonEps) && (it <= kNewtonMaxIter));
         r.X = Xi; r.Y = Yi; r.Z = Zi;
@@ -390,14 +457,14 @@ __device__ __forceinline__ void reflect_ray(const double* p, Ray& r) {
     if (cls == RT_CONIC) {
         double dFdX, dFdY;
         conic_derivs(p, r.X, r.Y, dFdX, dFdY);
-        double a2 = 2.0 * (r.cX * dFdX + r.cY * dFdY + r.cZ) / (dFdX * dFdX + dFdY * dFdY + 1.0);
+        double a2 = fdiv(2.0 * (r.cX * dFdX + r.cY * dFdY + r.cZ), dFdX * dFdX + dFdY * dFdY + 1.0);
         r.cX -= a2 * dFdX;
         r.cY -= a2 * dFdY;
         r.cZ -= a2;
     } else if (cls == RT_TORIC) {
         double dFdX, dFdY, dFdZ;
         toric_derivs(p, r.X, r.Y, r.Z, dFdX, dFdY, dFdZ);
-        double a2 = 2.0 * (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ) / (dFdX * dFdX + dFdY * dFdY + dFdZ * dFdZ);
+        double a2 = fdiv(2.0 * (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ), dFdX * dFdX + dFdY * dFdY + dFdZ * dFdZ);
         r.cX -= a2 * dFdX;
         r.cY -= a2 * dFdY;
         r.cZ -= a2 * dFdZ;
@@ -414,7 +481,7 @@ __device__ __forceinline__ bool refract_ray(const double* p, Ray& r, double mu) 
         a2 = a * a;
         if (b > a2) return false;
         b = a2 - b;
-        G = -a + sqrt(b);
+        G = -a + fsqrt(b);
         r.cX *= mu;
         r.cY *= mu;
         r.cZ = mu * r.cZ + G;
@@ -425,12 +492,13 @@ __device__ __forceinline__ bool refract_ray(const double* p, Ray& r, double mu) 
     else if (cls == RT_TORIC) toric_derivs(p, r.X, r.Y, r.Z, dFdX, dFdY, dFdZ);
     else return true;
     G = dFdX * dFdX + dFdY * dFdY + dFdZ * dFdZ;
-    a = mu * (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ) / G;
-    b = (mu * mu - 1.0) / G;
+    const double iG = rcp(G);
+    a = mu * (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ) * iG;
+    b = (mu * mu - 1.0) * iG;
     a2 = a * a;
     if (b > a2) return false;
     b = a2 - b;
-    G = -a + sqrt(b);
+    G = -a + fsqrt(b);
     r.cX = mu * r.cX + G * dFdX;
     r.cY = mu * r.cY + G * dFdY;
     r.cZ = mu * r.cZ + G * dFdZ;
@@ -449,7 +517,7 @@ __device__ __forceinline__ bool diffract_ray(const double* p, Ray& r, double mu,
             a2 = a * a;
             if (b > a2) return false;
             b = a2 - b;
-            G = -a + sqrt(b);
+            G = -a + fsqrt(b);
             r.cX = mu * r.cX - L;
             r.cY *= mu;
             r.cZ = mu * r.cZ + G;
@@ -459,7 +527,7 @@ __device__ __forceinline__ bool diffract_ray(const double* p, Ray& r, double mu,
             a2 = a * a;
             if (b > a2) return false;
             b = a2 - b;
-            G = -a - sqrt(b);
+            G = -a - fsqrt(b);
             r.cX -= L;
             r.cZ += G;
         }
@@ -468,37 +536,38 @@ __device__ __forceinline__ bool diffract_ray(const double* p, Ray& r, double mu,
     double dFdX, dFdY, dFdZ = 1.0, u, v, w;
     if (cls == RT_CONIC) {
         conic_derivs(p, r.X, r.Y, dFdX, dFdY);
-        G = dFdX / (dFdY * dFdY + 1.0);
-        u = 1.0 / sqrt(1.0 + dFdX * G);
+        G = fdiv(dFdX, dFdY * dFdY + 1.0);
+        u = rsqrt(1.0 + dFdX * G);
         v = -dFdY * u * G;
         w = -u * G;
     } else if (cls == RT_TORIC) {
         toric_derivs(p, r.X, r.Y, r.Z, dFdX, dFdY, dFdZ);
-        G = dFdX / (dFdY * dFdY + dFdZ * dFdZ);       // D5
-        u = 1.0 / sqrt(1.0 + dFdX * G);
+        G = fdiv(dFdX, dFdY * dFdY + dFdZ * dFdZ);    // D5
+        u = rsqrt(1.0 + dFdX * G);
         v = -dFdY * u * G;
         w = -dFdZ * u * G;
     } else {
         return true;
     }
     G = dFdX * dFdX + dFdY * dFdY + dFdZ * dFdZ;
+    const double iG = rcp(G);
     if (!refl) {                                       // :1320-1368, :1423-1470
-        a = mu * (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ) / G;
-        b = (mu * mu - 1.0 + L * L - 2.0 * L * mu * (r.cX * u + r.cY * v + r.cZ * w)) / G;
+        a = mu * (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ) * iG;
+        b = (mu * mu - 1.0 + L * L - 2.0 * L * mu * (r.cX * u + r.cY * v + r.cZ * w)) * iG;
         a2 = a * a;
         if (b > a2) return false;
         b = a2 - b;
-        G = -a + sqrt(b);
+        G = -a + fsqrt(b);
         r.cX = mu * r.cX - L * u + G * dFdX;
         r.cY = mu * r.cY - L * v + G * dFdY;
         r.cZ = mu * r.cZ - L * w + G * dFdZ;
     } else {                                           // :1370-1421, :1472-1520
-        a = (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ) / G;
-        b = (L * L - 2.0 * L * (r.cX * u + r.cY * v + r.cZ * w)) / G;
+        a = (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ) * iG;
+        b = (L * L - 2.0 * L * (r.cX * u + r.cY * v + r.cZ * w)) * iG;
         a2 = a * a;
         if (b > a2) return false;
         b = a2 - b;
-        G = -a - sqrt(b);
+        G = -a - fsqrt(b);
         if (cls == RT_CONIC) {
             r.cX = r.cX - L * u + G * dFdX;
             r.cY = r.cY - L * v + G * dFdY;

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void rt_trace_kernel(
         unsigned n_oob = 0, n_hit = 0, n_off = 0, n_edge = 0;
 
         for (int s = 0; s < n_surf; ++s) {
-            const double* p = table + (size_t)s * RT_SURF_STRIDE;
+            const double* p = table + (size_t)s * rt::DEV_STRIDE;
             const int type = (int)p[RT_S_TYPE];
             if (alive) {
                 rt::transform_ray(p, r);                                 // scheme.cpp:990
@@ -145,13 +145,14 @@ __global__ __launch_bounds__(kBlock) void rt_trace_kernel(
                                 alive = false;
                                 ++n_oob;
                             } else {
-                                double mu = n1;                          // surface.cu:1095-1096
-                                mu /= n2;
+                                // mu = n1/n2 (surface.cu:1095-1096), L = m lambda / n2 / d (:1568)
+                                const double in2 = rt::rcp(n2);
+                                double mu = n1 * in2;
                                 if (type == RT_LENS) {
                                     alive = rt::refract_ray(p, r, mu);
                                 } else {
                                     double m = rt::grating_order(p, r.lam);
-                                    double Lg = m * r.lam / n2 / p[RT_S_GRCONST];   // surface.cu:1568
+                                    double Lg = m * r.lam * in2 * p[rt::D_INV_D];
                                     alive = rt::diffract_ray(p, r, mu, Lg);
                                 }
                             }
@@ -169,8 +170,8 @@ __global__ __launch_bounds__(kBlock) void rt_trace_kernel(
                 if (type == RT_DETECTOR && p[RT_S_CCD_XDIM] > 0.0 && (do_hist || do_flux) && alive) {
                     // ccd_coordinates_kernel (rt_cuda_engine_ccd.cu:30-45) with a bounds check (D10)
                     const uint32_t xdim = (uint32_t)p[RT_S_CCD_XDIM], ydim = (uint32_t)p[RT_S_CCD_YDIM];
-                    double vx = (r.X + p[RT_S_CCD_XSIZE] / 2.0) / p[RT_S_CCD_XPIX];
-                    double vy = (r.Y + p[RT_S_CCD_YSIZE] / 2.0) / p[RT_S_CCD_YPIX];
+                    double vx = (r.X + p[rt::D_HALF_X]) / p[RT_S_CCD_XPIX];
+                    double vy = (r.Y + p[rt::D_HALF_Y]) / p[RT_S_CCD_YPIX];
                     uint32_t px = rt::trunc_u32(vx), py = rt::trunc_u32(vy);
                     if (px < xdim && py < ydim) {
                         size_t pix = (size_t)py * xdim + px;
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void rt_trace_kernel(
                             } else {
                                 double spec = sp * r.lam;
                                 double pix_area = p[RT_S_CCD_XPIX] * p[RT_S_CCD_YPIX] * 1.0E-2;
-                                double w = spec / rt::kHC * qe * pix_area;
+                                double w = spec * (1.0 / rt::kHC) * qe * pix_area;
                                 unsafeAtomicAdd(&flux[pix], w);
                             }
                         }
@@ -238,11 +239,13 @@ __global__ __launch_bounds__(kBlock) void rt_trace_kernel(
 struct rt_hip_scheme {
     int device;
     int n_surf;
-    size_t pool_len;
-    double* table_dev;
+    size_t pool_len;                 // device pool (doubles)
+    double* table_dev;               // n_surf records of rt::DEV_STRIDE
     double* pool_dev;
-    std::vector<double> table;
-    std::vector<double> pool;
+    std::vector<double> table;       // public packed table, as given
+    std::vector<double> pool;        // public spline pool, as given
+    std::vector<double> dtable;      // device records
+    std::vector<double> dpool;       // device spline records
 };
 
 namespace {
@@ -348,9 +351,53 @@ extern "C" int rt_hip_validate_scheme(const double* t, int n_surf, const double*
                 return set_error(RT_ERR_BAD_VALUE, "surface %d: bad QE/spectrum reference", s);
         }
     }
-    if (pool_len > (size_t)kMaxPoolLds)
-        return set_error(RT_ERR_BAD_VALUE, "spline pool of %zu doubles exceeds %d", pool_len, kMaxPoolLds);
+    if (pool_len > (size_t)kMaxPoolLds - 64)
+        return set_error(RT_ERR_BAD_VALUE, "spline pool of %zu doubles exceeds %d", pool_len, kMaxPoolLds - 64);
     return RT_OK;
+}
+
+// Device tables: each record gets its host-derived constants (rt::DevField)
+// and each referenced spline is re-laid out as [K, x0, (K-1)/(x[K-1]-x0),
+// x, y, y2] with the references patched to the new offsets.
+static void build_device_tables(rt_hip_scheme* s) {
+    const int n = s->n_surf;
+    s->dtable.assign((size_t)n * rt::DEV_STRIDE, 0.0);
+    s->dpool.clear();
+    std::vector<std::pair<size_t, size_t>> moved;   // (public offset, device offset)
+    auto remap = [&](double& kind, double& val) {
+        if (kind != (double)RT_FN_SPLINE) return;
+        size_t o = (size_t)val;
+        for (auto& m : moved)
+            if (m.first == o) { val = (double)m.second; return; }
+        const double* sp = s->pool.data() + o;
+        size_t K = (size_t)sp[0];
+        size_t d = s->dpool.size();
+        s->dpool.push_back((double)K);
+        s->dpool.push_back(sp[1]);
+        s->dpool.push_back((double)(K - 1) / (sp[K] - sp[1]));
+        s->dpool.insert(s->dpool.end(), sp + 1, sp + 1 + 3 * K);
+        moved.push_back({o, d});
+        val = (double)d;
+    };
+    for (int i = 0; i < n; ++i) {
+        double* d = s->dtable.data() + (size_t)i * rt::DEV_STRIDE;
+        const double* p = s->table.data() + (size_t)i * RT_SURF_STRIDE;
+        for (int k = 0; k < RT_SURF_STRIDE; ++k) d[k] = p[k];
+        int type = (int)p[RT_S_TYPE];
+        if (type == RT_LENS || type == RT_GRATING) {
+            remap(d[RT_S_N1KIND], d[RT_S_N1VAL]);
+            remap(d[RT_S_N2KIND], d[RT_S_N2VAL]);
+        }
+        if (type == RT_DETECTOR && p[RT_S_CCD_XDIM] > 0.0) {
+            remap(d[RT_S_QEKIND], d[RT_S_QEVAL]);
+            remap(d[RT_S_SPKIND], d[RT_S_SPVAL]);
+        }
+        d[rt::D_INV_C] = p[RT_S_C] != 0.0 ? 1.0 / p[RT_S_C] : 0.0;
+        d[rt::D_INV_D] = p[RT_S_GRCONST] != 0.0 ? 1.0 / p[RT_S_GRCONST] : 0.0;
+        d[rt::D_HALF_X] = p[RT_S_CCD_XSIZE] / 2.0;
+        d[rt::D_HALF_Y] = p[RT_S_CCD_YSIZE] / 2.0;
+    }
+    s->pool_len = s->dpool.size();
 }
 
 extern "C" int rt_hip_scheme_create(const double* surf_table, int n_surf, const double* spline_pool,
@@ -365,21 +412,22 @@ extern "C" int rt_hip_scheme_create(const double* surf_table, int n_surf, const 
     if (!s) return set_error(RT_ERR_BAD_ALLOC, "out of host memory");
     s->device = g_dev.device;
     s->n_surf = n_surf;
-    s->pool_len = pool_len;
     s->table.assign(surf_table, surf_table + (size_t)n_surf * RT_SURF_STRIDE);
     if (pool_len) s->pool.assign(spline_pool, spline_pool + pool_len);
+    build_device_tables(s);
     s->table_dev = nullptr;
     s->pool_dev = nullptr;
-    size_t tb = s->table.size() * sizeof(double), pb = (pool_len ? pool_len : 1) * sizeof(double);
+    size_t tb = s->dtable.size() * sizeof(double);
+    size_t pb = (s->dpool.empty() ? 1 : s->dpool.size()) * sizeof(double);
     if (hipMalloc(&s->table_dev, tb) != hipSuccess || hipMalloc(&s->pool_dev, pb) != hipSuccess) {
         (void)hipFree(s->table_dev);
         (void)hipFree(s->pool_dev);
         delete s;
         return set_error(RT_ERR_BAD_ALLOC, "hipMalloc of scheme tables failed");
     }
-    if (hipMemcpy(s->table_dev, s->table.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
-        (pool_len && hipMemcpy(s->pool_dev, s->pool.data(), pool_len * sizeof(double),
-                               hipMemcpyHostToDevice) != hipSuccess)) {
+    if (hipMemcpy(s->table_dev, s->dtable.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
+        (!s->dpool.empty() && hipMemcpy(s->pool_dev, s->dpool.data(), s->dpool.size() * sizeof(double),
+                                        hipMemcpyHostToDevice) != hipSuccess)) {
         (void)hipFree(s->table_dev);
         (void)hipFree(s->pool_dev);
         delete s;
