@@ -29,8 +29,10 @@
 #ifndef RT_HIP_H
 #define RT_HIP_H
 
+#ifndef __HIPCC_RTC__
 #include <stddef.h>
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -127,6 +129,8 @@ enum rt_stat_field {
     RT_STATS_LEN = RT_ST_SURF + 4 * RT_MAX_SURF
 };
 
+#ifndef __HIPCC_RTC__   /* the entry points are host API; device code needs only the layouts */
+
 /* ---- device / context ------------------------------------------------- */
 
 /* Select `device` for the calling thread and cache its properties. */
@@ -174,6 +178,8 @@ int rt_hip_trace(const rt_hip_scheme *scheme, const double *beam_desc, uint64_t 
                  unsigned long long *stats_dev,
                  double *dump_dev, signed char *dump_flag_dev,
                  void *stream);
+
+#endif /* __HIPCC_RTC__ */
 
 #ifdef __cplusplus
 }

@@ -7,10 +7,15 @@
 // rt_trace.hip.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#else
+typedef __hip_internal::uint64_t uint64_t;
+typedef __hip_internal::uint32_t uint32_t;
+#endif
 
-#include "../../include/rt_hip.h"
+#include "rt_hip.h"
 
 namespace rt {
 
@@ -26,6 +31,11 @@ struct Ray {
     double X, Y, Z, cX, cY, cZ, lam;
 };
 
+// Read-only kernel data (surface records, beam description) is read through
+// the constant address space, so wave-uniform reads become scalar loads.
+typedef const __attribute__((address_space(4))) double* cptr;
+typedef const __attribute__((address_space(4))) uint64_t* cptr64;
+
 // Device-side record: the public RT_SURF_STRIDE fields plus per-surface
 // constants derived on the host when the scheme is uploaded.
 enum DevField {
@@ -34,6 +44,58 @@ enum DevField {
     D_HALF_X = RT_SURF_STRIDE + 2,     // CCD xsize/2
     D_HALF_Y = RT_SURF_STRIDE + 3,     // CCD ysize/2
     DEV_STRIDE = RT_SURF_STRIDE + 8
+};
+
+// Intersection algorithm of a surface (engine_intersection_cycle dispatch,
+// rt_cuda_engine_surface.cu:593-636)
+enum IntersectKind { IK_PLANE = 0, IK_QUAD = 1, IK_CONIC_NEWTON = 2, IK_TORIC_NEWTON = 3, IK_NONE = 4 };
+
+// Surface descriptors.  Every stage is a template over one of these: DynSurf
+// reads the structure of a surface (class, type, shape, number of aspheric
+// terms ...) from its record at run time; StatSurf carries it as compile-time
+// constants so a kernel generated for one scheme folds every structural
+// branch and unrolls the polynomial loops.  Parameter values are always read
+// from the record.
+struct DynSurf {
+    cptr p;
+    __device__ int cls() const { return (int)p[RT_S_CLASS]; }
+    __device__ int type() const { return (int)p[RT_S_TYPE]; }
+    __device__ int shape() const { return (int)p[RT_S_SHAPE]; }
+    __device__ int npoly() const { return (int)p[RT_S_NPOLY]; }
+    __device__ int grtype() const { return (int)p[RT_S_GRTYPE]; }
+    __device__ int ordmode() const { return (int)p[RT_S_ORDMODE]; }
+    __device__ bool n1spline() const { return p[RT_S_N1KIND] == (double)RT_FN_SPLINE; }
+    __device__ bool n2spline() const { return p[RT_S_N2KIND] == (double)RT_FN_SPLINE; }
+    __device__ bool trans() const {
+        return (p[RT_S_DIST + 2] != 0.0) || (p[RT_S_DIST + 0] != 0.0) || (p[RT_S_DIST + 1] != 0.0);
+    }
+    __device__ bool rot(int j) const { return p[RT_S_ANG + j] != 0.0; }
+    __device__ bool ccd() const { return type() == RT_DETECTOR && p[RT_S_CCD_XDIM] > 0.0; }
+    __device__ int ikind() const {
+        const int c = cls(), n = npoly();
+        if (c == RT_PLANE || (c == RT_CONIC && n == 0 && p[RT_S_C] == 0.0)) return IK_PLANE;
+        if (c == RT_CONIC) return n == 0 ? IK_QUAD : IK_CONIC_NEWTON;
+        if (c == RT_TORIC) return IK_TORIC_NEWTON;
+        return IK_NONE;
+    }
+};
+
+template <int CLS, int TYPE, int SHAPE, int NPOLY, int GRT, int ORDM, int IK, int FNMASK, int ROTMASK,
+          bool TRANS, bool CCD>
+struct StatSurf {
+    cptr p;
+    __device__ static constexpr int cls() { return CLS; }
+    __device__ static constexpr int type() { return TYPE; }
+    __device__ static constexpr int shape() { return SHAPE; }
+    __device__ static constexpr int npoly() { return NPOLY; }
+    __device__ static constexpr int grtype() { return GRT; }
+    __device__ static constexpr int ordmode() { return ORDM; }
+    __device__ static constexpr bool n1spline() { return (FNMASK & 1) != 0; }
+    __device__ static constexpr bool n2spline() { return (FNMASK & 2) != 0; }
+    __device__ static constexpr bool trans() { return TRANS; }
+    __device__ static constexpr bool rot(int j) { return ((ROTMASK >> j) & 1) != 0; }
+    __device__ static constexpr bool ccd() { return CCD; }
+    __device__ static constexpr int ikind() { return IK; }
 };
 
 // ------------------------------------------------------------- FP64 math
@@ -115,7 +177,7 @@ __device__ __forceinline__ void philox_stream(uint64_t seed, uint64_t idx, uint3
 // circle: random_circle_kernel (rt_cuda_engine_beam.cu:126-152);
 // rectangle: uniform over the outer rectangle minus the inner one
 // (beam.cpp:232-244 semantics; the reference kernel :154-163 is a stub).
-__device__ __forceinline__ void gen_position(const double* B, uint64_t seed, uint64_t r, Ray& ray) {
+__device__ __forceinline__ void gen_position(cptr B, uint64_t seed, uint64_t r, Ray& ray) {
     uint32_t o[4];
     philox_stream(seed, r, 0u, o);
     double u1 = u01(o[0], o[1]);
@@ -149,20 +211,21 @@ __device__ __forceinline__ void gen_position(const double* B, uint64_t seed, uin
 struct BeamUniform {
     double cone_cos, cone_sin, gauss_tg;
 };
+enum { BU_CONE_COS = RT_BEAM_STRIDE, BU_CONE_SIN = RT_BEAM_STRIDE + 1, BU_GAUSS_TG = RT_BEAM_STRIDE + 2 };
 
 template <int TYPE>
-__device__ __forceinline__ void gen_direction(const BeamUniform& U, uint64_t seed, uint64_t r, Ray& ray) {
+__device__ __forceinline__ void gen_direction(cptr B, uint64_t seed, uint64_t r, Ray& ray) {
     if (TYPE == RT_BEAM_PARALLEL) {                    // beam.cpp:363-377
         ray.cX = 0.0; ray.cY = 0.0; ray.cZ = 1.0;
     } else if (TYPE == RT_BEAM_CONIC) {                // conic_cosins_kernel, beam.cu:586-611
         double t = atan2(ray.Y, ray.X);
         double st, ct;
         sincos(t, &st, &ct);
-        ray.cZ = U.cone_cos;
-        ray.cX = U.cone_sin * ct;
-        ray.cY = U.cone_sin * st;
+        ray.cZ = B[BU_CONE_COS];
+        ray.cX = B[BU_CONE_SIN] * ct;
+        ray.cY = B[BU_CONE_SIN] * st;
     } else {                                           // gauss_cosins_kernel, beam.cu:952-991
-        const double tg = U.gauss_tg;
+        const double tg = B[BU_GAUSS_TG];
         uint32_t o[4];
         philox_stream(seed, r, 1u, o);
         double u3 = u01(o[0], o[1]);
@@ -188,7 +251,7 @@ __device__ __forceinline__ void gen_direction(const BeamUniform& U, uint64_t see
 
 // wavelengths: beam.cpp:425-484; uniform_range_beam_kernel (beam.cu:701-710,
 // global index), random_range_beam_kernel (beam.cu:801-812)
-__device__ __forceinline__ double gen_lambda(const double* B, uint64_t seed, uint64_t r) {
+__device__ __forceinline__ double gen_lambda(cptr B, uint64_t seed, uint64_t r) {
     int distr = (int)B[RT_B_RANGE_DISTR];
     double r0 = B[RT_B_RANGE + 0], r1 = B[RT_B_RANGE + 1];
     uint64_t nper = (uint64_t)B[RT_B_NPER];
@@ -206,11 +269,11 @@ __device__ __forceinline__ double gen_lambda(const double* B, uint64_t seed, uin
     return (r1 - r0) * u01(o[0], o[1]) + r0;
 }
 
+// B: RT_BEAM_STRIDE beam fields followed by the BeamUniform values
 template <int TYPE>
-__device__ __forceinline__ void gen_ray(const double* B, const BeamUniform& U, uint64_t seed, uint64_t r,
-                                        Ray& ray) {
+__device__ __forceinline__ void gen_ray(cptr B, uint64_t seed, uint64_t r, Ray& ray) {
     gen_position(B, seed, r, ray);
-    gen_direction<TYPE>(U, seed, r, ray);
+    gen_direction<TYPE>(B, seed, r, ray);
     ray.lam = gen_lambda(B, seed, r);
 }
 
@@ -247,8 +310,8 @@ __device__ __forceinline__ double spline_eval(const double* s, double xv, int& e
     return a * y[il] + b * y[ir] + ((a * a * a - a) * y2[il] + (b * b * b - b) * y2[ir]) * (h * h) * (1.0 / 6.0);
 }
 
-__device__ __forceinline__ double fn_eval(const double* pool, double kind, double val, double xv, int& err) {
-    if (kind == (double)RT_FN_CONST) return val;
+__device__ __forceinline__ double fn_eval(const double* pool, bool spline, double val, double xv, int& err) {
+    if (!spline) return val;
     return spline_eval(pool + (size_t)val, xv, err);
 }
 
@@ -263,18 +326,19 @@ __device__ __forceinline__ void rot2(double sA, double cA, double& v1, double& v
     v1 = e;
 }
 
-__device__ __forceinline__ void transform_ray(const double* p, Ray& r) {
-    const double d0 = p[RT_S_DIST + 0], d1 = p[RT_S_DIST + 1], d2 = p[RT_S_DIST + 2];
-    if ((d2 != 0.0) || (d0 != 0.0) || (d1 != 0.0)) { r.X -= d0; r.Y -= d1; r.Z -= d2; }
-    if (p[RT_S_ANG + 0] != 0.0) {
+template <class SD>
+__device__ __forceinline__ void transform_ray(const SD& d, Ray& r) {
+    const cptr p = d.p;
+    if (d.trans()) { r.X -= p[RT_S_DIST + 0]; r.Y -= p[RT_S_DIST + 1]; r.Z -= p[RT_S_DIST + 2]; }
+    if (d.rot(0)) {
         double s = p[RT_S_SIN + 0], c = p[RT_S_COS + 0];
         rot2(s, c, r.X, r.Z); rot2(s, c, r.cX, r.cZ);
     }
-    if (p[RT_S_ANG + 1] != 0.0) {
+    if (d.rot(1)) {
         double s = p[RT_S_SIN + 1], c = p[RT_S_COS + 1];
         rot2(s, c, r.Y, r.Z); rot2(s, c, r.cY, r.cZ);
     }
-    if (p[RT_S_ANG + 2] != 0.0) {
+    if (d.rot(2)) {
         double s = p[RT_S_SIN + 2], c = p[RT_S_COS + 2];
         rot2(s, c, r.X, r.Y); rot2(s, c, r.cX, r.cY);
     }
@@ -290,9 +354,11 @@ __device__ __forceinline__ void normalize_ray(double& cX, double& cY, double& cZ
 }
 
 // conic_derivs (surface.cu:45-61)
-__device__ __forceinline__ void conic_derivs(const double* p, double X, double Y, double& dFdX, double& dFdY) {
+template <class SD>
+__device__ __forceinline__ void conic_derivs(const SD& d, double X, double Y, double& dFdX, double& dFdY) {
+    const cptr p = d.p;
     const double c = p[RT_S_C], k1 = p[RT_S_K] + 1;
-    const int N = (int)p[RT_S_NPOLY];
+    const int N = d.npoly();
     double r2 = X * X + Y * Y;
     double E = c * rsqrt(1.0 - k1 * c * c * r2);
     double sum = 0.0, rr = 1.0;
@@ -303,10 +369,12 @@ __device__ __forceinline__ void conic_derivs(const double* p, double X, double Y
 }
 
 // toric_derivs (surface.cu:65-94), asphere derivative with its factor Y (D18)
-__device__ __forceinline__ void toric_derivs(const double* p, double X, double Y, double Z,
+template <class SD>
+__device__ __forceinline__ void toric_derivs(const SD& d, double X, double Y, double Z,
                                              double& dFdX, double& dFdY, double& dFdZ) {
+    const cptr p = d.p;
     const double c = p[RT_S_C], k1 = p[RT_S_K] + 1, cR = p[RT_S_CR];
-    const int N = (int)p[RT_S_NPOLY];
+    const int N = d.npoly();
     double Y2 = Y * Y;
     double t = 1.0 - k1 * c * c * Y2;
     double irt = rsqrt(t);
@@ -327,12 +395,14 @@ __device__ __forceinline__ void toric_derivs(const double* p, double X, double Y
 // surface.cu:593-636).  Returns false for a lost ray; `iters` counts Newton
 // iterations.  A lost Newton ray keeps its last iterate, as the reference
 // writes Xi,Yi,Zi before flagging (surface.cu:319-326).
-__device__ __forceinline__ bool intersect_ray(const double* p, Ray& r, unsigned& iters) {
-    const int cls = (int)p[RT_S_CLASS];
-    const int N = (int)p[RT_S_NPOLY];
+template <class SD>
+__device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& iters) {
+    const cptr p = d.p;
+    const int ik = d.ikind();
+    const int N = d.npoly();
     const double c = p[RT_S_C];
     iters = 0;
-    if (cls == RT_PLANE || (cls == RT_CONIC && N == 0 && c == 0.0)) {
+    if (ik == IK_PLANE) {
         // plane_intersection (surface.cu:154-178)
         double s0 = fdiv(-r.Z, r.cZ);
         if (s0 < 0.0) return false;
@@ -341,7 +411,7 @@ __device__ __forceinline__ bool intersect_ray(const double* p, Ray& r, unsigned&
         r.Z = 0.0;
         return true;
     }
-    if (cls == RT_CONIC && N == 0) {
+    if (ik == IK_QUAD) {
         // conic_intersection (surface.cu:216-253), stable root (D7), root by sign(c) (D8)
         const double k1 = p[RT_S_K] + 1;
         double A = k1 * r.cZ * r.cZ + r.cX * r.cX + r.cY * r.cY;
@@ -362,7 +432,7 @@ __device__ __forceinline__ bool intersect_ray(const double* p, Ray& r, unsigned&
         r.Z += s * r.cZ;
         return true;
     }
-    if (cls == RT_CONIC) {
+    if (ik == IK_CONIC_NEWTON) {
         // iterative_conic_intersection (surface.cu:257-333), counters per ray (D6)
         const double k1 = p[RT_S_K] + 1;
         const double X0 = r.X, Y0 = r.Y, cX = r.cX, cY = r.cY, cZ = r.cZ;
@@ -397,7 +467,7 @@ __device__ __forceinline__ bool intersect_ray(const double* p, Ray& r, unsigned&
         if (it > kNewtonMaxIter) alive = false;
         return alive;
     }
-    if (cls == RT_TORIC) {
+    if (ik == IK_TORIC_NEWTON) {
         // iterative_toric_intersection (surface.cu:414-492) with the documented
         // F of surface.cpp:56-61 (D4) and the asphere derivative factor (D18)
         const double k1 = p[RT_S_K] + 1, cR = p[RT_S_CR];
@@ -438,8 +508,10 @@ __device__ __forceinline__ bool intersect_ray(const double* p, Ray& r, unsigned&
 }
 
 // Surface::ApplyConstrains (surface.cpp:264-321), Shade::ApplyConstrains (:652-699)
-__device__ __forceinline__ bool mask_ray(const double* p, double X, double Y) {
-    const int type = (int)p[RT_S_TYPE], shape = (int)p[RT_S_SHAPE];
+template <class SD>
+__device__ __forceinline__ bool mask_ray(const SD& d, double X, double Y) {
+    const cptr p = d.p;
+    const int type = d.type(), shape = d.shape();
     const double s0 = p[RT_S_SIZE0], s1 = p[RT_S_SIZE1], c0 = p[RT_S_CEN0], c1 = p[RT_S_CEN1];
     double xr = X - c0, yr = Y - c1;
     if (type == RT_SHADE) {
@@ -451,19 +523,20 @@ __device__ __forceinline__ bool mask_ray(const double* p, double X, double Y) {
 }
 
 // surface_reflection kernels (surface.cu:771-827)
-__device__ __forceinline__ void reflect_ray(const double* p, Ray& r) {
-    const int cls = (int)p[RT_S_CLASS];
+template <class SD>
+__device__ __forceinline__ void reflect_ray(const SD& d, Ray& r) {
+    const int cls = d.cls();
     if (cls == RT_PLANE) { r.cZ = -r.cZ; return; }
     if (cls == RT_CONIC) {
         double dFdX, dFdY;
-        conic_derivs(p, r.X, r.Y, dFdX, dFdY);
+        conic_derivs(d, r.X, r.Y, dFdX, dFdY);
         double a2 = fdiv(2.0 * (r.cX * dFdX + r.cY * dFdY + r.cZ), dFdX * dFdX + dFdY * dFdY + 1.0);
         r.cX -= a2 * dFdX;
         r.cY -= a2 * dFdY;
         r.cZ -= a2;
     } else if (cls == RT_TORIC) {
         double dFdX, dFdY, dFdZ;
-        toric_derivs(p, r.X, r.Y, r.Z, dFdX, dFdY, dFdZ);
+        toric_derivs(d, r.X, r.Y, r.Z, dFdX, dFdY, dFdZ);
         double a2 = fdiv(2.0 * (r.cX * dFdX + r.cY * dFdY + r.cZ * dFdZ), dFdX * dFdX + dFdY * dFdY + dFdZ * dFdZ);
         r.cX -= a2 * dFdX;
         r.cY -= a2 * dFdY;
@@ -472,8 +545,9 @@ __device__ __forceinline__ void reflect_ray(const double* p, Ray& r) {
 }
 
 // refraction kernels (surface.cu:956-1066); false on total internal reflection
-__device__ __forceinline__ bool refract_ray(const double* p, Ray& r, double mu) {
-    const int cls = (int)p[RT_S_CLASS];
+template <class SD>
+__device__ __forceinline__ bool refract_ray(const SD& d, Ray& r, double mu) {
+    const int cls = d.cls();
     double a, b, a2, G;
     if (cls == RT_PLANE) {
         a = mu * r.cZ;
@@ -488,8 +562,8 @@ __device__ __forceinline__ bool refract_ray(const double* p, Ray& r, double mu) 
         return true;
     }
     double dFdX, dFdY, dFdZ = 1.0;
-    if (cls == RT_CONIC) conic_derivs(p, r.X, r.Y, dFdX, dFdY);
-    else if (cls == RT_TORIC) toric_derivs(p, r.X, r.Y, r.Z, dFdX, dFdY, dFdZ);
+    if (cls == RT_CONIC) conic_derivs(d, r.X, r.Y, dFdX, dFdY);
+    else if (cls == RT_TORIC) toric_derivs(d, r.X, r.Y, r.Z, dFdX, dFdY, dFdZ);
     else return true;
     G = dFdX * dFdX + dFdY * dFdY + dFdZ * dFdZ;
     const double iG = rcp(G);
@@ -506,9 +580,10 @@ __device__ __forceinline__ bool refract_ray(const double* p, Ray& r, double mu) 
 }
 
 // grating kernels (surface.cu:1247-1520); false for an evanescent order
-__device__ __forceinline__ bool diffract_ray(const double* p, Ray& r, double mu, double L) {
-    const int cls = (int)p[RT_S_CLASS];
-    const bool refl = ((int)p[RT_S_GRTYPE] == RT_REFLECTIVE);
+template <class SD>
+__device__ __forceinline__ bool diffract_ray(const SD& d, Ray& r, double mu, double L) {
+    const int cls = d.cls();
+    const bool refl = (d.grtype() == RT_REFLECTIVE);
     double a, b, a2, G;
     if (cls == RT_PLANE) {
         if (!refl) {                                   // :1247-1282
@@ -535,13 +610,13 @@ __device__ __forceinline__ bool diffract_ray(const double* p, Ray& r, double mu,
     }
     double dFdX, dFdY, dFdZ = 1.0, u, v, w;
     if (cls == RT_CONIC) {
-        conic_derivs(p, r.X, r.Y, dFdX, dFdY);
+        conic_derivs(d, r.X, r.Y, dFdX, dFdY);
         G = fdiv(dFdX, dFdY * dFdY + 1.0);
         u = rsqrt(1.0 + dFdX * G);
         v = -dFdY * u * G;
         w = -u * G;
     } else if (cls == RT_TORIC) {
-        toric_derivs(p, r.X, r.Y, r.Z, dFdX, dFdY, dFdZ);
+        toric_derivs(d, r.X, r.Y, r.Z, dFdX, dFdY, dFdZ);
         G = fdiv(dFdX, dFdY * dFdY + dFdZ * dFdZ);    // D5
         u = rsqrt(1.0 + dFdX * G);
         v = -dFdY * u * G;
@@ -582,8 +657,10 @@ __device__ __forceinline__ bool diffract_ray(const double* p, Ray& r, double mu,
 }
 
 // grating order of a ray: Order[0] (surface.cpp:570) or the echelle blaze rule
-__device__ __forceinline__ double grating_order(const double* p, double lam) {
-    if ((int)p[RT_S_ORDMODE] == RT_ORDER_FIXED) return p[RT_S_ORDER];
+template <class SD>
+__device__ __forceinline__ double grating_order(const SD& d, double lam) {
+    const cptr p = d.p;
+    if (d.ordmode() == RT_ORDER_FIXED) return p[RT_S_ORDER];
     double m = floor(p[RT_S_BLAZEK] / lam + 0.5);
     m = fmax(m, p[RT_S_ORDMIN]);
     m = fmin(m, p[RT_S_ORDMAX]);
@@ -601,6 +678,222 @@ __device__ __forceinline__ uint32_t trunc_u32(double v) {
 __device__ __forceinline__ bool near_edge(double v) {
     double f = v - floor(v);
     return (f < 1e-9) || (1.0 - f < 1e-9);
+}
+
+// ------------------------------------------------------------ fused trace
+
+constexpr int kBlock = 256;               // 4 waves of 64
+constexpr int kMaxPoolLds = 12288;        // doubles of device spline pool staged in LDS (96 KiB)
+
+struct BeamArg {
+    double v[RT_BEAM_STRIDE];
+    BeamUniform u;
+};
+
+// kernel arguments (one struct, so generated kernels share the launch code)
+struct TraceArgs {
+    const double* table;          // n_surf device records of DEV_STRIDE
+    const double* pool;           // device spline pool
+    int n_surf, pool_len, flags, pad;
+    BeamArg beam;
+    uint64_t seed, ray_begin, n;
+    double *X, *Y, *Z, *cX, *cY, *cZ, *L;
+    signed char* F;
+    unsigned* hist;
+    double* flux;
+    unsigned long long* stats;
+    double* dump;
+    signed char* dump_flag;
+};
+
+struct TraceCtx {
+    cptr table;                   // device records (per-iteration copy of TraceArgs::table)
+    const double* pool;           // LDS copy
+    unsigned* st;                 // per-block LDS counters (RT_STATS_LEN)
+    unsigned* hist;
+    double* flux;
+    bool do_hist, do_flux;
+    unsigned n_oob, n_hit, n_off, n_edge;
+};
+
+__device__ __forceinline__ void lds_add(unsigned* counter, unsigned v) {
+    if (v) atomicAdd(counter, v);
+}
+
+// One surface of the Scheme::run stage loop (common-lib/scheme.cpp:983-1046)
+// for one ray: transform, normalize, intersection, aperture mask, action
+// (reflection, refraction or diffraction), normalize, then the CCD frame on
+// a detector.  Counters are wave-aggregated into LDS.
+template <class SD>
+__device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& alive, TraceCtx& c) {
+    // the record's loads are issued at this surface, not hoisted ahead of
+    // the previous surfaces' work (that would keep every surface's
+    // parameters live in SGPRs at once)
+    SD d = d0;
+    __asm__ volatile("" : "+s"(d.p));
+    const cptr p = d.p;
+    if (alive) {
+        transform_ray(d, r);                                     // scheme.cpp:990
+        normalize_ray(r.cX, r.cY, r.cZ);                         // :993
+    }
+    if (d.type() == RT_AUXTYPE) return;                          // :1000
+    unsigned long long entering = __ballot(alive);
+    if (__lane_id() == 0 && entering) atomicAdd(&c.st[RT_ST_SURF + 4 * s], (unsigned)__popcll(entering));
+    unsigned it = 0;
+    bool lost_int = false, lost_act = false;
+    if (alive) {
+        alive = intersect_ray(d, r, it);                         // :1002
+        lost_int = !alive;
+    }
+    if (alive) {
+        alive = mask_ray(d, r.X, r.Y);                           // :1014
+        if (alive) {                                             // :1035
+            if (d.type() == RT_MIRROR) {
+                reflect_ray(d, r);
+            } else if (d.type() == RT_LENS || d.type() == RT_GRATING) {
+                int e1 = 0, e2 = 0;
+                double n1 = fn_eval(c.pool, d.n1spline(), p[RT_S_N1VAL], r.lam, e1);
+                double n2 = fn_eval(c.pool, d.n2spline(), p[RT_S_N2VAL], r.lam, e2);
+                if (e1 | e2) {
+                    alive = false;
+                    ++c.n_oob;
+                } else {
+                    // mu = n1/n2 (surface.cu:1095-1096), L = m lambda / n2 / d (:1568)
+                    const double in2 = rcp(n2);
+                    double mu = n1 * in2;
+                    if (d.type() == RT_LENS) {
+                        alive = refract_ray(d, r, mu);
+                    } else {
+                        double m = grating_order(d, r.lam);
+                        double Lg = m * r.lam * in2 * p[D_INV_D];
+                        alive = diffract_ray(d, r, mu, Lg);
+                    }
+                }
+            }
+            if (alive) normalize_ray(r.cX, r.cY, r.cZ);          // :1040
+        }
+        lost_act = !alive;
+    }
+    if (d.ikind() == IK_CONIC_NEWTON || d.ikind() == IK_TORIC_NEWTON) lds_add(&c.st[RT_ST_SURF + 4 * s + 1], it);
+    unsigned long long li = __ballot(lost_int), la = __ballot(lost_act);
+    if (__lane_id() == 0) {
+        if (li) atomicAdd(&c.st[RT_ST_SURF + 4 * s + 2], (unsigned)__popcll(li));
+        if (la) atomicAdd(&c.st[RT_ST_SURF + 4 * s + 3], (unsigned)__popcll(la));
+    }
+    if (d.ccd() && (c.do_hist || c.do_flux) && alive) {
+        // ccd_coordinates_kernel (rt_cuda_engine_ccd.cu:30-45) with a bounds check (D10)
+        const uint32_t xdim = (uint32_t)p[RT_S_CCD_XDIM], ydim = (uint32_t)p[RT_S_CCD_YDIM];
+        double vx = (r.X + p[D_HALF_X]) / p[RT_S_CCD_XPIX];
+        double vy = (r.Y + p[D_HALF_Y]) / p[RT_S_CCD_YPIX];
+        uint32_t px = trunc_u32(vx), py = trunc_u32(vy);
+        if (px < xdim && py < ydim) {
+            size_t pix = (size_t)py * xdim + px;
+            ++c.n_hit;
+            if (near_edge(vx) || near_edge(vy)) ++c.n_edge;
+            if (c.do_hist) atomicAdd(&c.hist[pix], 1u);
+            if (c.do_flux) {
+                // compute_ccd_image_kernel (ccd.cu:219-232), weights of ccd.cu:286-287,360
+                int eq = 0, es = 0;
+                double qe = fn_eval(c.pool, p[RT_S_QEKIND] == (double)RT_FN_SPLINE, p[RT_S_QEVAL], r.lam, eq);
+                double sp = fn_eval(c.pool, p[RT_S_SPKIND] == (double)RT_FN_SPLINE, p[RT_S_SPVAL], r.lam, es);
+                if (eq | es) {
+                    ++c.n_oob;
+                } else {
+                    double spec = sp * r.lam;
+                    double pix_area = p[RT_S_CCD_XPIX] * p[RT_S_CCD_YPIX] * 1.0E-2;
+                    double w = spec * (1.0 / kHC) * qe * pix_area;
+                    unsafeAtomicAdd(&c.flux[pix], w);
+                }
+            }
+        } else {
+            ++c.n_off;
+        }
+    }
+}
+
+__device__ __forceinline__ void dump_state(const TraceArgs& a, int s, const Ray& r, bool alive, uint64_t i) {
+    const uint64_t n = a.n;
+    double* d = a.dump + (size_t)s * 7 * n;
+    d[0 * n + i] = r.X; d[1 * n + i] = r.Y; d[2 * n + i] = r.Z;
+    d[3 * n + i] = r.cX; d[4 * n + i] = r.cY; d[5 * n + i] = r.cZ;
+    d[6 * n + i] = r.lam;
+    a.dump_flag[(size_t)s * n + i] = (signed char)alive;
+}
+
+// Kernel body: the spline pool goes to LDS, every lane of a persistent grid
+// walks rays in a grid-stride loop (generated in-kernel when BEAM >= 0,
+// loaded otherwise), `surfaces` applies the scheme, counters are flushed per
+// block.  `surfaces(r, alive, ctx, i)` is the runtime loop over the table or
+// a generated straight-line sequence of StatSurf steps.
+template <int BEAM, class Surfaces>
+__device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces) {
+    extern __shared__ double s_pool[];
+    __shared__ unsigned s_stats[RT_STATS_LEN];
+    for (int i = threadIdx.x; i < RT_STATS_LEN; i += blockDim.x) s_stats[i] = 0u;
+    for (int i = threadIdx.x; i < a.pool_len; i += blockDim.x) s_pool[i] = a.pool[i];
+    __syncthreads();
+
+    TraceCtx c;
+    c.pool = s_pool;
+    c.st = s_stats;
+    c.hist = a.hist;
+    c.flux = a.flux;
+    c.do_hist = (a.flags & RT_TRACE_HIST) && a.hist;
+    c.do_flux = (a.flags & RT_TRACE_FLUX) && a.flux;
+    c.n_oob = c.n_hit = c.n_off = c.n_edge = 0;
+    const bool store = (a.flags & RT_TRACE_STORE) != 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+        // Surface parameters stay per-iteration scalar loads: the table
+        // pointer is passed through an empty asm so the loads cannot be
+        // hoisted out of this loop (hoisted, they outnumber the SGPRs and
+        // spill to VGPR lanes).
+        cptr tbl = (cptr)a.table;
+        __asm__ volatile("" : "+s"(tbl));
+        c.table = tbl;
+        Ray r;
+        bool alive;
+        if constexpr (BEAM >= 0) {
+            // beam fields and seed re-read per ray from the kernel argument
+            // segment (TraceArgs is the kernel's only argument, at offset 0)
+            typedef const __attribute__((address_space(4))) char* cbyte;
+            cbyte karg = (cbyte)__builtin_amdgcn_kernarg_segment_ptr();
+            __asm__ volatile("" : "+s"(karg));
+            cptr bm = (cptr)(karg + __builtin_offsetof(TraceArgs, beam));
+            cptr64 sd = (cptr64)(karg + __builtin_offsetof(TraceArgs, seed));
+            gen_ray<BEAM>(bm, *sd, a.ray_begin + i, r);
+            alive = true;
+        } else {
+            r.X = a.X[i]; r.Y = a.Y[i]; r.Z = a.Z[i];
+            r.cX = a.cX[i]; r.cY = a.cY[i]; r.cZ = a.cZ[i];
+            r.lam = a.L[i];
+            alive = a.F[i] != 0;
+        }
+        surfaces(r, alive, c, i);
+        if (store || BEAM >= 0) {
+            if (a.X) {
+                a.X[i] = r.X; a.Y[i] = r.Y; a.Z[i] = r.Z;
+                a.cX[i] = r.cX; a.cY[i] = r.cY; a.cZ[i] = r.cZ;
+                a.L[i] = r.lam;
+            }
+            if (a.F) a.F[i] = (signed char)alive;
+        }
+        unsigned long long good = __ballot(alive), all = __ballot(true);
+        if (__lane_id() == 0) {
+            atomicAdd(&s_stats[RT_ST_RAYS], (unsigned)__popcll(all));
+            if (good) atomicAdd(&s_stats[RT_ST_GOOD], (unsigned)__popcll(good));
+        }
+    }
+    lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
+    lds_add(&s_stats[RT_ST_CCD_HITS], c.n_hit);
+    lds_add(&s_stats[RT_ST_CCD_OFFCHIP], c.n_off);
+    lds_add(&s_stats[RT_ST_CCD_EDGE], c.n_edge);
+    __syncthreads();
+    if (a.stats) {
+        for (int k = threadIdx.x; k < RT_STATS_LEN; k += blockDim.x)
+            if (s_stats[k]) atomicAdd(&a.stats[k], (unsigned long long)s_stats[k]);
+    }
 }
 
 }  // namespace rt

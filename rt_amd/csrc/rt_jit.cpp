@@ -1,0 +1,178 @@
+// rt_jit.cpp -- generation, compilation (hiprtc) and caching of per-scheme
+// trace kernels.
+//
+// The generic trace kernel branches on every surface's class, type, shape
+// and number of aspheric terms at run time.  For a given scheme that
+// structure is fixed, so the engine writes a kernel whose surface loop is a
+// straight-line sequence of rt::surface_step<StatSurf<...>> calls (see
+// rt_device.h) and compiles it for the device once per structure.
+// Parameter values still come from the device table, so perturbed instances
+// of one scheme share the kernel.
+#include "rt_jit.h"
+
+#include <hip/hiprtc.h>
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <vector>
+
+#include "rt_device.h"
+#include "rt_embed.inc"   // kEmbedDevice, kEmbedAbi: text of rt_device.h and rt_hip.h
+
+namespace {
+
+std::mutex g_mu;
+int g_mode = -1;   // -1: not yet read from the environment
+struct Entry {
+    hipModule_t module = nullptr;
+    hipFunction_t fn = nullptr;
+    bool failed = false;
+    std::string log;
+};
+std::map<std::string, Entry> g_cache;
+
+int surface_ikind(const double* p) {
+    int c = (int)p[RT_S_CLASS], n = (int)p[RT_S_NPOLY];
+    if (c == RT_PLANE || (c == RT_CONIC && n == 0 && p[RT_S_C] == 0.0)) return rt::IK_PLANE;
+    if (c == RT_CONIC) return n == 0 ? rt::IK_QUAD : rt::IK_CONIC_NEWTON;
+    if (c == RT_TORIC) return rt::IK_TORIC_NEWTON;
+    return rt::IK_NONE;
+}
+
+}  // namespace
+
+std::string rt_jit_signature(const double* dtable, int n_surf) {
+    std::ostringstream os;
+    for (int s = 0; s < n_surf; ++s) {
+        const double* p = dtable + (size_t)s * rt::DEV_STRIDE;
+        int type = (int)p[RT_S_TYPE];
+        bool opt = (type == RT_LENS || type == RT_GRATING);
+        int grt = type == RT_GRATING ? (int)p[RT_S_GRTYPE] : 0;
+        int ordm = type == RT_GRATING ? (int)p[RT_S_ORDMODE] : 0;
+        int fnmask = opt ? ((p[RT_S_N1KIND] == (double)RT_FN_SPLINE ? 1 : 0) |
+                            (p[RT_S_N2KIND] == (double)RT_FN_SPLINE ? 2 : 0)) : 0;
+        int rot = (p[RT_S_ANG + 0] != 0.0 ? 1 : 0) | (p[RT_S_ANG + 1] != 0.0 ? 2 : 0) |
+                  (p[RT_S_ANG + 2] != 0.0 ? 4 : 0);
+        bool trans = p[RT_S_DIST + 0] != 0.0 || p[RT_S_DIST + 1] != 0.0 || p[RT_S_DIST + 2] != 0.0;
+        bool ccd = type == RT_DETECTOR && p[RT_S_CCD_XDIM] > 0.0;
+        os << "rt::StatSurf<" << (int)p[RT_S_CLASS] << "," << type << "," << (int)p[RT_S_SHAPE] << ","
+           << (int)p[RT_S_NPOLY] << "," << grt << "," << ordm << "," << surface_ikind(p) << "," << fnmask
+           << "," << rot << "," << (trans ? "true" : "false") << "," << (ccd ? "true" : "false") << ">;";
+    }
+    return os.str();
+}
+
+bool rt_jit_enabled() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_mode < 0) {
+        const char* e = getenv("RT_HIP_JIT");
+        g_mode = (e && e[0] == '0') ? 0 : 1;
+    }
+    return g_mode == 1;
+}
+
+extern "C" int rt_hip_set_jit(int mode) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int prev = g_mode < 0 ? 1 : g_mode;
+    g_mode = mode ? 1 : 0;
+    return prev;
+}
+
+static std::string generate_source(const std::string& sig, int beam_mode) {
+    std::ostringstream os;
+    os << "#include \"rt_device.h\"\n"
+       << "extern \"C\" __global__ __launch_bounds__(rt::kBlock) void rt_trace_jit(rt::TraceArgs a) {\n"
+       << "  rt::trace_body<" << beam_mode << ">(a, [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, uint64_t i) {\n";
+    size_t pos = 0;
+    int s = 0;
+    while (pos < sig.size()) {
+        size_t end = sig.find(';', pos);
+        std::string type = sig.substr(pos, end - pos);
+        os << "    rt::surface_step(" << type << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s
+           << ", r, alive, c);\n"
+           << "    if (a.dump) rt::dump_state(a, " << s << ", r, alive, i);\n";
+        pos = end + 1;
+        ++s;
+    }
+    os << "  });\n}\n";
+    return os.str();
+}
+
+// hiprtc compilation of `src` to a gfx950 code object (no device needed)
+static bool compile_code(const std::string& src, std::vector<char>& code, std::string& log) {
+    hiprtcProgram prog;
+    const char* headers[] = {kEmbedDevice, kEmbedAbi};
+    const char* names[] = {"rt_device.h", "rt_hip.h"};
+    if (hiprtcCreateProgram(&prog, src.c_str(), "rt_trace_jit.hip", 2, headers, names) != HIPRTC_SUCCESS) {
+        log = "hiprtcCreateProgram failed";
+        return false;
+    }
+    std::vector<const char*> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-munsafe-fp-atomics",
+                                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
+    size_t log_size = 0;
+    hiprtcGetProgramLogSize(prog, &log_size);
+    if (log_size > 1) {
+        std::vector<char> l(log_size + 1, 0);
+        hiprtcGetProgramLog(prog, l.data());
+        log = l.data();
+    }
+    if (rc != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return false;
+    }
+    size_t code_size = 0;
+    hiprtcGetCodeSize(prog, &code_size);
+    code.resize(code_size);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return true;
+}
+
+static bool compile(const std::string& src, Entry& e) {
+    std::vector<char> code;
+    if (!compile_code(src, code, e.log)) return false;
+    if (hipModuleLoadData(&e.module, code.data()) != hipSuccess) {
+        e.log += "\nhipModuleLoadData failed";
+        return false;
+    }
+    if (hipModuleGetFunction(&e.fn, e.module, "rt_trace_jit") != hipSuccess) {
+        e.log += "\nhipModuleGetFunction failed";
+        return false;
+    }
+    return true;
+}
+
+hipFunction_t rt_jit_kernel(const std::string& sig, int beam_mode, int device) {
+    std::ostringstream key;
+    key << device << "|" << beam_mode << "|" << sig;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Entry& e = g_cache[key.str()];
+    if (e.fn) return e.fn;
+    if (e.failed) return nullptr;
+    if (!compile(generate_source(sig, beam_mode), e)) {
+        e.failed = true;
+        fprintf(stderr, "rt_hip: generated kernel failed to build, using the generic kernel:\n%s\n",
+                e.log.c_str());
+        return nullptr;
+    }
+    return e.fn;
+}
+
+// Generate and compile (without loading) the kernel of a device table; the
+// source and the compiler log are returned in `text`.  Host only.
+extern "C" int rt_jit_check_dtable(const double* dtable, int n_surf, int beam_mode, char* text, size_t len,
+                                   size_t* code_size) {
+    std::string src = generate_source(rt_jit_signature(dtable, n_surf), beam_mode);
+    std::vector<char> code;
+    std::string log;
+    bool ok = compile_code(src, code, log);
+    if (text && len) snprintf(text, len, "%s\n%s", src.c_str(), log.c_str());
+    if (code_size) *code_size = code.size();
+    return ok ? 0 : RT_ERR_FAILED;
+}

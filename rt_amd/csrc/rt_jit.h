@@ -1,0 +1,19 @@
+// rt_jit.h -- per-scheme kernels generated at run time (hiprtc).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+// Structural key of a scheme: one StatSurf type per device record.
+std::string rt_jit_signature(const double* dtable, int n_surf);
+
+// Whether generated kernels are used (environment RT_HIP_JIT=0 disables;
+// rt_hip_set_jit overrides).
+bool rt_jit_enabled();
+
+// The kernel generated for (signature, beam mode) on `device`, compiled on
+// first use and cached for the life of the process; nullptr when the
+// runtime compiler is unavailable or fails (the caller then launches the
+// generic kernel, which reads the structure from the records).
+hipFunction_t rt_jit_kernel(const std::string& signature, int beam_mode, int device);

@@ -33,8 +33,11 @@
 #include <vector>
 
 #include "rt_device.h"
+#include "rt_jit.h"
 
 using rt::Ray;
+
+static thread_local const char* g_last_kernel = "none";
 
 // --------------------------------------------------------------- errors
 static thread_local std::string g_last_error;
@@ -60,184 +63,29 @@ static int set_error(int code, const char* fmt, ...) {
 
 // ----------------------------------------------------------------- kernel
 
-constexpr int kBlock = 256;               // 4 waves of 64
-constexpr int kMaxPoolLds = 12288;        // doubles of spline pool staged in LDS (96 KiB)
-constexpr int kLdsStats = RT_STATS_LEN;   // per-block counters
-
-struct BeamArg {
-    double v[RT_BEAM_STRIDE];
-    rt::BeamUniform u;
-};
+using rt::kBlock;
+using rt::kMaxPoolLds;
+using rt::TraceArgs;
 
 // BEAM < 0: rays are loaded; otherwise the beam is generated with direction type BEAM
 constexpr int kLoad = -1;
 
-__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
-
-// per-lane count added into an LDS counter (no cross-lane reads, so lanes
-// that already left the grid-stride loop do not matter)
-__device__ __forceinline__ void wave_add(unsigned* counter, unsigned v) {
-    if (v) atomicAdd(counter, v);
-}
-
+// Generic kernel: the structure of every surface is read from its record.
 template <int BEAM>
-__global__ __launch_bounds__(kBlock) void rt_trace_kernel(
-    const double* __restrict__ table, int n_surf, const double* __restrict__ pool_g, int pool_len,
-    BeamArg beam, uint64_t seed, uint64_t ray_begin, uint64_t n,
-    double* __restrict__ X, double* __restrict__ Y, double* __restrict__ Z,
-    double* __restrict__ cX, double* __restrict__ cY, double* __restrict__ cZ,
-    double* __restrict__ L, signed char* __restrict__ F, int flags,
-    unsigned* __restrict__ hist, double* __restrict__ flux,
-    unsigned long long* __restrict__ stats,
-    double* __restrict__ dump, signed char* __restrict__ dump_flag) {
-    extern __shared__ double s_pool[];
-    __shared__ unsigned s_stats[kLdsStats];
-
-    for (int i = threadIdx.x; i < kLdsStats; i += blockDim.x) s_stats[i] = 0u;
-    for (int i = threadIdx.x; i < pool_len; i += blockDim.x) s_pool[i] = pool_g[i];
-    __syncthreads();
-
-    const bool store = (flags & RT_TRACE_STORE) != 0;
-    const bool do_hist = (flags & RT_TRACE_HIST) && hist;
-    const bool do_flux = (flags & RT_TRACE_FLUX) && flux;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        Ray r;
-        bool alive;
-        if (BEAM >= 0) {
-            rt::gen_ray<BEAM>(beam.v, beam.u, seed, ray_begin + i, r);
-            alive = true;
-        } else {
-            r.X = X[i]; r.Y = Y[i]; r.Z = Z[i];
-            r.cX = cX[i]; r.cY = cY[i]; r.cZ = cZ[i];
-            r.lam = L[i];
-            alive = F[i] != 0;
+__global__ __launch_bounds__(kBlock) void rt_trace_kernel(TraceArgs a) {
+    rt::trace_body<BEAM>(a, [&](Ray& r, bool& alive, rt::TraceCtx& c, uint64_t i) {
+        for (int s = 0; s < a.n_surf; ++s) {
+            rt::surface_step(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, s, r, alive, c);
+            if (a.dump) rt::dump_state(a, s, r, alive, i);
         }
-        unsigned n_oob = 0, n_hit = 0, n_off = 0, n_edge = 0;
-
-        for (int s = 0; s < n_surf; ++s) {
-            const double* p = table + (size_t)s * rt::DEV_STRIDE;
-            const int type = (int)p[RT_S_TYPE];
-            if (alive) {
-                rt::transform_ray(p, r);                                 // scheme.cpp:990
-                rt::normalize_ray(r.cX, r.cY, r.cZ);                     // :993
-            }
-            if (type != RT_AUXTYPE) {
-                unsigned long long entering = __ballot(alive);
-                if (lane_id() == 0 && entering) atomicAdd(&s_stats[RT_ST_SURF + 4 * s], (unsigned)__popcll(entering));
-                unsigned it = 0;
-                bool lost_int = false, lost_act = false;
-                if (alive) {
-                    alive = rt::intersect_ray(p, r, it);                 // :1002
-                    lost_int = !alive;
-                }
-                if (alive) {
-                    alive = rt::mask_ray(p, r.X, r.Y);                   // :1014
-                    if (alive) {                                         // :1035
-                        if (type == RT_MIRROR) {
-                            rt::reflect_ray(p, r);
-                        } else if (type == RT_LENS || type == RT_GRATING) {
-                            int e1 = 0, e2 = 0;
-                            double n1 = rt::fn_eval(s_pool, p[RT_S_N1KIND], p[RT_S_N1VAL], r.lam, e1);
-                            double n2 = rt::fn_eval(s_pool, p[RT_S_N2KIND], p[RT_S_N2VAL], r.lam, e2);
-                            if (e1 | e2) {
-                                alive = false;
-                                ++n_oob;
-                            } else {
-                                // mu = n1/n2 (surface.cu:1095-1096), L = m lambda / n2 / d (:1568)
-                                const double in2 = rt::rcp(n2);
-                                double mu = n1 * in2;
-                                if (type == RT_LENS) {
-                                    alive = rt::refract_ray(p, r, mu);
-                                } else {
-                                    double m = rt::grating_order(p, r.lam);
-                                    double Lg = m * r.lam * in2 * p[rt::D_INV_D];
-                                    alive = rt::diffract_ray(p, r, mu, Lg);
-                                }
-                            }
-                        }
-                        if (alive) rt::normalize_ray(r.cX, r.cY, r.cZ);  // :1040
-                    }
-                    lost_act = !alive;
-                }
-                if (__ballot(it != 0)) wave_add(&s_stats[RT_ST_SURF + 4 * s + 1], it);
-                unsigned long long li = __ballot(lost_int), la = __ballot(lost_act);
-                if (lane_id() == 0) {
-                    if (li) atomicAdd(&s_stats[RT_ST_SURF + 4 * s + 2], (unsigned)__popcll(li));
-                    if (la) atomicAdd(&s_stats[RT_ST_SURF + 4 * s + 3], (unsigned)__popcll(la));
-                }
-                if (type == RT_DETECTOR && p[RT_S_CCD_XDIM] > 0.0 && (do_hist || do_flux) && alive) {
-                    // ccd_coordinates_kernel (rt_cuda_engine_ccd.cu:30-45) with a bounds check (D10)
-                    const uint32_t xdim = (uint32_t)p[RT_S_CCD_XDIM], ydim = (uint32_t)p[RT_S_CCD_YDIM];
-                    double vx = (r.X + p[rt::D_HALF_X]) / p[RT_S_CCD_XPIX];
-                    double vy = (r.Y + p[rt::D_HALF_Y]) / p[RT_S_CCD_YPIX];
-                    uint32_t px = rt::trunc_u32(vx), py = rt::trunc_u32(vy);
-                    if (px < xdim && py < ydim) {
-                        size_t pix = (size_t)py * xdim + px;
-                        ++n_hit;
-                        if (rt::near_edge(vx) || rt::near_edge(vy)) ++n_edge;
-                        if (do_hist) atomicAdd(&hist[pix], 1u);
-                        if (do_flux) {
-                            // compute_ccd_image_kernel (ccd.cu:219-232), weights of ccd.cu:286-287,360
-                            int eq = 0, es = 0;
-                            double qe = rt::fn_eval(s_pool, p[RT_S_QEKIND], p[RT_S_QEVAL], r.lam, eq);
-                            double sp = rt::fn_eval(s_pool, p[RT_S_SPKIND], p[RT_S_SPVAL], r.lam, es);
-                            if (eq | es) {
-                                ++n_oob;
-                            } else {
-                                double spec = sp * r.lam;
-                                double pix_area = p[RT_S_CCD_XPIX] * p[RT_S_CCD_YPIX] * 1.0E-2;
-                                double w = spec * (1.0 / rt::kHC) * qe * pix_area;
-                                unsafeAtomicAdd(&flux[pix], w);
-                            }
-                        }
-                    } else {
-                        ++n_off;
-                    }
-                }
-            }
-            if (dump) {
-                double* d = dump + (size_t)s * 7 * n;
-                d[0 * n + i] = r.X; d[1 * n + i] = r.Y; d[2 * n + i] = r.Z;
-                d[3 * n + i] = r.cX; d[4 * n + i] = r.cY; d[5 * n + i] = r.cZ;
-                d[6 * n + i] = r.lam;
-                dump_flag[(size_t)s * n + i] = (signed char)alive;
-            }
-        }
-
-        if (store || BEAM >= 0) {
-            if (X) {
-                X[i] = r.X; Y[i] = r.Y; Z[i] = r.Z;
-                cX[i] = r.cX; cY[i] = r.cY; cZ[i] = r.cZ;
-                L[i] = r.lam;
-            }
-            if (F) F[i] = (signed char)alive;
-        }
-        unsigned long long good = __ballot(alive), all = __ballot(true);
-        if (lane_id() == 0) {
-            atomicAdd(&s_stats[RT_ST_RAYS], (unsigned)__popcll(all));
-            if (good) atomicAdd(&s_stats[RT_ST_GOOD], (unsigned)__popcll(good));
-        }
-        if (__ballot(n_oob | n_hit | n_off | n_edge)) {
-            wave_add(&s_stats[RT_ST_SPLINE_OOB], n_oob);
-            wave_add(&s_stats[RT_ST_CCD_HITS], n_hit);
-            wave_add(&s_stats[RT_ST_CCD_OFFCHIP], n_off);
-            wave_add(&s_stats[RT_ST_CCD_EDGE], n_edge);
-        }
-    }
-
-    __syncthreads();
-    if (stats) {
-        for (int k = threadIdx.x; k < kLdsStats; k += blockDim.x)
-            if (s_stats[k]) atomicAdd(&stats[k], (unsigned long long)s_stats[k]);
-    }
+    });
 }
 
 // ------------------------------------------------------------------- C ABI
 
 struct rt_hip_scheme {
     int device;
+    std::string signature;           // structural key of the generated kernels
     int n_surf;
     size_t pool_len;                 // device pool (doubles)
     double* table_dev;               // n_surf records of rt::DEV_STRIDE
@@ -299,6 +147,12 @@ extern "C" int rt_hip_info(char* buf, size_t len) {
                      "Global memory: %zu\nNumber of compute units: %d\nClock (kHz): %d\n",
                      p.name, p.gcnArchName, (size_t)p.totalGlobalMem, p.multiProcessorCount, p.clockRate);
     if (n < 0 || (size_t)n >= len) return set_error(RT_ERR_BAD_ALLOC, "info buffer too small");
+    return RT_OK;
+}
+
+extern "C" int rt_hip_last_kernel(char* buf, size_t len) {
+    if (!buf || !len) return RT_ERR_BAD_VALUE;
+    snprintf(buf, len, "%s", g_last_kernel);
     return RT_OK;
 }
 
@@ -398,6 +252,7 @@ static void build_device_tables(rt_hip_scheme* s) {
         d[rt::D_HALF_Y] = p[RT_S_CCD_YSIZE] / 2.0;
     }
     s->pool_len = s->dpool.size();
+    s->signature = rt_jit_signature(s->dtable.data(), n);
 }
 
 extern "C" int rt_hip_scheme_create(const double* surf_table, int n_surf, const double* spline_pool,
@@ -435,6 +290,22 @@ extern "C" int rt_hip_scheme_create(const double* surf_table, int n_surf, const 
     }
     *out = s;
     return RT_OK;
+}
+
+extern "C" int rt_jit_check_dtable(const double* dtable, int n_surf, int beam_mode, char* text, size_t len,
+                                   size_t* code_size);
+
+extern "C" int rt_hip_jit_check(const double* surf_table, int n_surf, const double* spline_pool,
+                                size_t pool_len, int beam_mode, char* text, size_t len, size_t* code_size) {
+    int rc = rt_hip_validate_scheme(surf_table, n_surf, spline_pool, pool_len);
+    if (rc != RT_OK) return rc;
+    rt_hip_scheme tmp;
+    tmp.n_surf = n_surf;
+    tmp.table.assign(surf_table, surf_table + (size_t)n_surf * RT_SURF_STRIDE);
+    if (pool_len) tmp.pool.assign(spline_pool, spline_pool + pool_len);
+    build_device_tables(&tmp);
+    rc = rt_jit_check_dtable(tmp.dtable.data(), n_surf, beam_mode, text, len, code_size);
+    return rc == RT_OK ? RT_OK : set_error(rc, "generated kernel did not compile");
 }
 
 extern "C" int rt_hip_scheme_destroy(rt_hip_scheme* s) {
@@ -486,33 +357,51 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
     int rc = ensure_device();
     if (rc != RT_OK) return rc;
 
-    BeamArg b;
-    memset(&b, 0, sizeof b);
-    if (beam_desc) memcpy(b.v, beam_desc, sizeof b.v);
+    TraceArgs a;
+    memset(&a, 0, sizeof a);
+    if (beam_desc) memcpy(a.beam.v, beam_desc, sizeof a.beam.v);
+    if (gen) {
+        // uniform values of the direction generators, on the host (beam.cu:592-598, 1034, 1099)
+        double ang = a.beam.v[RT_B_ANGPAR] * rt::kDeg2Rad;
+        a.beam.u.cone_cos = cos(ang / 2.0);
+        a.beam.u.cone_sin = sin(ang / 2.0);
+        double stddev = a.beam.v[RT_B_ANGPAR] / 2.35482 * rt::kPi / 180 / 3600;
+        a.beam.u.gauss_tg = tan(stddev);
+    }
+    a.table = sch->table_dev;
+    a.pool = sch->pool_dev;
+    a.n_surf = sch->n_surf;
+    a.pool_len = (int)sch->pool_len;
+    a.flags = flags;
+    a.seed = seed;
+    a.ray_begin = ray_begin;
+    a.n = n_rays;
+    a.X = X; a.Y = Y; a.Z = Z; a.cX = cX; a.cY = cY; a.cZ = cZ; a.L = lambda; a.F = flag;
+    a.hist = ccd_hist; a.flux = ccd_flux; a.stats = stats; a.dump = dump; a.dump_flag = dump_flag;
+
     const int cus = g_dev.cus > 0 ? g_dev.cus : 256;
     uint64_t want = (n_rays + kBlock - 1) / kBlock;
     uint64_t cap = (uint64_t)cus * 8;      // persistent grid: 8 blocks (32 waves) per CU
     unsigned grid = (unsigned)(want < cap ? want : cap);
     size_t lds = sch->pool_len * sizeof(double);
     hipStream_t st = (hipStream_t)stream;
-    if (gen) {
-        // uniform values of the direction generators, on the host (beam.cu:592-598, 1034, 1099)
-        double ang = b.v[RT_B_ANGPAR] * rt::kDeg2Rad;
-        b.u.cone_cos = cos(ang / 2.0);
-        b.u.cone_sin = sin(ang / 2.0);
-        double stddev = b.v[RT_B_ANGPAR] / 2.35482 * rt::kPi / 180 / 3600;
-        b.u.gauss_tg = tan(stddev);
+    const int beam_mode = gen ? (int)a.beam.v[RT_B_TYPE] : kLoad;
+
+    hipFunction_t fn = nullptr;
+    if (rt_jit_enabled()) fn = rt_jit_kernel(sch->signature, beam_mode, sch->device);
+    if (fn) {
+        void* params[] = {&a};
+        HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, kBlock, 1, 1, (unsigned)lds, st, params, nullptr),
+                RT_ERR_FAILED);
+        g_last_kernel = "generated";
+    } else {
+        auto kern = beam_mode == RT_BEAM_PARALLEL ? rt_trace_kernel<RT_BEAM_PARALLEL>
+                  : beam_mode == RT_BEAM_CONIC    ? rt_trace_kernel<RT_BEAM_CONIC>
+                  : beam_mode == RT_BEAM_GAUSS    ? rt_trace_kernel<RT_BEAM_GAUSS>
+                                                  : rt_trace_kernel<kLoad>;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, a);
+        g_last_kernel = "generic";
     }
-    auto kern = rt_trace_kernel<kLoad>;
-    if (gen) {
-        int type = (int)b.v[RT_B_TYPE];
-        kern = type == RT_BEAM_PARALLEL ? rt_trace_kernel<RT_BEAM_PARALLEL>
-             : type == RT_BEAM_CONIC    ? rt_trace_kernel<RT_BEAM_CONIC>
-                                        : rt_trace_kernel<RT_BEAM_GAUSS>;
-    }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, sch->table_dev, sch->n_surf, sch->pool_dev,
-                       (int)sch->pool_len, b, seed, (uint64_t)ray_begin, (uint64_t)n_rays, X, Y, Z, cX, cY,
-                       cZ, lambda, flag, flags, ccd_hist, ccd_flux, stats, dump, dump_flag);
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     return RT_OK;
 }

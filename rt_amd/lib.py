@@ -41,6 +41,10 @@ def lib():
         L.rt_hip_scheme_destroy.argtypes = [_vp]
         L.rt_hip_trace.argtypes = ([_vp, _dp, C.c_uint64, C.c_size_t, C.c_size_t] + [_vp] * 8 +
                                    [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp])
+        L.rt_hip_set_jit.argtypes = [C.c_int]
+        L.rt_hip_last_kernel.argtypes = [C.c_char_p, C.c_size_t]
+        L.rt_hip_jit_check.argtypes = [_dp, C.c_int, _dp, C.c_size_t, C.c_int, C.c_char_p, C.c_size_t,
+                                       C.POINTER(C.c_size_t)]
         _LIB = L
     return _LIB
 
@@ -120,6 +124,29 @@ class DeviceScheme:
         rc = lib().rt_hip_trace(self.handle, _dptr(b), seed, begin, n, *ptrs, _tp(flag), flags,
                                 _tp(hist), _tp(flux), _tp(stats), _tp(dump), _tp(dump_flag), stream)
         check(rc, "rt_hip_trace")
+
+
+def set_jit(on: bool) -> bool:
+    """Use kernels generated per scheme structure (True) or the generic kernel."""
+    return bool(lib().rt_hip_set_jit(1 if on else 0))
+
+
+def last_kernel() -> str:
+    buf = C.create_string_buffer(64)
+    lib().rt_hip_last_kernel(buf, len(buf))
+    return buf.value.decode()
+
+
+def jit_check(scheme: "S.Scheme", beam_mode: int):
+    """Generate and compile (without a device) the kernel of `scheme`;
+    returns (status, code bytes, source + compiler log)."""
+    t = np.ascontiguousarray(scheme.table)
+    p = np.ascontiguousarray(scheme.pool)
+    buf = C.create_string_buffer(1 << 17)
+    size = C.c_size_t(0)
+    rc = lib().rt_hip_jit_check(_dptr(t), scheme.n_surf, _dptr(p), len(p), beam_mode, buf, len(buf),
+                                C.byref(size))
+    return rc, size.value, buf.value.decode()
 
 
 def init(device: int = 0):

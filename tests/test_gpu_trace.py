@@ -40,7 +40,15 @@ def rel_err(a, b, mask, L=1.0):
     return float(np.max((np.abs(a - b) / scale)[..., mask]))
 
 
-def gpu_trace(hip, sch, seed=0, dump=True, flux=False, rays=None, flag=None):
+@pytest.fixture(params=["generated", "generic"])
+def kernel(request, hip):
+    """run each parity test through the per-scheme generated kernel and the generic one"""
+    prev = hip.set_jit(request.param == "generated")
+    yield request.param
+    hip.set_jit(prev)
+
+
+def gpu_trace(hip, sch, seed=0, dump=True, flux=False, rays=None, flag=None, expect_kernel=None):
     n = sch.beam.n_rays if rays is None else rays.shape[1]
     ds = hip.DeviceScheme(sch)
     det = sch.detector()
@@ -63,6 +71,8 @@ def gpu_trace(hip, sch, seed=0, dump=True, flux=False, rays=None, flag=None):
     print(f"[gpu trace] {sch.name} n={n} seed={seed}", flush=True)
     ds.trace(R, F, seed=seed, flags=flags, hist=hist, flux=fl, stats=stats, dump=D, dump_flag=DF)
     torch.cuda.synchronize()
+    if expect_kernel:
+        assert hip.last_kernel() == expect_kernel, f"ran the {hip.last_kernel()} kernel"
     out = dict(rays=R.cpu().numpy(), flag=F.cpu().numpy(), hist=hist.cpu().numpy().view(np.uint32),
                stats=stats.cpu().numpy().view(np.uint64))
     if dump:
@@ -93,18 +103,18 @@ def compare(sch, g, ref):
 
 
 @pytest.mark.parametrize("seed", [0, 7])
-def test_czerny_turner_parity(hip, seed):
+def test_czerny_turner_parity(hip, kernel, seed):
     sch = schemes.czerny_turner(n_per_lambda=4000, nlambda=5)
-    g = gpu_trace(hip, sch, seed=seed)
+    g = gpu_trace(hip, sch, seed=seed, expect_kernel=kernel)
     ref = O.trace(sch, seed=seed, hist=True, dump=True)
     compare(sch, g, ref)
     assert ref.stats[S.ST_CCD_HITS] > 0.9 * sch.beam.n_rays
 
 
 @pytest.mark.parametrize("seed", [0, 11])
-def test_echelle_parity(hip, seed):
+def test_echelle_parity(hip, kernel, seed):
     sch = schemes.echelle(n_per_lambda=8, nlambda=2048)
-    g = gpu_trace(hip, sch, seed=seed, flux=True)
+    g = gpu_trace(hip, sch, seed=seed, flux=True, expect_kernel=kernel)
     ref = O.trace(sch, seed=seed, hist=True, flux=True, dump=True)
     compare(sch, g, ref)
     # FP64 flux accumulates in a different order: relative tolerance per pixel
@@ -113,12 +123,12 @@ def test_echelle_parity(hip, seed):
     assert np.max(np.abs(g["flux"][nz] - ref.flux[nz]) / ref.flux[nz]) < 1e-12
 
 
-def test_loaded_rays_parity(hip):
+def test_loaded_rays_parity(hip, kernel):
     """rays handed in (not generated): the per-surface loop on given input"""
     sch = schemes.echelle(n_per_lambda=4, nlambda=512)
     rays = O.gen_rays(sch.beam.pack(), 3, 0, sch.beam.n_rays)
     flag = np.ones(sch.beam.n_rays, dtype=np.int8)
     flag[::17] = 0
-    g = gpu_trace(hip, sch, rays=rays, flag=flag)
+    g = gpu_trace(hip, sch, rays=rays, flag=flag, expect_kernel=kernel)
     ref = O.trace(sch, rays=rays, flag=flag, hist=True, dump=True)
     compare(sch, g, ref)
