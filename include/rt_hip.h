@@ -147,6 +147,11 @@ int rt_hip_last_error(char *buf, size_t len);
 
 typedef struct rt_hip_scheme rt_hip_scheme;
 
+/* Host-side validation of a packed scheme (no device needed): RT_OK or
+   RT_ERR_BAD_VALUE with the reason in rt_hip_last_error. */
+int rt_hip_validate_scheme(const double *surf_table, int n_surf,
+                           const double *spline_pool, size_t pool_len);
+
 /* Validate and upload a packed scheme (host arrays) to device memory. */
 int rt_hip_scheme_create(const double *surf_table, int n_surf,
                          const double *spline_pool, size_t pool_len,
@@ -178,6 +183,25 @@ int rt_hip_trace(const rt_hip_scheme *scheme, const double *beam_desc, uint64_t 
                  unsigned long long *stats_dev,
                  double *dump_dev, signed char *dump_flag_dev,
                  void *stream);
+
+/* ---- generated kernels ------------------------------------------------ */
+
+/* The trace launches a kernel generated for the scheme's structure (surface
+   classes, types, shapes, numbers of aspheric terms) and compiled once per
+   structure with hiprtc; mode 0 forces the generic kernel that reads the
+   structure from the table at run time.  Returns the previous mode.  The
+   environment variable RT_HIP_JIT=0 sets mode 0 at start-up. */
+int rt_hip_set_jit(int mode);
+
+/* "generated" or "generic": the kernel the calling thread's last trace used. */
+int rt_hip_last_kernel(char *buf, size_t len);
+
+/* Generate and compile (without loading, no device needed) the kernel of a
+   packed scheme for beam mode (-1: rays given, else RT_BEAM_*); `text`
+   receives the generated source and the compiler log, *code_size the code
+   object size.  RT_OK when it compiles. */
+int rt_hip_jit_check(const double *surf_table, int n_surf, const double *spline_pool,
+                     size_t pool_len, int beam_mode, char *text, size_t len, size_t *code_size);
 
 #endif /* __HIPCC_RTC__ */
 

@@ -22,7 +22,7 @@ _i8p = C.POINTER(C.c_int8)
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "librt_oracle.so")
+        path = os.environ.get("RT_ORACLE_LIB") or os.path.join(_HERE, "librt_oracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/librt_oracle.so is not built (run `make -C oracle`)")
         L = C.CDLL(path)
@@ -142,3 +142,95 @@ def trace(scheme, seed=0, begin=0, n=None, rays=None, flag=None, flags=None, his
 
 def stage(name, *args):
     return getattr(lib(), "oracle_" + name)(*args)
+
+
+# ---- per-stage wrappers over (7, n) or (6, n) ray arrays ----------------------
+
+def _rows(rays):
+    R = np.ascontiguousarray(rays, dtype=np.float64).copy()
+    return R, [R[k] for k in range(R.shape[0])]
+
+
+def transform(angles, dist, rays):
+    R, rows = _rows(rays)
+    a = np.ascontiguousarray(angles, dtype=np.float64)
+    d = np.ascontiguousarray(dist, dtype=np.float64)
+    lib().oracle_beam_transform(R.shape[1], _p(a), _p(d), *[_p(r) for r in rows[:6]])
+    return R
+
+
+def normalize(rays):
+    R, rows = _rows(rays)
+    lib().oracle_normalize(R.shape[1], _p(rows[3]), _p(rows[4]), _p(rows[5]))
+    return R
+
+
+def intersect(rec, rays):
+    R, rows = _rows(rays)
+    n = R.shape[1]
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    flag = np.ones(n, dtype=np.int8)
+    nb = np.zeros(1, dtype=np.uint64)
+    it = np.zeros(1, dtype=np.uint64)
+    lib().oracle_surface_intersection(_p(rec), n, *[_p(r) for r in rows[:6]], _p(nb, _u64p),
+                                      _p(flag, _i8p), _p(it, _u64p))
+    return R, flag, int(nb[0]), int(it[0])
+
+
+def mask(rec, X, Y):
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    flag = np.ones(len(X), dtype=np.int8)
+    nb = np.zeros(1, dtype=np.uint64)
+    lib().oracle_apply_mask(_p(rec), len(X), _p(X), _p(Y), _p(nb, _u64p), _p(flag, _i8p))
+    return flag
+
+
+def reflect(rec, rays):
+    R, rows = _rows(rays)
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    lib().oracle_surface_reflection(_p(rec), R.shape[1], *[_p(r) for r in rows[:6]])
+    return R
+
+
+def refract(rec, rays, mu):
+    R, rows = _rows(rays)
+    n = R.shape[1]
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    mu = np.ascontiguousarray(np.broadcast_to(mu, (n,)), dtype=np.float64)
+    flag = np.ones(n, dtype=np.int8)
+    nb = np.zeros(1, dtype=np.uint64)
+    lib().oracle_surface_refraction(_p(rec), n, *[_p(r) for r in rows[:6]], _p(mu), _p(nb, _u64p),
+                                    _p(flag, _i8p))
+    return R, flag
+
+
+def diffract(rec, rays, mu, L):
+    R, rows = _rows(rays)
+    n = R.shape[1]
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    mu = np.ascontiguousarray(np.broadcast_to(mu, (n,)), dtype=np.float64)
+    L = np.ascontiguousarray(np.broadcast_to(L, (n,)), dtype=np.float64)
+    flag = np.ones(n, dtype=np.int8)
+    nb = np.zeros(1, dtype=np.uint64)
+    lib().oracle_surface_diffraction(_p(rec), n, *[_p(r) for r in rows[:6]], _p(mu), _p(L),
+                                     _p(nb, _u64p), _p(flag, _i8p))
+    return R, flag
+
+
+def ccd_coordinates(X, Y, xsize, ysize, xpix, ypix):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    cx = np.zeros(len(X), dtype=np.uint32)
+    cy = np.zeros(len(X), dtype=np.uint32)
+    lib().oracle_ccd_coordinates(len(X), _p(X), _p(Y), xsize, ysize, xpix, ypix, _p(cx, _u32p), _p(cy, _u32p))
+    return cx, cy
+
+
+def grating_energy(lam, order, blaze, alpha, gamma, gr_const, rule_start=0.0, rule_stop=1.0):
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    e = np.ones(len(lam))
+    lib().oracle_grating_energy_distr(len(lam), _p(lam), order, blaze, alpha, gamma, gr_const,
+                                      rule_start, rule_stop, _p(e))
+    return e
