@@ -38,7 +38,7 @@ def _enum_values(text):
     vals = {}
     for name, val in re.findall(r"#define (RT_[A-Z0-9_]+) (\d+)", text):
         vals[name] = int(val)
-    for name, val in re.findall(r"\b(RT_[A-Z0-9_]+)\s*=\s*([A-Z0-9_+* ]+?)\s*[,}\n]", text):
+    for name, val in re.findall(r"\b(RT_[A-Z0-9_]+)\s*=\s*([A-Z0-9_+* ]+?)\s*(?:,|}|/\*|\n)", text):
         vals[name] = int(eval(val, {}, dict(vals)))
     return vals
 
