@@ -103,6 +103,7 @@ enum rt_beam_field {
     RT_B_RANGE = 11,     /* 11..12 wavelength range (um)                    */
     RT_B_NLAMBDA = 13,   /* wavelengths of a composite beam                 */
     RT_B_COMPOSITE = 14, /* 1: lambda[r] = grid[r mod NLAMBDA]              */
+    RT_B_ANGUNIT = 15,   /* 1: a conic beam's ANGPAR is in radians (plugin API) */
     RT_BEAM_STRIDE = 16
 };
 
@@ -183,6 +184,73 @@ int rt_hip_trace(const rt_hip_scheme *scheme, const double *beam_desc, uint64_t 
                  unsigned long long *stats_dev,
                  double *dump_dev, signed char *dump_flag_dev,
                  void *stream);
+
+/* ---- per-stage kernels (the plugin API of rt_engine_func.h runs on these) --
+   All ray arrays are device SoA buffers of n elements; `rec` is one packed
+   surface record on the host; n_bad_dev / iters_dev (nullable) are device
+   uint64 counters that are incremented, never cleared. */
+
+/* beam_transform (rt_engine_func.h:41-42; rt_cuda_engine_beam.cu:416-579);
+   angles in radians, sin/cos evaluated on the host */
+int rt_hip_stage_transform(const double *angles, const double *dist, size_t n,
+                           double *X, double *Y, double *Z, double *cX, double *cY, double *cZ,
+                           void *stream);
+/* Beam::Normalize (common-lib/beam.cpp:527-571) */
+int rt_hip_stage_normalize(size_t n, double *cX, double *cY, double *cZ, void *stream);
+/* surface_intersection (rt_engine_func.h:46-49; rt_cuda_engine_surface.cu:655-760) */
+int rt_hip_stage_intersect(const double *rec, size_t n, double *X, double *Y, double *Z,
+                           double *cX, double *cY, double *cZ, signed char *flag,
+                           unsigned long long *n_bad_dev, unsigned long long *iters_dev, void *stream);
+/* Surface::ApplyConstrains / Shade::ApplyConstrains (surface.cpp:264-321,652-699) */
+int rt_hip_stage_mask(const double *rec, size_t n, double *X, double *Y, signed char *flag,
+                      unsigned long long *n_bad_dev, void *stream);
+/* surface_reflection (rt_engine_func.h:52-54; rt_cuda_engine_surface.cu:883-951) */
+int rt_hip_stage_reflect(const double *rec, size_t n, double *X, double *Y, double *Z,
+                         double *cX, double *cY, double *cZ, void *stream);
+/* surface_refraction (rt_engine_func.h:57-61): mu = n1(lambda)/n2(lambda) per ray */
+int rt_hip_stage_refract(const double *rec, size_t n, double *X, double *Y, double *Z,
+                         double *cX, double *cY, double *cZ, const double *mu, signed char *flag,
+                         unsigned long long *n_bad_dev, void *stream);
+/* surface_diffration (rt_engine_func.h:64-70): L = m lambda / n2(lambda) / d per ray,
+   mu per ray for transmissive gratings (NULL for reflective ones) */
+int rt_hip_stage_diffract(const double *rec, size_t n, double *X, double *Y, double *Z,
+                          double *cX, double *cY, double *cZ, const double *mu, const double *L,
+                          signed char *flag, unsigned long long *n_bad_dev, void *stream);
+
+/* beam_random (rt_engine_func.h:30-31): positions of rays [begin, begin+n) */
+int rt_hip_beam_positions(const double *beam_desc, uint64_t seed, size_t begin, size_t n,
+                          double *X, double *Y, double *Z, void *stream);
+/* beam_conic_cosins / beam_gauss_cosins (rt_engine_func.h:33,35) by beam type;
+   X, Y are read for conic beams */
+int rt_hip_beam_directions(const double *beam_desc, uint64_t seed, size_t begin, size_t n,
+                           const double *X, const double *Y, double *cX, double *cY, double *cZ,
+                           void *stream);
+/* uniform_range_beam / random_range_beam (rt_engine_func.h:37,39) by range distribution */
+int rt_hip_beam_wavelengths(const double *beam_desc, uint64_t seed, size_t begin, size_t n,
+                            double *lambda, void *stream);
+
+/* ccd_coordinates (rt_engine_func.h:96-99; rt_cuda_engine_ccd.cu:96-214) */
+int rt_hip_ccd_coordinates(size_t n, const double *X, const double *Y, double xsize, double ysize,
+                           double xpix, double ypix, unsigned *ccdX, unsigned *ccdY, void *stream);
+/* compute_ccd_image accumulation (rt_engine_func.h:101-104; ccd.cu:219-232):
+   image[y*xdim+x] += w (w NULL: 1) and/or hist += 1, bounds-checked, the
+   off-chip rays counted */
+int rt_hip_ccd_accumulate(size_t n, const unsigned *ccdX, const unsigned *ccdY, const double *w,
+                          size_t xdim, size_t ydim, double *image, unsigned *hist,
+                          unsigned long long *offchip_dev, void *stream);
+/* surface_QE multiply (rt_engine_func.h:73; rt_cuda_engine_surface.cu:1775-1784): v *= f */
+int rt_hip_scale(size_t n, double *v, const double *f, void *stream);
+/* grating_energy_distr (rt_engine_func.h:75-78; rt_cuda_engine_surface.cu:2098-2290) */
+int rt_hip_grating_energy(size_t n, const double *lambda, long order, double blaze, double alpha,
+                          double gamma, double gr_const, double rule_start, double rule_stop,
+                          double *energy, void *stream);
+/* spline_init (rt_engine_func.h:91): host, serial natural-spline recurrence */
+int rt_hip_spline_init(size_t n, const double *x, const double *y, double *y2);
+/* TabulatedFunction::operator[] (tabulated_function.cpp:173-204) at n device
+   points, spline [K, x, y, y2] given on the host and staged in LDS; points out
+   of range give 0 and are counted in *oob_dev */
+int rt_hip_spline_eval(const double *spline, size_t n, const double *x, double *y,
+                       unsigned long long *oob_dev, void *stream);
 
 /* ---- generated kernels ------------------------------------------------ */
 

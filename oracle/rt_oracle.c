@@ -161,7 +161,9 @@ static void gen_direction(const double *B, uint64_t seed, uint64_t r, ray_t *ray
     if (type == RT_BEAM_PARALLEL) {            /* beam.cpp:363-377 */
         ray->cX = 0.0; ray->cY = 0.0; ray->cZ = 1.0;
     } else if (type == RT_BEAM_CONIC) {        /* beam.cu:586-611, angle: beam.cpp:383 */
-        double ang = B[RT_B_ANGPAR] * DEG2RAD;
+        /* params[3]*deg2rad with the macro RT_PI/180.0 expanded: (params[3]*RT_PI)/180.0;
+           RT_B_ANGUNIT 1: the angle is already in radians (engine API argument) */
+        double ang = B[RT_B_ANGUNIT] != 0.0 ? B[RT_B_ANGPAR] : B[RT_B_ANGPAR] * RT_PI / 180.0;
         double cs = cos(ang / 2.0), sn = sin(ang / 2.0);
         double t = atan2(ray->Y, ray->X);
         ray->cZ = cs;

@@ -1,0 +1,25 @@
+// rt_host.h -- host-side helpers shared by the translation units of librt_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+// Record an error message for rt_hip_last_error and return `code`.
+int rt_set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define HIP_TRY(expr, code)                                                                    \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return rt_set_error(code, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                                __LINE__);                                                     \
+    } while (0)
+
+// Current device selected and its properties cached (RT_OK or an error).
+int rt_ensure_device();
+// Compute units of the current device (after rt_ensure_device).
+int rt_device_cus();
+
+// Device record (rt::DEV_STRIDE doubles) of one public packed record: the
+// public fields plus the host-derived constants; spline references are
+// copied unchanged.
+void rt_device_record(const double* pub, double* dev);

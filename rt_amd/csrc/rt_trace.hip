@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "rt_device.h"
+#include "rt_host.h"
 #include "rt_jit.h"
 
 using rt::Ray;
@@ -42,9 +43,8 @@ static thread_local const char* g_last_kernel = "none";
 // --------------------------------------------------------------- errors
 static thread_local std::string g_last_error;
 
-static int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
-static int set_error(int code, const char* fmt, ...) {
-    char buf[512];
+int rt_set_error(int code, const char* fmt, ...) {
+    char buf[1024];
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
@@ -52,14 +52,7 @@ static int set_error(int code, const char* fmt, ...) {
     g_last_error = buf;
     return code;
 }
-
-#define HIP_TRY(expr, code)                                                                 \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return set_error(code, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
-                             __LINE__);                                                     \
-    } while (0)
+#define set_error rt_set_error
 
 // ----------------------------------------------------------------- kernel
 
@@ -105,7 +98,9 @@ struct DeviceInfo {
 };
 thread_local DeviceInfo g_dev;
 
-int ensure_device() {
+}  // namespace
+
+int rt_ensure_device() {
     if (g_dev.valid) return RT_OK;
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev), RT_ERR_INIT_FAILED);
@@ -115,6 +110,11 @@ int ensure_device() {
     g_dev.valid = true;
     return RT_OK;
 }
+
+int rt_device_cus() { return g_dev.cus > 0 ? g_dev.cus : 256; }
+
+namespace {
+int ensure_device() { return rt_ensure_device(); }
 
 bool fn_ref_ok(double kind, double val, size_t pool_len, const double* pool) {
     if (kind == (double)RT_FN_CONST) return true;
@@ -210,6 +210,14 @@ extern "C" int rt_hip_validate_scheme(const double* t, int n_surf, const double*
     return RT_OK;
 }
 
+void rt_device_record(const double* p, double* d) {
+    for (int k = 0; k < rt::DEV_STRIDE; ++k) d[k] = k < RT_SURF_STRIDE ? p[k] : 0.0;
+    d[rt::D_INV_C] = p[RT_S_C] != 0.0 ? 1.0 / p[RT_S_C] : 0.0;
+    d[rt::D_INV_D] = p[RT_S_GRCONST] != 0.0 ? 1.0 / p[RT_S_GRCONST] : 0.0;
+    d[rt::D_HALF_X] = p[RT_S_CCD_XSIZE] / 2.0;
+    d[rt::D_HALF_Y] = p[RT_S_CCD_YSIZE] / 2.0;
+}
+
 // Device tables: each record gets its host-derived constants (rt::DevField)
 // and each referenced spline is re-laid out as [K, x0, (K-1)/(x[K-1]-x0),
 // x, y, y2] with the references patched to the new offsets.
@@ -236,7 +244,7 @@ static void build_device_tables(rt_hip_scheme* s) {
     for (int i = 0; i < n; ++i) {
         double* d = s->dtable.data() + (size_t)i * rt::DEV_STRIDE;
         const double* p = s->table.data() + (size_t)i * RT_SURF_STRIDE;
-        for (int k = 0; k < RT_SURF_STRIDE; ++k) d[k] = p[k];
+        rt_device_record(p, d);
         int type = (int)p[RT_S_TYPE];
         if (type == RT_LENS || type == RT_GRATING) {
             remap(d[RT_S_N1KIND], d[RT_S_N1VAL]);
@@ -246,10 +254,6 @@ static void build_device_tables(rt_hip_scheme* s) {
             remap(d[RT_S_QEKIND], d[RT_S_QEVAL]);
             remap(d[RT_S_SPKIND], d[RT_S_SPVAL]);
         }
-        d[rt::D_INV_C] = p[RT_S_C] != 0.0 ? 1.0 / p[RT_S_C] : 0.0;
-        d[rt::D_INV_D] = p[RT_S_GRCONST] != 0.0 ? 1.0 / p[RT_S_GRCONST] : 0.0;
-        d[rt::D_HALF_X] = p[RT_S_CCD_XSIZE] / 2.0;
-        d[rt::D_HALF_Y] = p[RT_S_CCD_YSIZE] / 2.0;
     }
     s->pool_len = s->dpool.size();
     s->signature = rt_jit_signature(s->dtable.data(), n);
@@ -362,7 +366,8 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
     if (beam_desc) memcpy(a.beam.v, beam_desc, sizeof a.beam.v);
     if (gen) {
         // uniform values of the direction generators, on the host (beam.cu:592-598, 1034, 1099)
-        double ang = a.beam.v[RT_B_ANGPAR] * rt::kDeg2Rad;
+        // params[3]*deg2rad as the macro expands (beam.cpp:383): (params[3]*RT_PI)/180.0
+        double ang = a.beam.v[RT_B_ANGUNIT] != 0.0 ? a.beam.v[RT_B_ANGPAR] : a.beam.v[RT_B_ANGPAR] * rt::kPi / 180.0;
         a.beam.u.cone_cos = cos(ang / 2.0);
         a.beam.u.cone_sin = sin(ang / 2.0);
         double stddev = a.beam.v[RT_B_ANGPAR] / 2.35482 * rt::kPi / 180 / 3600;
@@ -379,7 +384,7 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
     a.X = X; a.Y = Y; a.Z = Z; a.cX = cX; a.cY = cY; a.cZ = cZ; a.L = lambda; a.F = flag;
     a.hist = ccd_hist; a.flux = ccd_flux; a.stats = stats; a.dump = dump; a.dump_flag = dump_flag;
 
-    const int cus = g_dev.cus > 0 ? g_dev.cus : 256;
+    const int cus = rt_device_cus();
     uint64_t want = (n_rays + kBlock - 1) / kBlock;
     uint64_t cap = (uint64_t)cus * 8;      // persistent grid: 8 blocks (32 waves) per CU
     unsigned grid = (unsigned)(want < cap ? want : cap);

@@ -46,7 +46,7 @@ S_QEKIND, S_QEVAL, S_SPKIND, S_SPVAL = 50, 51, 52, 53
 SURF_STRIDE = 56
 
 (B_TYPE, B_SHAPE, B_RANGE_DISTR, B_NPER, B_P1, B_P2, B_P3, B_ANGPAR) = range(8)
-B_CEN, B_RANGE, B_NLAMBDA, B_COMPOSITE = 8, 11, 13, 14
+B_CEN, B_RANGE, B_NLAMBDA, B_COMPOSITE, B_ANGUNIT = 8, 11, 13, 14, 15
 BEAM_STRIDE = 16
 
 RT_TRACE_GENERATE, RT_TRACE_STORE, RT_TRACE_HIST, RT_TRACE_FLUX = 1, 2, 4, 8

@@ -154,7 +154,7 @@ def test_gauss_directions():
 def test_conic_directions():
     b = _beam(type=S.RT_BEAM_CONIC, params=(500, 0.0, 5.0, 30.0), range_distr=S.RT_RANGE_MONO)
     R = O.gen_rays(b, 1, 0, 500)
-    half = 30.0 * S.DEG2RAD / 2.0
+    half = 30.0 * S.RT_PI / 180.0 / 2.0              # beam.cpp:383 macro expansion
     assert np.all(R[5] == math.cos(half))
     az = np.arctan2(R[4], R[3])
     assert np.allclose(np.cos(az - np.arctan2(R[1], R[0])), 1.0, atol=1e-12)
