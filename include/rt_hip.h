@@ -2,7 +2,7 @@
  * rt_hip.h -- C ABI of the MI355X (gfx950) per-ray tracing engine.
  *
  * This is the thin extern "C" boundary underneath the rt engine plugin API
- * (common-lib/rt_engine_func.h:25-104 of amirych/rt).  The 17 C++ functions
+ * (common-lib/rt_engine_func.h:25-104 of amirych/rt).  The 19 C++ functions
  * of that header are implemented in librt_engine_hip.so on top of the
  * entry points below; callers that keep rays resident in HBM (a fused
  * Scheme::run, the Python tests, bench.py) call these directly.

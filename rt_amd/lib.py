@@ -45,6 +45,25 @@ def lib():
         L.rt_hip_last_kernel.argtypes = [C.c_char_p, C.c_size_t]
         L.rt_hip_jit_check.argtypes = [_dp, C.c_int, _dp, C.c_size_t, C.c_int, C.c_char_p, C.c_size_t,
                                        C.POINTER(C.c_size_t)]
+        # per-stage entry points: host arrays (records, angles, beam descriptions) as
+        # double*, device buffers and the stream as void*
+        n, u64, lng, dbl = C.c_size_t, C.c_uint64, C.c_long, C.c_double
+        L.rt_hip_stage_transform.argtypes = [_dp, _dp, n] + [_vp] * 6 + [_vp]
+        L.rt_hip_stage_normalize.argtypes = [n] + [_vp] * 3 + [_vp]
+        L.rt_hip_stage_intersect.argtypes = [_dp, n] + [_vp] * 6 + [_vp, _vp, _vp, _vp]
+        L.rt_hip_stage_mask.argtypes = [_dp, n, _vp, _vp, _vp, _vp, _vp]
+        L.rt_hip_stage_reflect.argtypes = [_dp, n] + [_vp] * 6 + [_vp]
+        L.rt_hip_stage_refract.argtypes = [_dp, n] + [_vp] * 6 + [_vp, _vp, _vp, _vp]
+        L.rt_hip_stage_diffract.argtypes = [_dp, n] + [_vp] * 6 + [_vp, _vp, _vp, _vp, _vp]
+        L.rt_hip_beam_positions.argtypes = [_dp, u64, n, n, _vp, _vp, _vp, _vp]
+        L.rt_hip_beam_directions.argtypes = [_dp, u64, n, n, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.rt_hip_beam_wavelengths.argtypes = [_dp, u64, n, n, _vp, _vp]
+        L.rt_hip_ccd_coordinates.argtypes = [n, _vp, _vp, dbl, dbl, dbl, dbl, _vp, _vp, _vp]
+        L.rt_hip_ccd_accumulate.argtypes = [n, _vp, _vp, _vp, n, n, _vp, _vp, _vp, _vp]
+        L.rt_hip_scale.argtypes = [n, _vp, _vp, _vp]
+        L.rt_hip_grating_energy.argtypes = [n, _vp, lng, dbl, dbl, dbl, dbl, dbl, dbl, _vp, _vp]
+        L.rt_hip_spline_init.argtypes = [n, _dp, _dp, _dp]
+        L.rt_hip_spline_eval.argtypes = [_dp, n, _vp, _vp, _vp, _vp]
         _LIB = L
     return _LIB
 

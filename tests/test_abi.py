@@ -92,3 +92,76 @@ def test_generated_kernel_compiles(make, mode):
     assert rc == 0, text[-2000:]
     assert size > 10000
     assert text.count("rt::surface_step(") == make().n_surf
+
+
+ENGINE_HEADER = os.path.join(ROOT, "include", "rt", "rt_engine_func.h")
+ENGINE_LIB = os.path.join(ROOT, "rt_amd", "librt_engine_hip.so")
+COMMON_LIB = os.path.join(ROOT, "rt_amd", "librt_common_hip.so")
+
+# The 19 entry points of common-lib/rt_engine_func.h:25-104 as the reference's
+# callers link them (C++ linkage: these demangled signatures are the ABI).
+PLUGIN_SIGNATURES = [
+    "rt_engine_init()",
+    "rt_engine_info(std::__cxx11::list<std::__cxx11::basic_string<char, std::char_traits<char>, "
+    "std::allocator<char> >, std::allocator<std::__cxx11::basic_string<char, std::char_traits<char>, "
+    "std::allocator<char> > > >&)",
+    "beam_random(Beam::BeamShape, double*, double*, unsigned long, double*, double*, double*)",
+    "beam_conic_cosins(unsigned long, double, double*, double*, double*, double*, double*)",
+    "beam_gauss_cosins(unsigned long, double, double*, double*, double*)",
+    "uniform_range_beam(unsigned long, double*, double*)",
+    "random_range_beam(unsigned long, double*, double*)",
+    "beam_transform(unsigned long, double*, double*, double*, double*, double*, double*, double*, double*)",
+    "surface_intersection(Surface::SurfaceClass, double*, unsigned long, double*, double*, double*, double*, "
+    "double*, double*, unsigned long*, char*)",
+    "surface_reflection(Surface::SurfaceClass, double*, unsigned long, double*, double*, double*, double*, "
+    "double*, double*)",
+    "surface_refraction(Surface::SurfaceClass, double*, unsigned long, double*, double*, double*, double*, "
+    "double*, double*, double*, TabulatedFunction&, TabulatedFunction&, unsigned long*, char*)",
+    "surface_diffration(Surface::SurfaceClass, Grating::GratingType, double*, long, double, unsigned long, "
+    "double*, double*, double*, double*, double*, double*, double*, TabulatedFunction&, TabulatedFunction&, "
+    "unsigned long*, char*)",
+    "surface_QE(unsigned long, double*, double*, TabulatedFunction&)",
+    "grating_energy_distr(unsigned long, double*, long, double, double, double, double, double, double, double*)",
+    "rt_engine_free_vector(double*)",
+    "rt_engine_alocate_vector(unsigned long)",
+    "spline_init(unsigned long, double*, double*, double*)",
+    "ccd_coordinates(unsigned long, double*, double*, double, double, double, double, unsigned int*, "
+    "unsigned int*)",
+    "compute_ccd_image(unsigned long, unsigned int*, unsigned int*, double*, TabulatedFunction&, "
+    "TabulatedFunction&, unsigned long, unsigned long, double, double, double*)",
+]
+
+
+def _demangled_exports(path):
+    out = subprocess.run(["nm", "-D", "-C", "--defined-only", path], capture_output=True, text=True).stdout
+    return {line.split(" T ", 1)[1] for line in out.splitlines() if " T " in line}
+
+
+def test_plugin_library_exports_the_19_entry_points():
+    assert len(PLUGIN_SIGNATURES) == 19
+    header = open(ENGINE_HEADER).read()
+    declared = re.findall(r"^(?:RT_engine_error|void|real_t \*)\s*(\w+)\(", header, re.M)
+    assert sorted(declared) == sorted(s.split("(")[0] for s in PLUGIN_SIGNATURES)
+    exported = _demangled_exports(ENGINE_LIB)
+    missing = [s for s in PLUGIN_SIGNATURES if s not in exported]
+    assert not missing, f"not exported with the reference signature: {missing}"
+    # the engine must not define TabulatedFunction: its callers bring their own
+    assert not any(s.startswith("TabulatedFunction::") for s in exported)
+    common = _demangled_exports(COMMON_LIB)
+    assert "TabulatedFunction::operator[](double const&)" in common
+
+
+def test_plugin_library_loads_without_a_device():
+    # the engine takes TabulatedFunction::operator[] from its caller: here librt_common_hip.so
+    C.CDLL(COMMON_LIB)
+    L = C.CDLL(ENGINE_LIB)
+    sym = "_Z11spline_initmPdS_S_"                  # spline_init(size_t, real_t*, real_t*, real_t*)
+    f = getattr(L, sym)
+    f.restype = C.c_int
+    f.argtypes = [C.c_size_t] + [C.POINTER(C.c_double)] * 3
+    x = np.linspace(0.0, 3.0, 7)
+    y = np.sin(x)
+    y2 = np.zeros(7)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    assert f(7, dp(x), dp(y), dp(y2)) == 0
+    assert np.array_equal(y2, S.natural_spline_y2(x, y))
