@@ -26,6 +26,20 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "fused_trace_echelle_hbm_traffic.txt")
+HOST_CPU_SHARE = 16         # host threads one GPU's job may use
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fused trace on this workload, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        for line in open(TRAFFIC_SUMMARY):
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -51,7 +65,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine is measured on one GPU; --gpus > 1 is not supported "
-                                   "(rays shard by range, see DESIGN.md)"}))
+                                   "(rays would shard by ray range with one reduction of the CCD frame)"}))
         sys.exit(2)
 
     import torch
@@ -118,7 +132,8 @@ def main():
                    "nlambda": args.nlambda, "surfaces": sch.n_surf, "ccd": "4096x4096",
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
+                     "traffic": measured_traffic()},
         "detail": {"kernel_ms": kernel_ms, "ray_surface_per_step": ray_surf // args.steps,
                    "ccd_hits_per_step": int(st[S.ST_CCD_HITS]) // args.steps,
                    "flop_per_step": fl},
@@ -127,7 +142,7 @@ def main():
     if not args.no_cpu:
         from oracle import oracle as O
         m = min(args.cpu_sample, n) if args.cpu_sample > 0 else n
-        threads = len(os.sched_getaffinity(0))
+        threads = min(HOST_CPU_SHARE, len(os.sched_getaffinity(0)))
         t0 = time.perf_counter()
         r = O.trace(sch, seed=0, n=m, nthreads=threads)
         cpu_s = time.perf_counter() - t0

@@ -284,7 +284,12 @@ extern "C" int rt_hip_stage_intersect(const double* rec, size_t n, double* X, do
                                       double* cY, double* cZ, signed char* flag, unsigned long long* n_bad,
                                       unsigned long long* iters, void* stream) {
     if (!rec || !flag || !rays_ok(X, Y, Z, cX, cY, cZ)) return rt_set_error(RT_ERR_BAD_VALUE, "NULL argument");
-    int rc = rt_hip_validate_scheme(rec, 1, nullptr, 0);
+    // the intersection reads the geometry only: validate it without the record's
+    // tabulated-function and CCD fields, which refer to a scheme's spline pool
+    double geo[RT_SURF_STRIDE];
+    memcpy(geo, rec, sizeof geo);
+    geo[RT_S_TYPE] = RT_STOP;
+    int rc = rt_hip_validate_scheme(geo, 1, nullptr, 0);
     if (rc != RT_OK) return rc;
     return launch_stage(rec, n, OP_INTERSECT, X, Y, Z, cX, cY, cZ, nullptr, nullptr, flag, n_bad, iters, stream);
 }

@@ -165,3 +165,23 @@ def test_plugin_library_loads_without_a_device():
     dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
     assert f(7, dp(x), dp(y), dp(y2)) == 0
     assert np.array_equal(y2, S.natural_spline_y2(x, y))
+
+
+def test_stage_intersect_accepts_scheme_records():
+    """A record taken from a packed scheme refers to the scheme's spline pool
+    (refractive indices, QE); the intersection stage checks only its geometry."""
+    L = lib.lib()
+    sch = schemes.echelle(1, 4)
+    t = np.asarray(sch.table).reshape(sch.n_surf, S.SURF_STRIDE)
+    dummy = np.zeros(8)
+    flag = np.ones(8, dtype=np.int8)
+    for s in range(sch.n_surf):
+        rec = np.ascontiguousarray(t[s])
+        rc = L.rt_hip_stage_intersect(rec.ctypes.data_as(C.POINTER(C.c_double)), 0,
+                                      *[dummy.ctypes.data] * 6, flag.ctypes.data, None, None, None)
+        assert rc == 0, (s, lib.last_error())
+    bad = np.ascontiguousarray(t[6]).copy()
+    bad[S.S_NPOLY] = 11
+    rc = L.rt_hip_stage_intersect(bad.ctypes.data_as(C.POINTER(C.c_double)), 0, *[dummy.ctypes.data] * 6,
+                                  flag.ctypes.data, None, None, None)
+    assert rc == 4 and "aspheric" in lib.last_error()

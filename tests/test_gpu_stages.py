@@ -262,9 +262,9 @@ def test_ccd_stages(hip):
     X, Y = res.rays[0, ok], res.rays[1, ok]
     det = sch.detector().ccd
     xs, ys = det["xdim"] * det["xpix"], det["ydim"] * det["ypix"]
-    # stretch the spot over the chip and past its edges
-    X = X * (0.6 * xs / max(1e-9, np.ptp(X))) + 0.1 * xs
-    Y = Y * (0.6 * ys / max(1e-9, np.ptp(Y))) - 0.1 * ys
+    # stretch the spot past the chip edges on both axes (about a fifth of the rays off-chip)
+    X = (X - X.mean()) * (1.2 * xs / max(1e-9, np.ptp(X))) + 0.05 * xs
+    Y = (Y - Y.mean()) * (1.2 * ys / max(1e-9, np.ptp(Y))) - 0.05 * ys
     n = len(X)
     rx, ry = O.ccd_coordinates(X, Y, xs, ys, det["xpix"], det["ypix"])
     tx, ty = dev(X), dev(Y)
@@ -341,4 +341,16 @@ def test_spline_scale_and_grating_energy(hip):
         check(L.rt_hip_grating_energy(len(lam), tl.data_ptr(), order, blaze, blaze, 0.0, d, 0.0, 1.0,
                                       te.data_ptr(), stream()), hip, "rt_hip_grating_energy")
         ge = te.cpu().numpy()
-        assert np.all(np.abs(ge - ref_e) <= 1e-11 * np.maximum(np.abs(ref_e), 1e-3)), order
+        # where the order does not propagate (|sin beta| > 1) both give NaN, as the reference does
+        nan = np.isnan(ref_e)
+        assert np.array_equal(np.isnan(ge), nan), order
+        assert nan.sum() < len(lam)
+        ok = ~nan
+        # The formula is ill-conditioned near grazing orders: asin(m lambda/d - sin alpha)
+        # with an argument near +-1 amplifies a one-ulp difference between two math
+        # libraries' sin/asin by 1/sqrt(1-x^2); perturbing sin(alpha) or the asin argument
+        # by one ulp on the host moves these values by up to 2.9e-11 of the peak.  Bar:
+        # 1e-10 of the distribution's peak, absolute.
+        err = np.abs(ge[ok] - ref_e[ok])
+        peak = float(np.max(np.abs(ref_e[ok])))
+        assert float(np.max(err)) <= 1e-10 * peak, (order, float(np.max(err)), peak)

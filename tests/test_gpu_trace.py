@@ -132,3 +132,62 @@ def test_loaded_rays_parity(hip, kernel):
     g = gpu_trace(hip, sch, rays=rays, flag=flag, expect_kernel=kernel)
     ref = O.trace(sch, rays=rays, flag=flag, hist=True, dump=True)
     compare(sch, g, ref)
+
+
+def test_full_size_echelle_properties(hip):
+    """The echelle at its full 10,000,384 rays: properties that need no oracle
+    at this size (determinism, shards compose to the whole bit for bit, counts
+    conserved), plus oracle parity on three ray blocks of the same launch."""
+    sch = schemes.echelle()
+    n = sch.beam.n_rays
+    ds = hip.DeviceScheme(sch)
+    det = sch.detector()
+    npix = det.ccd["ydim"] * det.ccd["xdim"]
+    flags = S.RT_TRACE_GENERATE | S.RT_TRACE_STORE | S.RT_TRACE_HIST | S.RT_TRACE_FLUX
+
+    def run(begin, count, seed=21):
+        R = torch.zeros((7, count), dtype=torch.float64, device="cuda")
+        F = torch.zeros(count, dtype=torch.int8, device="cuda")
+        H = torch.zeros(npix, dtype=torch.int32, device="cuda")
+        X = torch.zeros(npix, dtype=torch.float64, device="cuda")
+        st = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
+        print(f"[full size] begin={begin} n={count}", flush=True)
+        ds.trace(R, F, seed=seed, begin=begin, n=count, flags=flags, hist=H, flux=X, stats=st)
+        torch.cuda.synchronize()
+        return R, F, H, X, st.cpu().numpy().view(np.uint64)
+
+    R1, F1, H1, X1, st1 = run(0, n)
+    R2, F2, H2, X2, st2 = run(0, n)
+    assert torch.equal(R1, R2) and torch.equal(F1, F2) and torch.equal(H1, H2), "trace is not deterministic"
+    assert np.array_equal(st1, st2)
+    del R2, F2, H2, X2
+    h = n // 2 + 123                    # shard boundary inside a wave and inside a wavelength group
+    Ra, Fa, Ha, Xa, sta = run(0, h)
+    Rb, Fb, Hb, Xb, stb = run(h, n - h)
+    assert torch.equal(torch.cat([Ra, Rb], dim=1), R1), "shards differ from the whole"
+    assert torch.equal(torch.cat([Fa, Fb]), F1)
+    assert torch.equal(Ha + Hb, H1), "shard hit counts do not add up"
+    assert np.array_equal(sta + stb, st1)
+    x1 = X1.cpu().numpy()
+    xs = (Xa + Xb).cpu().numpy()
+    assert np.all(np.abs(xs - x1) <= 1e-12 * np.abs(x1)), "shard flux does not add up"
+    del Ra, Rb, Fa, Fb, Ha, Hb, Xa, Xb
+    # conservation
+    good = int(F1.sum().item())
+    assert int(st1[S.ST_RAYS]) == n and int(st1[S.ST_GOOD]) == good
+    assert int(H1.sum().item()) == int(st1[S.ST_CCD_HITS])
+    assert int(st1[S.ST_CCD_HITS]) + int(st1[S.ST_CCD_OFFCHIP]) == good      # the CCD is the last surface
+    for s in range(sch.n_surf - 1):
+        entering, lost_i, lost_a = (int(st1[S.ST_SURF + 4 * s + k]) for k in (0, 2, 3))
+        nxt = int(st1[S.ST_SURF + 4 * (s + 1)])
+        assert nxt == entering - lost_i - lost_a, f"ray count not conserved after surface {s}"
+    assert good > 0.95 * n
+    # oracle parity on three blocks of the full launch
+    Lsc = length_scale(sch)
+    for b0 in (0, n // 2 - 1000, n - 2048):
+        ref = O.trace(sch, seed=21, begin=b0, n=2048)
+        g = R1[:, b0:b0 + 2048].cpu().numpy()
+        f = F1[b0:b0 + 2048].cpu().numpy()
+        assert np.array_equal(f, ref.flag)
+        assert rel_err(g, ref.rays, ref.flag == 1, Lsc) <= RTOL
+    ds.close()
