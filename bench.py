@@ -86,6 +86,11 @@ def main():
     flags = S.RT_TRACE_GENERATE | S.RT_TRACE_STORE | S.RT_TRACE_HIST | S.RT_TRACE_FLUX
     stream = torch.cuda.current_stream()
 
+    # launch configuration of the generated kernel chosen by the autotuner on a
+    # 2e6-ray sample of the same beam (setup, outside the timed region)
+    tuned_ms = ds.autotune(n_sample=min(n, 2_000_000), flags=flags, seed=999)
+    block, waves, grid_mult = ds.get_launch()
+
     def step(k):
         ds.trace(rays, flag, seed=k, flags=flags, hist=hist, flux=flux, stats=stats,
                  stream=stream.cuda_stream)
@@ -134,7 +139,9 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
                      "traffic": measured_traffic()},
-        "detail": {"kernel_ms": kernel_ms, "ray_surface_per_step": ray_surf // args.steps,
+        "detail": {"kernel_ms": kernel_ms, "launch": {"block": block, "waves_per_simd": waves,
+                                                      "grid_mult": grid_mult, "autotune_sample_ms": tuned_ms},
+                   "ray_surface_per_step": ray_surf // args.steps,
                    "ccd_hits_per_step": int(st[S.ST_CCD_HITS]) // args.steps,
                    "flop_per_step": fl},
     }

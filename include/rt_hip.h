@@ -261,6 +261,22 @@ int rt_hip_spline_eval(const double *spline, size_t n, const double *x, double *
    environment variable RT_HIP_JIT=0 sets mode 0 at start-up. */
 int rt_hip_set_jit(int mode);
 
+/* Launch of a scheme's generated kernel: lanes per workgroup (64..1024,
+   powers of two), the minimum number of waves per SIMD the compiler must fit
+   its registers to (0: its own choice), and persistent workgroups per CU as a
+   multiple of those that are resident at once.  Default 256, 0, 2.  No
+   counterpart in the reference (its launches are fixed per kernel). */
+int rt_hip_scheme_set_launch(rt_hip_scheme *scheme, int block, int waves, int grid_mult);
+int rt_hip_scheme_get_launch(const rt_hip_scheme *scheme, int *block, int *waves, int *grid_mult);
+
+/* Autotuner: traces n_sample rays of the beam (generated from seed, with
+   `flags`) under each candidate launch (block 128/256, waves 0/5/6, grid
+   multiple 1/2), keeps the fastest in the scheme and returns its time in
+   *best_ms.  Allocates its sample buffers; call it once after
+   rt_hip_scheme_create, outside any timed or captured region. */
+int rt_hip_autotune(rt_hip_scheme *scheme, const double *beam_desc, uint64_t seed, size_t n_sample,
+                    int flags, double *best_ms, void *stream);
+
 /* "generated" or "generic": the kernel the calling thread's last trace used. */
 int rt_hip_last_kernel(char *buf, size_t len);
 

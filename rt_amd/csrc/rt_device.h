@@ -43,7 +43,9 @@ enum DevField {
     D_INV_D = RT_SURF_STRIDE + 1,      // 1/grating constant
     D_HALF_X = RT_SURF_STRIDE + 2,     // CCD xsize/2
     D_HALF_Y = RT_SURF_STRIDE + 3,     // CCD ysize/2
-    DEV_STRIDE = RT_SURF_STRIDE + 8
+    D_K1C2 = RT_SURF_STRIDE + 4,       // (k+1) c c, the product the reference forms per ray
+    D_IA = RT_SURF_STRIDE + 8,         // i a_i, i = 1..RT_MAX_POLY (aspheric derivative terms)
+    DEV_STRIDE = RT_SURF_STRIDE + 24
 };
 
 // Intersection algorithm of a surface (engine_intersection_cycle dispatch,
@@ -66,6 +68,7 @@ struct DynSurf {
     __device__ int ordmode() const { return (int)p[RT_S_ORDMODE]; }
     __device__ bool n1spline() const { return p[RT_S_N1KIND] == (double)RT_FN_SPLINE; }
     __device__ bool n2spline() const { return p[RT_S_N2KIND] == (double)RT_FN_SPLINE; }
+    __device__ static constexpr bool n1reuse() { return false; }
     __device__ bool trans() const {
         return (p[RT_S_DIST + 2] != 0.0) || (p[RT_S_DIST + 0] != 0.0) || (p[RT_S_DIST + 1] != 0.0);
     }
@@ -92,6 +95,8 @@ struct StatSurf {
     __device__ static constexpr int ordmode() { return ORDM; }
     __device__ static constexpr bool n1spline() { return (FNMASK & 1) != 0; }
     __device__ static constexpr bool n2spline() { return (FNMASK & 2) != 0; }
+    // n1 is the n2 table of the previous surface's action: its value is reused
+    __device__ static constexpr bool n1reuse() { return (FNMASK & 4) != 0; }
     __device__ static constexpr bool trans() { return TRANS; }
     __device__ static constexpr bool rot(int j) { return ((ROTMASK >> j) & 1) != 0; }
     __device__ static constexpr bool ccd() { return CCD; }
@@ -145,6 +150,41 @@ __device__ __forceinline__ double fsqrt_with(double x, double ry) {
 }
 
 __device__ __forceinline__ double fsqrt(double x) { return fsqrt_with(x, rsqrt(x)); }
+
+// Variants for operands that the caller has shown to be finite and away from
+// zero (or whose special values end the caller's iteration the same way):
+// the same refinement without the special-value selects.
+__device__ __forceinline__ double rcp_fin(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    double q = fma(r, e, r);
+    e = fma(-x, q, 1.0);
+    return fma(q, e, q);
+}
+
+// a / b for b finite, nonzero; a may be any value (inf and NaN propagate)
+__device__ __forceinline__ double fdiv_fin(double a, double b) {
+    double r = rcp_fin(b);
+    double q = a * r;
+    double c = fma(fma(-b, q, a), r, q);
+    return __builtin_isfinite(q) ? c : q;
+}
+
+// 1/sqrt(x) for x > 0 finite; x <= 0 gives NaN (the Newton loops end on it)
+__device__ __forceinline__ double rsqrt_pos(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double h = 0.5 * x;
+    double e = fma(-h * y, y, 0.5);
+    double z = fma(y, e, y);
+    e = fma(-h * z, z, 0.5);
+    return fma(z, e, z);
+}
+
+__device__ __forceinline__ double fsqrt_pos(double x, double ry) {
+    double s = x * ry;
+    double t = fma(-s, s, x);
+    return fma(t, 0.5 * ry, s);
+}
 
 // ---------------------------------------------------------------- Philox
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
@@ -348,22 +388,51 @@ __device__ __forceinline__ void transform_ray(const SD& d, Ray& r) {
 // scaled by the largest component (7 divisions and a square root); direction
 // cosines are O(1), so the unscaled norm cannot overflow and one reciprocal
 // square root gives the same vector to about 1 ulp.
+//
+// Between stages the vector is already of unit length to rounding (rotations
+// and the reflection, Snell and grating laws preserve it), |c|^2 = 1 + d with
+// |d| ~ 1e-16: then 1/sqrt(1+d) = 1 - d/2 + 3d^2/8 - ..., and 1 - d/2 is
+// within 3d^2/8 < 4e-17 of it for |d| < 1e-8, below half an ulp.  A wave
+// whose rays are all that close takes that form; otherwise the reciprocal
+// square root.
 __device__ __forceinline__ void normalize_ray(double& cX, double& cY, double& cZ) {
-    double inv = rsqrt(cX * cX + cY * cY + cZ * cZ);
+    const double s = cX * cX + cY * cY + cZ * cZ;
+    const double d = s - 1.0;
+    const bool near = fabs(d) < 1e-8;
+    double inv;
+    if (__all(near)) inv = fma(-0.5, d, 1.0);
+    else inv = near ? fma(-0.5, d, 1.0) : rsqrt(s);
     cX *= inv; cY *= inv; cZ *= inv;
+}
+
+// The even-asphere polynomial and its derivative in r2 = r^2 by one Horner
+// pass over both: P = sum_{i=1..N} a_i r2^i, Q = sum_{i=1..N} i a_i r2^(i-1)
+// (the reference sums the same terms by ascending powers, surface.cu:45-61,
+// 257-333; the value differs only by rounding).
+template <class SD>
+__device__ __forceinline__ void asphere_pq(const SD& d, double r2, double& P, double& Q) {
+    const cptr p = d.p;
+    const int N = d.npoly();
+    if (N == 0) { P = 0.0; Q = 0.0; return; }
+    double pa = p[RT_S_A + N - 1], qa = p[D_IA + N - 1];
+    for (int i = N - 1; i >= 1; --i) {
+        pa = fma(pa, r2, p[RT_S_A + i - 1]);
+        qa = fma(qa, r2, p[D_IA + i - 1]);
+    }
+    P = pa * r2;
+    Q = qa;
 }
 
 // conic_derivs (surface.cu:45-61)
 template <class SD>
 __device__ __forceinline__ void conic_derivs(const SD& d, double X, double Y, double& dFdX, double& dFdY) {
     const cptr p = d.p;
-    const double c = p[RT_S_C], k1 = p[RT_S_K] + 1;
-    const int N = d.npoly();
+    const double c = p[RT_S_C];
     double r2 = X * X + Y * Y;
-    double E = c * rsqrt(1.0 - k1 * c * c * r2);
-    double sum = 0.0, rr = 1.0;
-    for (int i = 1; i <= N; ++i, rr *= r2) sum += (double)i * p[RT_S_A + i - 1] * rr;
-    E += 2.0 * sum;
+    double E = c * rsqrt(1.0 - p[D_K1C2] * r2);
+    double P, Q;
+    asphere_pq(d, r2, P, Q);
+    E += 2.0 * Q;
     dFdX = -E * X;
     dFdY = -E * Y;
 }
@@ -373,19 +442,16 @@ template <class SD>
 __device__ __forceinline__ void toric_derivs(const SD& d, double X, double Y, double Z,
                                              double& dFdX, double& dFdY, double& dFdZ) {
     const cptr p = d.p;
-    const double c = p[RT_S_C], k1 = p[RT_S_K] + 1, cR = p[RT_S_CR];
-    const int N = d.npoly();
+    const double c = p[RT_S_C], cR = p[RT_S_CR];
     double Y2 = Y * Y;
-    double t = 1.0 - k1 * c * c * Y2;
+    double t = 1.0 - p[D_K1C2] * Y2;
     double irt = rsqrt(t);
     double Fy = fdiv(c * Y2, 1.0 + fsqrt_with(t, irt));
-    double sum = 0.0, rr = Y2;
-    for (int i = 0; i < N; ++i, rr *= Y2) sum += p[RT_S_A + i] * rr;
-    Fy += sum;
+    double P, Q;
+    asphere_pq(d, Y2, P, Q);
+    Fy += P;
     double dFy = c * Y * irt;
-    sum = 0.0; rr = 1.0;
-    for (int i = 1; i <= N; ++i, rr *= Y2) sum += (double)i * p[RT_S_A + i - 1] * rr;
-    dFy += 2.0 * sum * Y;
+    dFy += 2.0 * Q * Y;
     dFdX = -cR * X;
     dFdY = (cR * Fy - 1.0) * dFy;
     dFdZ = 1.0 - cR * Z;
@@ -399,7 +465,6 @@ template <class SD>
 __device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& iters) {
     const cptr p = d.p;
     const int ik = d.ikind();
-    const int N = d.npoly();
     const double c = p[RT_S_C];
     iters = 0;
     if (ik == IK_PLANE) {
@@ -434,7 +499,10 @@ __device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& ite
     }
     if (ik == IK_CONIC_NEWTON) {
         // iterative_conic_intersection (surface.cu:257-333), counters per ray (D6)
-        const double k1 = p[RT_S_K] + 1;
+        // Special values inside the loop: t <= 0 (outside the conic) gives NaN
+        // through rsqrt_pos, and NaN or infinite F, dF end the loop at this
+        // iterate exactly as the IEEE forms do (s2 is not used after the loop).
+        const double K = p[D_K1C2];
         const double X0 = r.X, Y0 = r.Y, cX = r.cX, cY = r.cY, cZ = r.cZ;
         double s0 = fdiv(-r.Z, cZ);
         if (!(s0 >= 0)) return false;
@@ -447,19 +515,16 @@ __device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& ite
             Yi = Y0 + (s1 + s0) * cY;
             Zi = s1 * cZ;
             double r2 = Xi * Xi + Yi * Yi;
-            double t = 1.0 - k1 * c * c * r2;
-            double irt = rsqrt(t);
-            double F = Zi - fdiv(c * r2, 1.0 + fsqrt_with(t, irt));
-            double sum = 0.0, rr = r2;
-            for (int i = 0; i < N; ++i, rr *= r2) sum += p[RT_S_A + i] * rr;
-            F -= sum;
-            double E = c * irt;
-            sum = 0.0; rr = 1.0;
-            for (int i = 1; i <= N; ++i, rr *= r2) sum += (double)i * p[RT_S_A + i - 1] * rr;
-            E += 2.0 * sum;
+            double t = 1.0 - K * r2;
+            double irt = rsqrt_pos(t);
+            double F = Zi - (c * r2) * rcp_fin(1.0 + fsqrt_pos(t, irt));
+            double P, Q;
+            asphere_pq(d, r2, P, Q);
+            F -= P;
+            double E = c * irt + 2.0 * Q;
             double dF = -E * Xi * cX - E * Yi * cY + cZ;
             if (fabs(dF) < kNewtonEps) { alive = false; break; }
-            s2 = s1 - fdiv(F, dF);
+            s2 = s1 - fdiv_fin(F, dF);
             ++it;
         } while ((fabs(s2 - s1) >= kNewtonEps) && (it <= kNewtonMaxIter));
         r.X = Xi; r.Y = Yi; r.Z = Zi;
@@ -470,7 +535,7 @@ __device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& ite
     if (ik == IK_TORIC_NEWTON) {
         // iterative_toric_intersection (surface.cu:414-492) with the documented
         // F of surface.cpp:56-61 (D4) and the asphere derivative factor (D18)
-        const double k1 = p[RT_S_K] + 1, cR = p[RT_S_CR];
+        const double K = p[D_K1C2], cR = p[RT_S_CR];
         const double X0 = r.X, Y0 = r.Y, cX = r.cX, cY = r.cY, cZ = r.cZ;
         double s0 = fdiv(-r.Z, cZ);
         if (!(s0 >= 0)) return false;
@@ -483,20 +548,16 @@ __device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& ite
             Yi = Y0 + (s1 + s0) * cY;
             Zi = s1 * cZ;
             double Y2 = Yi * Yi;
-            double t = 1.0 - k1 * c * c * Y2;
-            double irt = rsqrt(t);
-            double Fy = fdiv(c * Y2, 1.0 + fsqrt_with(t, irt));
-            double sum = 0.0, rr = Y2;
-            for (int i = 0; i < N; ++i, rr *= Y2) sum += p[RT_S_A + i] * rr;
-            Fy += sum;
+            double t = 1.0 - K * Y2;
+            double irt = rsqrt_pos(t);
+            double P, Q;
+            asphere_pq(d, Y2, P, Q);
+            double Fy = (c * Y2) * rcp_fin(1.0 + fsqrt_pos(t, irt)) + P;
             double F = Zi - Fy - 0.5 * cR * (Xi * Xi + Zi * Zi - Fy * Fy);
-            double dFy = c * Yi * irt;
-            sum = 0.0; rr = 1.0;
-            for (int i = 1; i <= N; ++i, rr *= Y2) sum += (double)i * p[RT_S_A + i - 1] * rr;
-            dFy += 2.0 * sum * Yi;
+            double dFy = c * Yi * irt + 2.0 * Q * Yi;
             double dF = -cR * Xi * cX + (cR * Fy - 1.0) * dFy * cY + (1.0 - cR * Zi) * cZ;
             if (fabs(dF) < kNewtonEps) { alive = false; break; }
-            s2 = s1 - fdiv(F, dF);
+            s2 = s1 - fdiv_fin(F, dF);
             ++it;
         } while ((fabs(s2 - s1) >= kNewtonEps) && (it <= kNewtonMaxIter));
         r.X = Xi; r.Y = Yi; r.Z = Zi;
@@ -714,6 +775,7 @@ struct TraceCtx {
     double* flux;
     bool do_hist, do_flux;
     unsigned n_oob, n_hit, n_off, n_edge;
+    double n_prev;                // n2 of the last action (reused where the next n1 is the same table)
 };
 
 __device__ __forceinline__ void lds_add(unsigned* counter, unsigned v) {
@@ -752,8 +814,9 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
                 reflect_ray(d, r);
             } else if (d.type() == RT_LENS || d.type() == RT_GRATING) {
                 int e1 = 0, e2 = 0;
-                double n1 = fn_eval(c.pool, d.n1spline(), p[RT_S_N1VAL], r.lam, e1);
+                double n1 = d.n1reuse() ? c.n_prev : fn_eval(c.pool, d.n1spline(), p[RT_S_N1VAL], r.lam, e1);
                 double n2 = fn_eval(c.pool, d.n2spline(), p[RT_S_N2VAL], r.lam, e2);
+                c.n_prev = n2;
                 if (e1 | e2) {
                     alive = false;
                     ++c.n_oob;
@@ -811,6 +874,16 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
     }
 }
 
+// TraceArgs is the only argument of the trace kernels, at offset 0 of the
+// kernel argument segment; a pointer to it through the constant address
+// space, re-derived at each use (the empty asm keeps the loads at the use).
+typedef const __attribute__((address_space(4))) TraceArgs* cargs;
+__device__ __forceinline__ cargs kernel_args() {
+    cargs ka = (cargs)__builtin_amdgcn_kernarg_segment_ptr();
+    __asm__ volatile("" : "+s"(ka));
+    return ka;
+}
+
 __device__ __forceinline__ void dump_state(const TraceArgs& a, int s, const Ray& r, bool alive, uint64_t i) {
     const uint64_t n = a.n;
     double* d = a.dump + (size_t)s * 7 * n;
@@ -841,6 +914,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
     c.do_hist = (a.flags & RT_TRACE_HIST) && a.hist;
     c.do_flux = (a.flags & RT_TRACE_FLUX) && a.flux;
     c.n_oob = c.n_hit = c.n_off = c.n_edge = 0;
+    c.n_prev = 0.0;
     const bool store = (a.flags & RT_TRACE_STORE) != 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 
@@ -865,19 +939,23 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
             gen_ray<BEAM>(bm, *sd, a.ray_begin + i, r);
             alive = true;
         } else {
-            r.X = a.X[i]; r.Y = a.Y[i]; r.Z = a.Z[i];
-            r.cX = a.cX[i]; r.cY = a.cY[i]; r.cZ = a.cZ[i];
-            r.lam = a.L[i];
-            alive = a.F[i] != 0;
+            const cargs ka = kernel_args();
+            r.X = ka->X[i]; r.Y = ka->Y[i]; r.Z = ka->Z[i];
+            r.cX = ka->cX[i]; r.cY = ka->cY[i]; r.cZ = ka->cZ[i];
+            r.lam = ka->L[i];
+            alive = ka->F[i] != 0;
         }
         surfaces(r, alive, c, i);
         if (store || BEAM >= 0) {
-            if (a.X) {
-                a.X[i] = r.X; a.Y[i] = r.Y; a.Z[i] = r.Z;
-                a.cX[i] = r.cX; a.cY[i] = r.cY; a.cZ[i] = r.cZ;
-                a.L[i] = r.lam;
+            // the ray pointers are read from the argument segment where they are
+            // used, not held in SGPRs across the surface chain
+            const cargs ka = kernel_args();
+            if (ka->X) {
+                ka->X[i] = r.X; ka->Y[i] = r.Y; ka->Z[i] = r.Z;
+                ka->cX[i] = r.cX; ka->cY[i] = r.cY; ka->cZ[i] = r.cZ;
+                ka->L[i] = r.lam;
             }
-            if (a.F) a.F[i] = (signed char)alive;
+            if (ka->F) ka->F[i] = (signed char)alive;
         }
         unsigned long long good = __ballot(alive), all = __ballot(true);
         if (__lane_id() == 0) {

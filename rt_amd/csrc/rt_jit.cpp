@@ -48,14 +48,18 @@ int surface_ikind(const double* p) {
 
 std::string rt_jit_signature(const double* dtable, int n_surf) {
     std::ostringstream os;
+    double prev_n2 = -1.0;   // device pool offset of the n2 table the previous action evaluated
     for (int s = 0; s < n_surf; ++s) {
         const double* p = dtable + (size_t)s * rt::DEV_STRIDE;
         int type = (int)p[RT_S_TYPE];
         bool opt = (type == RT_LENS || type == RT_GRATING);
         int grt = type == RT_GRATING ? (int)p[RT_S_GRTYPE] : 0;
         int ordm = type == RT_GRATING ? (int)p[RT_S_ORDMODE] : 0;
-        int fnmask = opt ? ((p[RT_S_N1KIND] == (double)RT_FN_SPLINE ? 1 : 0) |
-                            (p[RT_S_N2KIND] == (double)RT_FN_SPLINE ? 2 : 0)) : 0;
+        bool n1s = p[RT_S_N1KIND] == (double)RT_FN_SPLINE, n2s = p[RT_S_N2KIND] == (double)RT_FN_SPLINE;
+        // bit 4: n1 is the table the previous non-auxiliary surface's action evaluated as
+        // its n2, for the same ray and wavelength, so its value is reused
+        int fnmask = opt ? ((n1s ? 1 : 0) | (n2s ? 2 : 0) | (n1s && p[RT_S_N1VAL] == prev_n2 ? 4 : 0)) : 0;
+        if (type != RT_AUXTYPE) prev_n2 = (opt && n2s) ? p[RT_S_N2VAL] : -1.0;
         int rot = (p[RT_S_ANG + 0] != 0.0 ? 1 : 0) | (p[RT_S_ANG + 1] != 0.0 ? 2 : 0) |
                   (p[RT_S_ANG + 2] != 0.0 ? 4 : 0);
         bool trans = p[RT_S_DIST + 0] != 0.0 || p[RT_S_DIST + 1] != 0.0 || p[RT_S_DIST + 2] != 0.0;
@@ -83,10 +87,12 @@ extern "C" int rt_hip_set_jit(int mode) {
     return prev;
 }
 
-static std::string generate_source(const std::string& sig, int beam_mode) {
+static std::string generate_source(const std::string& sig, int beam_mode, bool dump, int block, int waves) {
     std::ostringstream os;
     os << "#include \"rt_device.h\"\n"
-       << "extern \"C\" __global__ __launch_bounds__(rt::kBlock) void rt_trace_jit(rt::TraceArgs a) {\n"
+       << "extern \"C\" __global__ __launch_bounds__(" << block;
+    if (waves > 0) os << ", " << waves;      // minimum waves per SIMD the register allocation must allow
+    os << ") void rt_trace_jit(rt::TraceArgs a) {\n"
        << "  rt::trace_body<" << beam_mode << ">(a, [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, uint64_t i) {\n";
     size_t pos = 0;
     int s = 0;
@@ -94,8 +100,8 @@ static std::string generate_source(const std::string& sig, int beam_mode) {
         size_t end = sig.find(';', pos);
         std::string type = sig.substr(pos, end - pos);
         os << "    rt::surface_step(" << type << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s
-           << ", r, alive, c);\n"
-           << "    if (a.dump) rt::dump_state(a, " << s << ", r, alive, i);\n";
+           << ", r, alive, c);\n";
+        if (dump) os << "    rt::dump_state(a, " << s << ", r, alive, i);\n";
         pos = end + 1;
         ++s;
     }
@@ -148,14 +154,14 @@ static bool compile(const std::string& src, Entry& e) {
     return true;
 }
 
-hipFunction_t rt_jit_kernel(const std::string& sig, int beam_mode, int device) {
+hipFunction_t rt_jit_kernel(const std::string& sig, int beam_mode, bool dump, int block, int waves, int device) {
     std::ostringstream key;
-    key << device << "|" << beam_mode << "|" << sig;
+    key << device << "|" << beam_mode << "|" << block << "|" << waves << "|" << (dump ? "dump|" : "") << sig;
     std::lock_guard<std::mutex> lk(g_mu);
     Entry& e = g_cache[key.str()];
     if (e.fn) return e.fn;
     if (e.failed) return nullptr;
-    if (!compile(generate_source(sig, beam_mode), e)) {
+    if (!compile(generate_source(sig, beam_mode, dump, block, waves), e)) {
         e.failed = true;
         fprintf(stderr, "rt_hip: generated kernel failed to build, using the generic kernel:\n%s\n",
                 e.log.c_str());
@@ -168,7 +174,7 @@ hipFunction_t rt_jit_kernel(const std::string& sig, int beam_mode, int device) {
 // source and the compiler log are returned in `text`.  Host only.
 extern "C" int rt_jit_check_dtable(const double* dtable, int n_surf, int beam_mode, char* text, size_t len,
                                    size_t* code_size) {
-    std::string src = generate_source(rt_jit_signature(dtable, n_surf), beam_mode);
+    std::string src = generate_source(rt_jit_signature(dtable, n_surf), beam_mode, false, rt::kBlock, 0);
     std::vector<char> code;
     std::string log;
     bool ok = compile_code(src, code, log);

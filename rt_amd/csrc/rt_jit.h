@@ -12,8 +12,10 @@ std::string rt_jit_signature(const double* dtable, int n_surf);
 // rt_hip_set_jit overrides).
 bool rt_jit_enabled();
 
-// The kernel generated for (signature, beam mode) on `device`, compiled on
-// first use and cached for the life of the process; nullptr when the
-// runtime compiler is unavailable or fails (the caller then launches the
-// generic kernel, which reads the structure from the records).
-hipFunction_t rt_jit_kernel(const std::string& signature, int beam_mode, int device);
+// The kernel generated for (signature, beam mode, per-surface state dump or
+// not, lanes per workgroup, minimum waves per SIMD or 0) on `device`,
+// compiled on first use and cached for the life of the process; nullptr
+// when the runtime compiler is unavailable or fails (the caller then
+// launches the generic kernel, which reads the structure from the records).
+hipFunction_t rt_jit_kernel(const std::string& signature, int beam_mode, bool dump, int block, int waves,
+                            int device);

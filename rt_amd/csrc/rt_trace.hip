@@ -87,6 +87,10 @@ struct rt_hip_scheme {
     std::vector<double> pool;        // public spline pool, as given
     std::vector<double> dtable;      // device records
     std::vector<double> dpool;       // device spline records
+    // launch of the generated kernel (rt_hip_scheme_set_launch, rt_hip_autotune)
+    int block = rt::kBlock;          // lanes per workgroup
+    int waves = 0;                   // minimum waves per SIMD asked of the compiler (0: its choice)
+    int grid_mult = 2;               // persistent workgroups per CU = grid_mult x resident ones
 };
 
 namespace {
@@ -216,6 +220,8 @@ void rt_device_record(const double* p, double* d) {
     d[rt::D_INV_D] = p[RT_S_GRCONST] != 0.0 ? 1.0 / p[RT_S_GRCONST] : 0.0;
     d[rt::D_HALF_X] = p[RT_S_CCD_XSIZE] / 2.0;
     d[rt::D_HALF_Y] = p[RT_S_CCD_YSIZE] / 2.0;
+    d[rt::D_K1C2] = (p[RT_S_K] + 1) * p[RT_S_C] * p[RT_S_C];
+    for (int i = 1; i <= RT_MAX_POLY; ++i) d[rt::D_IA + i - 1] = (double)i * p[RT_S_A + i - 1];
 }
 
 // Device tables: each record gets its host-derived constants (rt::DevField)
@@ -385,21 +391,31 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
     a.hist = ccd_hist; a.flux = ccd_flux; a.stats = stats; a.dump = dump; a.dump_flag = dump_flag;
 
     const int cus = rt_device_cus();
-    uint64_t want = (n_rays + kBlock - 1) / kBlock;
-    uint64_t cap = (uint64_t)cus * 8;      // persistent grid: 8 blocks (32 waves) per CU
-    unsigned grid = (unsigned)(want < cap ? want : cap);
     size_t lds = sch->pool_len * sizeof(double);
     hipStream_t st = (hipStream_t)stream;
     const int beam_mode = gen ? (int)a.beam.v[RT_B_TYPE] : kLoad;
 
     hipFunction_t fn = nullptr;
-    if (rt_jit_enabled()) fn = rt_jit_kernel(sch->signature, beam_mode, sch->device);
+    if (rt_jit_enabled())
+        fn = rt_jit_kernel(sch->signature, beam_mode, dump != nullptr, sch->block, sch->waves, sch->device);
     if (fn) {
+        // persistent grid: the workgroups that are resident at once on every CU
+        // (from the compiled kernel's registers and LDS), times grid_mult
+        int resident = 0;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, sch->block, lds) != hipSuccess ||
+            resident < 1)
+            resident = 1;
+        uint64_t want = (n_rays + sch->block - 1) / sch->block;
+        uint64_t cap = (uint64_t)cus * resident * sch->grid_mult;
+        unsigned grid = (unsigned)(want < cap ? want : cap);
         void* params[] = {&a};
-        HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, kBlock, 1, 1, (unsigned)lds, st, params, nullptr),
+        HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, sch->block, 1, 1, (unsigned)lds, st, params, nullptr),
                 RT_ERR_FAILED);
         g_last_kernel = "generated";
     } else {
+        uint64_t want = (n_rays + kBlock - 1) / kBlock;
+        uint64_t cap = (uint64_t)cus * 8;      // persistent grid: 8 blocks (32 waves) per CU
+        unsigned grid = (unsigned)(want < cap ? want : cap);
         auto kern = beam_mode == RT_BEAM_PARALLEL ? rt_trace_kernel<RT_BEAM_PARALLEL>
                   : beam_mode == RT_BEAM_CONIC    ? rt_trace_kernel<RT_BEAM_CONIC>
                   : beam_mode == RT_BEAM_GAUSS    ? rt_trace_kernel<RT_BEAM_GAUSS>
@@ -408,5 +424,108 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
         g_last_kernel = "generic";
     }
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    return RT_OK;
+}
+
+// ------------------------------------------------------------ launch tuning
+
+extern "C" int rt_hip_scheme_set_launch(rt_hip_scheme* sch, int block, int waves, int grid_mult) {
+    if (!sch) return set_error(RT_ERR_BAD_VALUE, "scheme is NULL");
+    if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
+        return set_error(RT_ERR_BAD_VALUE, "block must be 64, 128, 256, 512 or 1024 lanes");
+    if (waves < 0 || waves > 8) return set_error(RT_ERR_BAD_VALUE, "waves must be in [0, 8]");
+    if (grid_mult < 1 || grid_mult > 16) return set_error(RT_ERR_BAD_VALUE, "grid_mult must be in [1, 16]");
+    sch->block = block;
+    sch->waves = waves;
+    sch->grid_mult = grid_mult;
+    return RT_OK;
+}
+
+extern "C" int rt_hip_scheme_get_launch(const rt_hip_scheme* sch, int* block, int* waves, int* grid_mult) {
+    if (!sch) return set_error(RT_ERR_BAD_VALUE, "scheme is NULL");
+    if (block) *block = sch->block;
+    if (waves) *waves = sch->waves;
+    if (grid_mult) *grid_mult = sch->grid_mult;
+    return RT_OK;
+}
+
+namespace {
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    bool alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8) == hipSuccess; }
+};
+}  // namespace
+
+// Time every candidate launch of the generated kernel on n_sample rays of the
+// beam (generated from `seed`, traced with `flags`) and keep the fastest.
+extern "C" int rt_hip_autotune(rt_hip_scheme* sch, const double* beam_desc, uint64_t seed, size_t n_sample,
+                               int flags, double* best_ms, void* stream) {
+    if (!sch) return set_error(RT_ERR_BAD_VALUE, "scheme is NULL");
+    if (!beam_desc) return set_error(RT_ERR_BAD_VALUE, "autotuning traces a generated beam: beam_desc is NULL");
+    if (n_sample == 0) return set_error(RT_ERR_BAD_VALUE, "n_sample must be > 0");
+    int rc = validate_beam(beam_desc);
+    if (rc != RT_OK) return rc;
+    rc = ensure_device();
+    if (rc != RT_OK) return rc;
+    flags |= RT_TRACE_GENERATE;
+    // buffers of the sample and of the detector frames
+    DevBuf rays, flag, hist, flux;
+    size_t npix = 0;
+    for (int s = 0; s < sch->n_surf; ++s) {
+        const double* p = sch->table.data() + (size_t)s * RT_SURF_STRIDE;
+        if ((int)p[RT_S_TYPE] == RT_DETECTOR && p[RT_S_CCD_XDIM] > 0.0)
+            npix = (size_t)p[RT_S_CCD_XDIM] * (size_t)p[RT_S_CCD_YDIM];
+    }
+    const bool store = (flags & RT_TRACE_STORE) != 0;
+    if ((store && (!rays.alloc(7 * n_sample * sizeof(double)) || !flag.alloc(n_sample))) ||
+        ((flags & RT_TRACE_HIST) && npix && !hist.alloc(npix * sizeof(unsigned))) ||
+        ((flags & RT_TRACE_FLUX) && npix && !flux.alloc(npix * sizeof(double))))
+        return set_error(RT_ERR_BAD_ALLOC, "autotune buffers");
+    double* R = (double*)rays.p;
+    auto run = [&]() {
+        return rt_hip_trace(sch, beam_desc, seed, 0, n_sample, store ? R : nullptr,
+                            store ? R + n_sample : nullptr, store ? R + 2 * n_sample : nullptr,
+                            store ? R + 3 * n_sample : nullptr, store ? R + 4 * n_sample : nullptr,
+                            store ? R + 5 * n_sample : nullptr, store ? R + 6 * n_sample : nullptr,
+                            store ? (signed char*)flag.p : nullptr, flags, (unsigned*)hist.p, (double*)flux.p,
+                            nullptr, nullptr, nullptr, stream);
+    };
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), RT_ERR_FAILED);
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return set_error(RT_ERR_FAILED, "hipEventCreate");
+    }
+    const int blocks[] = {128, 256};
+    const int wavess[] = {0, 5, 6};
+    const int mults[] = {1, 2};
+    double best = 1e300;
+    int bb = sch->block, bw = sch->waves, bm = sch->grid_mult;
+    rc = RT_OK;
+    for (int b : blocks)
+        for (int w : wavess)
+            for (int m : mults) {
+                sch->block = b; sch->waves = w; sch->grid_mult = m;
+                if ((rc = run()) != RT_OK) break;                      // first launch compiles
+                if (hipStreamSynchronize(st) != hipSuccess) { rc = set_error(RT_ERR_FAILED, "autotune"); break; }
+                float ms = 1e30f;
+                for (int k = 0; k < 3 && rc == RT_OK; ++k) {
+                    (void)hipEventRecord(e0, st);
+                    rc = run();
+                    (void)hipEventRecord(e1, st);
+                    if (hipEventSynchronize(e1) != hipSuccess) rc = set_error(RT_ERR_FAILED, "autotune");
+                    float t = 0.0f;
+                    if (rc == RT_OK && hipEventElapsedTime(&t, e0, e1) == hipSuccess && t < ms) ms = t;
+                }
+                if (rc != RT_OK) break;
+                if (ms < best) { best = ms; bb = b; bw = w; bm = m; }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    sch->block = bb; sch->waves = bw; sch->grid_mult = bm;
+    if (rc != RT_OK) return rc;
+    if (best_ms) *best_ms = best;
     return RT_OK;
 }

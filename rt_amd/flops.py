@@ -15,7 +15,7 @@ import numpy as np
 
 from . import scheme as S
 
-NORMALIZE = 13          # beam.cpp:541-569 as restated: 2x(mul,div,div) + 2 add + sqrt + mul + 3 div
+NORMALIZE = 9           # |c|^2 (3 mul, 2 add), one reciprocal square root, 3 mul
 PLANE_INT = 5           # s0 = -Z/cZ; X += cX s0; Y += cY s0
 CONIC_QUAD = 39         # A, B, C, D, sqrt, stable q, root, update
 SPLINE = 21             # splint with a binary search (no FP in the search)
@@ -32,8 +32,11 @@ def transform_cost(rec) -> int:
 
 
 def newton_iter_cost(rec) -> int:
+    """one Newton iteration: the point (6), sag with sqrt and divide, the
+    aspheric polynomial and its derivative by one Horner pass (4N - 3), dF/ds,
+    the update and the convergence test"""
     n = int(rec[S.S_NPOLY])
-    return (29 + 6 * n) if int(rec[S.S_CLASS]) == S.RT_CONIC else (41 + 6 * n)
+    return (27 + 4 * n) if int(rec[S.S_CLASS]) == S.RT_CONIC else (39 + 4 * n)
 
 
 def is_newton(rec) -> bool:
@@ -52,24 +55,26 @@ def intersect_cost(rec) -> int:
 
 def derivs_cost(rec) -> int:
     n = int(rec[S.S_NPOLY])
-    return (11 + 3 * n) if int(rec[S.S_CLASS]) == S.RT_CONIC else (19 + 6 * n)
+    return (9 + 2 * n) if int(rec[S.S_CLASS]) == S.RT_CONIC else (20 + 4 * n)
 
 
 def fn_cost(kind) -> int:
     return SPLINE if kind == S.RT_FN_SPLINE else 0
 
 
-def action_cost(rec) -> int:
+def action_cost(rec, n1_reused: bool = False) -> int:
+    """reflection / refraction / diffraction with the surface normal; n1(lambda)
+    is not counted when the previous action already evaluated the same table"""
     t, cls, n = int(rec[S.S_TYPE]), int(rec[S.S_CLASS]), int(rec[S.S_NPOLY])
     if t == S.RT_MIRROR:
         if cls == S.RT_PLANE:
             return 0
-        return (26 + 3 * n) if cls == S.RT_CONIC else (37 + 6 * n)
-    idx = fn_cost(rec[S.S_N1KIND]) + fn_cost(rec[S.S_N2KIND]) + 1
+        return (26 + 2 * n) if cls == S.RT_CONIC else (37 + 4 * n)
+    idx = (0 if n1_reused else fn_cost(rec[S.S_N1KIND])) + fn_cost(rec[S.S_N2KIND]) + 1
     if t == S.RT_LENS:
         if cls == S.RT_PLANE:
             return idx + 11
-        return idx + ((36 + 3 * n) if cls == S.RT_CONIC else (47 + 6 * n))
+        return idx + ((36 + 2 * n) if cls == S.RT_CONIC else (47 + 4 * n))
     if t == S.RT_GRATING:
         refl = int(rec[S.S_GRTYPE]) == S.RT_REFLECTIVE
         order = 2 if int(rec[S.S_ORDMODE]) == S.RT_ORDER_BLAZE else 0
@@ -77,23 +82,41 @@ def action_cost(rec) -> int:
         if cls == S.RT_PLANE:
             return base + (10 if refl else 19)
         if cls == S.RT_CONIC:
-            return base + (55 + 3 * n if refl else 63 + 3 * n)
-        return base + (68 + 6 * n if refl else 76 + 6 * n)
+            return base + (55 + 2 * n if refl else 63 + 2 * n)
+        return base + (68 + 4 * n if refl else 76 + 4 * n)
     return 0
+
+
+def n1_reuse(table) -> list:
+    """per surface: does its action take n1 from the previous action's n2 (the
+    same table at the same wavelength)?  The rule of the kernel generator."""
+    out, prev = [], None
+    for rec in table:
+        t = int(rec[S.S_TYPE])
+        opt = t in (S.RT_LENS, S.RT_GRATING)
+        n1s = rec[S.S_N1KIND] == S.RT_FN_SPLINE
+        out.append(bool(opt and n1s and prev is not None and rec[S.S_N1VAL] == prev))
+        if t != S.RT_AUXTYPE:
+            prev = rec[S.S_N2VAL] if (opt and rec[S.S_N2KIND] == S.RT_FN_SPLINE) else None
+    return out
 
 
 def mask_cost(rec) -> int:
     return 5 if int(rec[S.S_SHAPE]) == S.RT_CIRCLE else 0
 
 
-def trace_flops(scheme: "S.Scheme", stats: np.ndarray, generated: bool = True) -> float:
-    """Algorithmic FP64 operations of the trace(s) that produced `stats`."""
+def trace_flops(scheme: "S.Scheme", stats: np.ndarray, generated: bool = True,
+                generated_kernel: bool = True) -> float:
+    """Algorithmic FP64 operations of the trace(s) that produced `stats`
+    (`generated`: the beam was generated in the kernel; `generated_kernel`:
+    the per-scheme kernel ran, which reuses repeated refractive indices)."""
     t = np.asarray(scheme.table).reshape(scheme.n_surf, S.SURF_STRIDE)
     st = np.asarray(stats).astype(np.float64)
     total = 0.0
     if generated:
         b = scheme.beam
         total += st[S.ST_RAYS] * (BEAM_POS_CIRCLE + BEAM_DIR[b.type] + BEAM_LAMBDA)
+    reuse = n1_reuse(t) if generated_kernel else [False] * scheme.n_surf
     for s in range(scheme.n_surf):
         rec = t[s]
         if int(rec[S.S_TYPE]) == S.RT_AUXTYPE:
@@ -106,7 +129,7 @@ def trace_flops(scheme: "S.Scheme", stats: np.ndarray, generated: bool = True) -
         if is_newton(rec):
             total += iters * newton_iter_cost(rec)
         total += after_int * mask_cost(rec)
-        total += after_act * (action_cost(rec) + NORMALIZE)
+        total += after_act * (action_cost(rec, reuse[s]) + NORMALIZE)
         if int(rec[S.S_TYPE]) == S.RT_DETECTOR and rec[S.S_CCD_XDIM] > 0:
             total += after_act * CCD_MAP + st[S.ST_CCD_HITS] * FLUX
     return total

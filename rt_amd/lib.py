@@ -42,6 +42,9 @@ def lib():
         L.rt_hip_trace.argtypes = ([_vp, _dp, C.c_uint64, C.c_size_t, C.c_size_t] + [_vp] * 8 +
                                    [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp])
         L.rt_hip_set_jit.argtypes = [C.c_int]
+        L.rt_hip_scheme_set_launch.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+        L.rt_hip_scheme_get_launch.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.rt_hip_autotune.argtypes = [_vp, _dp, C.c_uint64, C.c_size_t, C.c_int, C.POINTER(C.c_double), _vp]
         L.rt_hip_last_kernel.argtypes = [C.c_char_p, C.c_size_t]
         L.rt_hip_jit_check.argtypes = [_dp, C.c_int, _dp, C.c_size_t, C.c_int, C.c_char_p, C.c_size_t,
                                        C.POINTER(C.c_size_t)]
@@ -143,6 +146,28 @@ class DeviceScheme:
         rc = lib().rt_hip_trace(self.handle, _dptr(b), seed, begin, n, *ptrs, _tp(flag), flags,
                                 _tp(hist), _tp(flux), _tp(stats), _tp(dump), _tp(dump_flag), stream)
         check(rc, "rt_hip_trace")
+
+    def set_launch(self, block: int = 256, waves: int = 0, grid_mult: int = 2):
+        """Launch of the generated kernel (rt_hip_scheme_set_launch)."""
+        check(lib().rt_hip_scheme_set_launch(self.handle, block, waves, grid_mult), "rt_hip_scheme_set_launch")
+
+    def get_launch(self):
+        b, w, m = C.c_int(), C.c_int(), C.c_int()
+        check(lib().rt_hip_scheme_get_launch(self.handle, C.byref(b), C.byref(w), C.byref(m)),
+              "rt_hip_scheme_get_launch")
+        return b.value, w.value, m.value
+
+    def autotune(self, n_sample: int, flags: int, seed: int = 0, beam=None, stream=None) -> float:
+        """Time the candidate launches on n_sample generated rays and keep the
+        fastest (rt_hip_autotune); returns its time in ms."""
+        import torch
+        b = np.ascontiguousarray((beam if beam is not None else self.scheme.beam).pack())
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        ms = C.c_double(0.0)
+        check(lib().rt_hip_autotune(self.handle, _dptr(b), seed, n_sample, flags, C.byref(ms), stream),
+              "rt_hip_autotune")
+        return ms.value
 
 
 def set_jit(on: bool) -> bool:

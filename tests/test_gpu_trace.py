@@ -191,3 +191,35 @@ def test_full_size_echelle_properties(hip):
         assert np.array_equal(f, ref.flag)
         assert rel_err(g, ref.rays, ref.flag == 1, Lsc) <= RTOL
     ds.close()
+
+
+def test_launch_configurations_agree_and_autotune(hip):
+    """Every launch of the generated kernel computes the same rays: the launch
+    (lanes per workgroup, waves per SIMD, grid size) changes only the schedule."""
+    sch = schemes.echelle(n_per_lambda=8, nlambda=512)
+    base = gpu_trace(hip, sch, seed=4, dump=False, expect_kernel="generated")
+    for block, waves, mult in [(128, 5, 1), (256, 6, 2), (512, 0, 1), (64, 0, 4)]:
+        ds = hip.DeviceScheme(sch)
+        ds.set_launch(block, waves, mult)
+        assert ds.get_launch() == (block, waves, mult)
+        n = sch.beam.n_rays
+        R = torch.zeros((7, n), dtype=torch.float64, device="cuda")
+        F = torch.zeros(n, dtype=torch.int8, device="cuda")
+        det = sch.detector()
+        H = torch.zeros(det.ccd["ydim"] * det.ccd["xdim"], dtype=torch.int32, device="cuda")
+        st = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
+        print(f"[launch] block={block} waves={waves} grid_mult={mult}", flush=True)
+        ds.trace(R, F, seed=4, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE | S.RT_TRACE_HIST, hist=H, stats=st)
+        torch.cuda.synchronize()
+        assert hip.last_kernel() == "generated"
+        assert np.array_equal(R.cpu().numpy(), base["rays"]), (block, waves, mult)
+        assert np.array_equal(F.cpu().numpy(), base["flag"])
+        assert np.array_equal(H.cpu().numpy().view(np.uint32), base["hist"])
+        assert np.array_equal(st.cpu().numpy().view(np.uint64), base["stats"])
+        ds.close()
+    ds = hip.DeviceScheme(sch)
+    print("[launch] autotune", flush=True)
+    ms = ds.autotune(n_sample=200000, flags=S.RT_TRACE_STORE | S.RT_TRACE_HIST | S.RT_TRACE_FLUX)
+    b, w, m = ds.get_launch()
+    assert ms > 0 and b in (128, 256) and w in (0, 5, 6) and m in (1, 2)
+    ds.close()
