@@ -782,6 +782,29 @@ __device__ __forceinline__ void lds_add(unsigned* counter, unsigned v) {
     if (v) atomicAdd(counter, v);
 }
 
+// Hit counts pre-aggregated within the wave before the global atomics: the
+// first pending lane's pixel takes every lane of the wave that hit the same
+// pixel in one atomic add.  The peel is bounded: it stops after kPeelRounds
+// rounds, or after a round that merged a single lane (hits scattered over the
+// wave, e.g. one wavelength per lane), and the lanes left add their own hit.
+constexpr int kPeelRounds = 8;
+
+__device__ __forceinline__ void hist_add_wave(unsigned* hist, uint32_t pix, bool on) {
+    unsigned long long pending = __ballot(on);
+    bool mine = on;
+    for (int round = 0; round < kPeelRounds && pending; ++round) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t lp = (uint32_t)__builtin_amdgcn_readlane((int)pix, leader);
+        const bool match = mine && pix == lp;
+        const unsigned long long m = __ballot(match);
+        if ((int)__lane_id() == leader) atomicAdd(&hist[lp], (unsigned)__popcll(m));
+        mine = mine && !match;
+        pending &= ~m;
+        if (__popcll(m) == 1) break;
+    }
+    if (mine) atomicAdd(&hist[pix], 1u);
+}
+
 // One surface of the Scheme::run stage loop (common-lib/scheme.cpp:983-1046)
 // for one ray: transform, normalize, intersection, aperture mask, action
 // (reflection, refraction or diffraction), normalize, then the CCD frame on
@@ -843,34 +866,40 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
         if (li) atomicAdd(&c.st[RT_ST_SURF + 4 * s + 2], (unsigned)__popcll(li));
         if (la) atomicAdd(&c.st[RT_ST_SURF + 4 * s + 3], (unsigned)__popcll(la));
     }
-    if (d.ccd() && (c.do_hist || c.do_flux) && alive) {
-        // ccd_coordinates_kernel (rt_cuda_engine_ccd.cu:30-45) with a bounds check (D10)
+    if (d.ccd() && (c.do_hist || c.do_flux)) {
+        // ccd_coordinates_kernel (rt_cuda_engine_ccd.cu:30-45) with a bounds check (D10);
+        // xdim * ydim <= 2^32 (validated), so a pixel index fits 32 bits
         const uint32_t xdim = (uint32_t)p[RT_S_CCD_XDIM], ydim = (uint32_t)p[RT_S_CCD_YDIM];
-        double vx = (r.X + p[D_HALF_X]) / p[RT_S_CCD_XPIX];
-        double vy = (r.Y + p[D_HALF_Y]) / p[RT_S_CCD_YPIX];
-        uint32_t px = trunc_u32(vx), py = trunc_u32(vy);
-        if (px < xdim && py < ydim) {
-            size_t pix = (size_t)py * xdim + px;
-            ++c.n_hit;
-            if (near_edge(vx) || near_edge(vy)) ++c.n_edge;
-            if (c.do_hist) atomicAdd(&c.hist[pix], 1u);
-            if (c.do_flux) {
-                // compute_ccd_image_kernel (ccd.cu:219-232), weights of ccd.cu:286-287,360
-                int eq = 0, es = 0;
-                double qe = fn_eval(c.pool, p[RT_S_QEKIND] == (double)RT_FN_SPLINE, p[RT_S_QEVAL], r.lam, eq);
-                double sp = fn_eval(c.pool, p[RT_S_SPKIND] == (double)RT_FN_SPLINE, p[RT_S_SPVAL], r.lam, es);
-                if (eq | es) {
-                    ++c.n_oob;
-                } else {
-                    double spec = sp * r.lam;
-                    double pix_area = p[RT_S_CCD_XPIX] * p[RT_S_CCD_YPIX] * 1.0E-2;
-                    double w = spec * (1.0 / kHC) * qe * pix_area;
-                    unsafeAtomicAdd(&c.flux[pix], w);
+        bool on = false;
+        uint32_t pix = 0;
+        if (alive) {
+            double vx = (r.X + p[D_HALF_X]) / p[RT_S_CCD_XPIX];
+            double vy = (r.Y + p[D_HALF_Y]) / p[RT_S_CCD_YPIX];
+            uint32_t px = trunc_u32(vx), py = trunc_u32(vy);
+            on = px < xdim && py < ydim;
+            if (on) {
+                pix = py * xdim + px;
+                ++c.n_hit;
+                if (near_edge(vx) || near_edge(vy)) ++c.n_edge;
+                if (c.do_flux) {
+                    // compute_ccd_image_kernel (ccd.cu:219-232), weights of ccd.cu:286-287,360
+                    int eq = 0, es = 0;
+                    double qe = fn_eval(c.pool, p[RT_S_QEKIND] == (double)RT_FN_SPLINE, p[RT_S_QEVAL], r.lam, eq);
+                    double sp = fn_eval(c.pool, p[RT_S_SPKIND] == (double)RT_FN_SPLINE, p[RT_S_SPVAL], r.lam, es);
+                    if (eq | es) {
+                        ++c.n_oob;
+                    } else {
+                        double spec = sp * r.lam;
+                        double pix_area = p[RT_S_CCD_XPIX] * p[RT_S_CCD_YPIX] * 1.0E-2;
+                        double w = spec * (1.0 / kHC) * qe * pix_area;
+                        unsafeAtomicAdd(&c.flux[pix], w);
+                    }
                 }
+            } else {
+                ++c.n_off;
             }
-        } else {
-            ++c.n_off;
         }
+        if (c.do_hist) hist_add_wave(c.hist, pix, on);
     }
 }
 

@@ -144,11 +144,9 @@ __global__ __launch_bounds__(kThreads) void rt_ccd_kernel(CcdArgs a) {
         } else {
             uint32_t px = a.ccdX[i], py = a.ccdY[i];
             bool off = !(px < a.xdim && py < a.ydim);
-            if (!off) {
-                size_t pix = (size_t)py * a.xdim + px;
-                if (a.image) unsafeAtomicAdd(&a.image[pix], a.w ? a.w[i] : 1.0);
-                if (a.hist) atomicAdd(&a.hist[pix], 1u);
-            }
+            uint32_t pix = off ? 0u : py * a.xdim + px;     // xdim * ydim <= 2^32 (checked on the host)
+            if (!off && a.image) unsafeAtomicAdd(&a.image[pix], a.w ? a.w[i] : 1.0);
+            if (a.hist) rt::hist_add_wave(a.hist, pix, !off);
             unsigned long long b = __ballot(off);
             if (__lane_id() == 0 && b && a.offchip) atomicAdd(a.offchip, (unsigned long long)__popcll(b));
         }
@@ -389,7 +387,8 @@ extern "C" int rt_hip_ccd_accumulate(size_t n, const unsigned* ccdX, const unsig
                                      size_t xdim, size_t ydim, double* image, unsigned* hist,
                                      unsigned long long* offchip, void* stream) {
     if (!ccdX || !ccdY || (!image && !hist)) return rt_set_error(RT_ERR_BAD_VALUE, "NULL argument");
-    if (xdim == 0 || ydim == 0 || xdim > 0xffffffffu || ydim > 0xffffffffu)
+    if (xdim == 0 || ydim == 0 || xdim > 0xffffffffu || ydim > 0xffffffffu ||
+        (unsigned long long)xdim * ydim > 4294967296ull)
         return rt_set_error(RT_ERR_BAD_VALUE, "bad CCD dimensions");
     if (n == 0) return RT_OK;
     int rc = rt_ensure_device();

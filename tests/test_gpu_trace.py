@@ -223,3 +223,18 @@ def test_launch_configurations_agree_and_autotune(hip):
     b, w, m = ds.get_launch()
     assert ms > 0 and b in (128, 256) and w in (0, 5, 6) and m in (1, 2)
     ds.close()
+
+
+def test_clustered_ccd_hits_are_counted_exactly(hip, kernel):
+    """A 0.05 mm beam puts each wavelength's rays on one or two pixels, so every
+    wave's hits merge before the atomics (the pre-aggregation path); the
+    frame must still equal the oracle's count for count."""
+    sch = schemes.czerny_turner(n_per_lambda=4000, nlambda=5)
+    sch.beam.params = (4000, 0.0, 0.05)
+    g = gpu_trace(hip, sch, seed=2, dump=False, expect_kernel=kernel)
+    ref = O.trace(sch, seed=2, hist=True)
+    assert np.array_equal(g["flag"], ref.flag)
+    assert np.array_equal(g["hist"], ref.hist), "CCD hit counts differ"
+    assert np.array_equal(g["stats"], ref.stats)
+    nz = np.count_nonzero(ref.hist)
+    assert 0 < nz <= 40 and ref.hist.sum() > 0.9 * sch.beam.n_rays, nz
