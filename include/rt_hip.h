@@ -277,13 +277,58 @@ int rt_hip_scheme_get_launch(const rt_hip_scheme *scheme, int *block, int *waves
 int rt_hip_autotune(rt_hip_scheme *scheme, const double *beam_desc, uint64_t seed, size_t n_sample,
                     int flags, double *best_ms, void *stream);
 
+/* ---- batched traces ---------------------------------------------------- */
+
+/* Many instances of one scheme structure (same classes, types, shapes,
+   polynomial counts, tables; any parameter values, e.g. Monte-Carlo
+   tolerance perturbations) with uneven ray counts, traced in one persistent
+   launch.  tables: n_inst packed tables of n_surf records; one spline pool
+   for all; ray_counts[n_inst].  Instance i's beam is generated from
+   seed + i, rays 0 .. ray_counts[i]-1, exactly as rt_hip_trace would for
+   that instance alone.  No counterpart in the reference, which traces one
+   scheme per run (common-lib/scheme.cpp:878-1099). */
+typedef struct rt_hip_batch rt_hip_batch;
+enum { RT_SPOT_STRIDE = 6 };   /* per instance: n, centroid x, y, variance x, y, RMS radius */
+
+int rt_hip_batch_create(const double *tables, int n_inst, int n_surf, const double *spline_pool,
+                        size_t pool_len, const uint64_t *ray_counts, rt_hip_batch **out);
+/* new parameter values for every instance (same structure and pool layout) */
+int rt_hip_batch_update(rt_hip_batch *batch, const double *tables, const double *spline_pool,
+                        size_t pool_len);
+int rt_hip_batch_destroy(rt_hip_batch *batch);
+int rt_hip_batch_info(const rt_hip_batch *batch, uint64_t *total_rays, uint64_t *n_tiles);
+int rt_hip_batch_set_launch(rt_hip_batch *batch, int block, int waves, int grid_mult);
+/* Trace every instance (flags: RT_TRACE_STORE optional; CCD frames are not
+   formed in batches).  With STORE the rays go to X..flag concatenated by
+   instance.  spots (device, n_inst x RT_SPOT_STRIDE) receives each
+   instance's spot on the last surface from its surviving rays, combined in a
+   fixed order (bit-reproducible).  stats is accumulated over all instances. */
+int rt_hip_batch_trace(const rt_hip_batch *batch, const double *beam_desc, uint64_t seed, int flags,
+                       double *X, double *Y, double *Z, double *cX, double *cY, double *cZ,
+                       double *lambda, signed char *flag, double *spots,
+                       unsigned long long *stats, void *stream);
+
+/* The same batched trace recorded as a hipGraph on `stream` (not the null
+   stream) and replayed with rt_hip_graph_launch; table contents may change
+   between replays (rt_hip_batch_update).  The graph holds no memset nodes:
+   stats keep accumulating over replays. */
+typedef struct rt_hip_graph rt_hip_graph;
+int rt_hip_batch_graph_create(const rt_hip_batch *batch, const double *beam_desc, uint64_t seed, int flags,
+                              double *X, double *Y, double *Z, double *cX, double *cY, double *cZ,
+                              double *lambda, signed char *flag, double *spots,
+                              unsigned long long *stats, void *stream, rt_hip_graph **out);
+int rt_hip_graph_launch(rt_hip_graph *graph, void *stream);
+int rt_hip_graph_destroy(rt_hip_graph *graph);
+
 /* "generated" or "generic": the kernel the calling thread's last trace used. */
 int rt_hip_last_kernel(char *buf, size_t len);
 
 /* Generate and compile (without loading, no device needed) the kernel of a
-   packed scheme for beam mode (-1: rays given, else RT_BEAM_*); `text`
-   receives the generated source and the compiler log, *code_size the code
-   object size.  RT_OK when it compiles. */
+   packed scheme for beam mode (-1: rays given, else RT_BEAM_*, or'ed with
+   RT_JIT_BATCH for the batched kernel); `text` receives the generated source
+   and the compiler log, *code_size the code object size.  RT_OK when it
+   compiles. */
+enum { RT_JIT_BATCH = 0x100 };
 int rt_hip_jit_check(const double *surf_table, int n_surf, const double *spline_pool,
                      size_t pool_len, int beam_mode, char *text, size_t len, size_t *code_size);
 

@@ -765,6 +765,14 @@ struct TraceArgs {
     unsigned long long* stats;
     double* dump;
     signed char* dump_flag;
+    // batched traces (trace_batch_body): instance i owns records
+    // table[i * inst_stride ..], rays [ray_start[i], ray_start[i+1]) of the
+    // concatenated outputs and tiles [tile_start[i], tile_start[i+1]) of one
+    // workgroup each; its beam is generated from seed + i
+    const uint64_t* tile_start;
+    const uint64_t* ray_start;
+    double* tile_moments;         // per tile: n, mean x, mean y, M2 x, M2 y of the rays alive at the end
+    uint64_t n_inst, n_tiles, inst_stride;
 };
 
 struct TraceCtx {
@@ -996,6 +1004,112 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
     lds_add(&s_stats[RT_ST_CCD_HITS], c.n_hit);
     lds_add(&s_stats[RT_ST_CCD_OFFCHIP], c.n_off);
     lds_add(&s_stats[RT_ST_CCD_EDGE], c.n_edge);
+    __syncthreads();
+    if (a.stats) {
+        for (int k = threadIdx.x; k < RT_STATS_LEN; k += blockDim.x)
+            if (s_stats[k]) atomicAdd(&a.stats[k], (unsigned long long)s_stats[k]);
+    }
+}
+
+// Sum over the workgroup of K values per lane, the same in every lane: a
+// butterfly within each wave, then the wave partials in wave order, so the
+// result is the same bit for bit on every run.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double* s_red) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (__lane_id() == 0)
+        for (int k = 0; k < K; ++k) s_red[w * K + k] = v[k];
+    __syncthreads();
+    for (int k = 0; k < K; ++k) {
+        double t = 0.0;
+        for (int j = 0; j < nw; ++j) t += s_red[j * K + k];
+        v[k] = t;
+    }
+}
+
+// Batched kernel body: a persistent grid walks workgroup tiles; a tile holds
+// up to blockDim rays of one instance, so that instance's surface records are
+// wave-uniform (scalar loads) while instances of uneven ray counts share one
+// launch.  Each tile leaves the spot moments of its surviving rays (position on
+// the last surface) for a fixed-order per-instance combination afterwards.
+template <int BEAM, class Surfaces>
+__device__ __forceinline__ void trace_batch_body(const TraceArgs& a, Surfaces surfaces) {
+    extern __shared__ double s_pool[];
+    __shared__ unsigned s_stats[RT_STATS_LEN];
+    __shared__ double s_red[16 * 3];
+    for (int i = threadIdx.x; i < RT_STATS_LEN; i += blockDim.x) s_stats[i] = 0u;
+    for (int i = threadIdx.x; i < a.pool_len; i += blockDim.x) s_pool[i] = a.pool[i];
+    __syncthreads();
+
+    TraceCtx c;
+    c.pool = s_pool;
+    c.st = s_stats;
+    c.hist = nullptr;
+    c.flux = nullptr;
+    c.do_hist = c.do_flux = false;
+    c.n_oob = c.n_hit = c.n_off = c.n_edge = 0;
+    c.n_prev = 0.0;
+    const bool store = (a.flags & RT_TRACE_STORE) != 0;
+
+    for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+        // instance of tile t: the last i with tile_start[i] <= t (uniform)
+        const cargs ka = kernel_args();
+        const cptr64 ts = (cptr64)ka->tile_start;
+        uint64_t lo = 0, hi = a.n_inst;
+        while (hi - lo > 1) {
+            uint64_t mid = (lo + hi) >> 1;
+            if (ts[mid] <= t) lo = mid; else hi = mid;
+        }
+        const uint64_t inst = lo;
+        const cptr64 rs = (cptr64)ka->ray_start;
+        const uint64_t r0 = rs[inst], n_i = rs[inst + 1] - r0;
+        const uint64_t r = (t - ts[inst]) * blockDim.x + threadIdx.x;
+        const bool valid = r < n_i;
+        cptr tbl = (cptr)a.table + inst * a.inst_stride;
+        __asm__ volatile("" : "+s"(tbl));
+        c.table = tbl;
+        Ray ray;
+        bool alive = valid;
+        if (valid) {
+            cptr bm = (cptr)((const __attribute__((address_space(4))) char*)ka + __builtin_offsetof(TraceArgs, beam));
+            gen_ray<BEAM>(bm, ka->seed + inst, r, ray);
+        } else {
+            ray.X = ray.Y = ray.Z = ray.cX = ray.cY = ray.lam = 0.0;
+            ray.cZ = 1.0;
+        }
+        surfaces(ray, alive, c, r0 + r);
+        if (store && valid) {
+            const cargs kb = kernel_args();
+            const uint64_t g = r0 + r;
+            if (kb->X) {
+                kb->X[g] = ray.X; kb->Y[g] = ray.Y; kb->Z[g] = ray.Z;
+                kb->cX[g] = ray.cX; kb->cY[g] = ray.cY; kb->cZ[g] = ray.cZ;
+                kb->L[g] = ray.lam;
+            }
+            if (kb->F) kb->F[g] = (signed char)alive;
+        }
+        unsigned long long good = __ballot(alive), all = __ballot(valid);
+        if (__lane_id() == 0) {
+            if (all) atomicAdd(&s_stats[RT_ST_RAYS], (unsigned)__popcll(all));
+            if (good) atomicAdd(&s_stats[RT_ST_GOOD], (unsigned)__popcll(good));
+        }
+        // spot moments of the tile: count and means, then squared deviations
+        double m1[3] = {alive ? 1.0 : 0.0, alive ? ray.X : 0.0, alive ? ray.Y : 0.0};
+        block_sum<3>(m1, s_red);
+        const double cnt = m1[0];
+        const double mx = cnt > 0.0 ? m1[1] / cnt : 0.0, my = cnt > 0.0 ? m1[2] / cnt : 0.0;
+        double m2[2] = {alive ? (ray.X - mx) * (ray.X - mx) : 0.0, alive ? (ray.Y - my) * (ray.Y - my) : 0.0};
+        block_sum<2>(m2, s_red);
+        if (threadIdx.x == 0 && a.tile_moments) {
+            double* o = a.tile_moments + t * 5;
+            o[0] = cnt; o[1] = mx; o[2] = my; o[3] = m2[0]; o[4] = m2[1];
+        }
+    }
+    lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
     __syncthreads();
     if (a.stats) {
         for (int k = threadIdx.x; k < RT_STATS_LEN; k += blockDim.x)

@@ -87,13 +87,15 @@ extern "C" int rt_hip_set_jit(int mode) {
     return prev;
 }
 
-static std::string generate_source(const std::string& sig, int beam_mode, bool dump, int block, int waves) {
+static std::string generate_source(const std::string& sig, int beam_mode, bool dump, int block, int waves,
+                                   bool batch = false) {
     std::ostringstream os;
     os << "#include \"rt_device.h\"\n"
        << "extern \"C\" __global__ __launch_bounds__(" << block;
     if (waves > 0) os << ", " << waves;      // minimum waves per SIMD the register allocation must allow
     os << ") void rt_trace_jit(rt::TraceArgs a) {\n"
-       << "  rt::trace_body<" << beam_mode << ">(a, [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, uint64_t i) {\n";
+       << "  rt::" << (batch ? "trace_batch_body<" : "trace_body<") << beam_mode
+       << ">(a, [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, uint64_t i) {\n";
     size_t pos = 0;
     int s = 0;
     while (pos < sig.size()) {
@@ -154,14 +156,16 @@ static bool compile(const std::string& src, Entry& e) {
     return true;
 }
 
-hipFunction_t rt_jit_kernel(const std::string& sig, int beam_mode, bool dump, int block, int waves, int device) {
+hipFunction_t rt_jit_kernel(const std::string& sig, int beam_mode, bool dump, int block, int waves, int device,
+                            bool batch) {
     std::ostringstream key;
-    key << device << "|" << beam_mode << "|" << block << "|" << waves << "|" << (dump ? "dump|" : "") << sig;
+    key << device << "|" << beam_mode << "|" << block << "|" << waves << "|" << (dump ? "dump|" : "")
+        << (batch ? "batch|" : "") << sig;
     std::lock_guard<std::mutex> lk(g_mu);
     Entry& e = g_cache[key.str()];
     if (e.fn) return e.fn;
     if (e.failed) return nullptr;
-    if (!compile(generate_source(sig, beam_mode, dump, block, waves), e)) {
+    if (!compile(generate_source(sig, beam_mode, dump, block, waves, batch), e)) {
         e.failed = true;
         fprintf(stderr, "rt_hip: generated kernel failed to build, using the generic kernel:\n%s\n",
                 e.log.c_str());
@@ -174,7 +178,9 @@ hipFunction_t rt_jit_kernel(const std::string& sig, int beam_mode, bool dump, in
 // source and the compiler log are returned in `text`.  Host only.
 extern "C" int rt_jit_check_dtable(const double* dtable, int n_surf, int beam_mode, char* text, size_t len,
                                    size_t* code_size) {
-    std::string src = generate_source(rt_jit_signature(dtable, n_surf), beam_mode, false, rt::kBlock, 0);
+    const bool batch = (beam_mode & RT_JIT_BATCH) != 0;
+    if (batch) beam_mode &= ~RT_JIT_BATCH;
+    std::string src = generate_source(rt_jit_signature(dtable, n_surf), beam_mode, false, rt::kBlock, 0, batch);
     std::vector<char> code;
     std::string log;
     bool ok = compile_code(src, code, log);

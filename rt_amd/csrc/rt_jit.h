@@ -18,4 +18,4 @@ bool rt_jit_enabled();
 // when the runtime compiler is unavailable or fails (the caller then
 // launches the generic kernel, which reads the structure from the records).
 hipFunction_t rt_jit_kernel(const std::string& signature, int beam_mode, bool dump, int block, int waves,
-                            int device);
+                            int device, bool batch = false);

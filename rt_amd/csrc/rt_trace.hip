@@ -74,6 +74,43 @@ __global__ __launch_bounds__(kBlock) void rt_trace_kernel(TraceArgs a) {
     });
 }
 
+// Generic batched kernel (structure read from the records of each instance)
+template <int BEAM>
+__global__ __launch_bounds__(kBlock) void rt_trace_batch_kernel(TraceArgs a) {
+    rt::trace_batch_body<BEAM>(a, [&](Ray& r, bool& alive, rt::TraceCtx& c, uint64_t) {
+        for (int s = 0; s < a.n_surf; ++s)
+            rt::surface_step(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, s, r, alive, c);
+    });
+}
+
+// Per-instance spot from the tile moments, combined in tile order (Chan,
+// Golub and LeVeque's pairwise update): n, centroid x, y, variance x, y and
+// RMS radius sqrt(var x + var y).  One thread per instance.
+__global__ void rt_batch_spots_kernel(const double* tm, const uint64_t* tile_start, uint64_t n_inst,
+                                      double* spots) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    double n = 0.0, mx = 0.0, my = 0.0, qx = 0.0, qy = 0.0;
+    for (uint64_t t = tile_start[i]; t < tile_start[i + 1]; ++t) {
+        const double* o = tm + t * 5;
+        const double nb = o[0];
+        if (nb == 0.0) continue;
+        const double nn = n + nb, dx = o[1] - mx, dy = o[2] - my, f = nb / nn;
+        mx += dx * f;
+        my += dy * f;
+        qx += o[3] + dx * dx * n * f;
+        qy += o[4] + dy * dy * n * f;
+        n = nn;
+    }
+    double* out = spots + i * RT_SPOT_STRIDE;
+    out[0] = n;
+    out[1] = mx;
+    out[2] = my;
+    out[3] = n > 0.0 ? qx / n : 0.0;
+    out[4] = n > 0.0 ? qy / n : 0.0;
+    out[5] = n > 0.0 ? sqrt((qx + qy) / n) : 0.0;
+}
+
 // ------------------------------------------------------------------- C ABI
 
 struct rt_hip_scheme {
@@ -343,6 +380,16 @@ static int validate_beam(const double* b) {
     return RT_OK;
 }
 
+// uniform values of the direction generators, on the host (beam.cu:592-598, 1034, 1099);
+// params[3]*deg2rad as the macro expands (beam.cpp:383): (params[3]*RT_PI)/180.0
+static void beam_uniforms(TraceArgs& a) {
+    double ang = a.beam.v[RT_B_ANGUNIT] != 0.0 ? a.beam.v[RT_B_ANGPAR] : a.beam.v[RT_B_ANGPAR] * rt::kPi / 180.0;
+    a.beam.u.cone_cos = cos(ang / 2.0);
+    a.beam.u.cone_sin = sin(ang / 2.0);
+    double stddev = a.beam.v[RT_B_ANGPAR] / 2.35482 * rt::kPi / 180 / 3600;
+    a.beam.u.gauss_tg = tan(stddev);
+}
+
 extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, uint64_t seed,
                             size_t ray_begin, size_t n_rays, double* X, double* Y, double* Z,
                             double* cX, double* cY, double* cZ, double* lambda, signed char* flag,
@@ -370,15 +417,7 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
     TraceArgs a;
     memset(&a, 0, sizeof a);
     if (beam_desc) memcpy(a.beam.v, beam_desc, sizeof a.beam.v);
-    if (gen) {
-        // uniform values of the direction generators, on the host (beam.cu:592-598, 1034, 1099)
-        // params[3]*deg2rad as the macro expands (beam.cpp:383): (params[3]*RT_PI)/180.0
-        double ang = a.beam.v[RT_B_ANGUNIT] != 0.0 ? a.beam.v[RT_B_ANGPAR] : a.beam.v[RT_B_ANGPAR] * rt::kPi / 180.0;
-        a.beam.u.cone_cos = cos(ang / 2.0);
-        a.beam.u.cone_sin = sin(ang / 2.0);
-        double stddev = a.beam.v[RT_B_ANGPAR] / 2.35482 * rt::kPi / 180 / 3600;
-        a.beam.u.gauss_tg = tan(stddev);
-    }
+    if (gen) beam_uniforms(a);
     a.table = sch->table_dev;
     a.pool = sch->pool_dev;
     a.n_surf = sch->n_surf;
@@ -527,5 +566,337 @@ extern "C" int rt_hip_autotune(rt_hip_scheme* sch, const double* beam_desc, uint
     sch->block = bb; sch->waves = bw; sch->grid_mult = bm;
     if (rc != RT_OK) return rc;
     if (best_ms) *best_ms = best;
+    return RT_OK;
+}
+
+// -------------------------------------------------------------- batched traces
+
+struct rt_hip_batch {
+    int device = 0, n_inst = 0, n_surf = 0;
+    std::string signature;           // of the union structure (rotations/translations of any instance)
+    size_t pool_len = 0;
+    std::vector<double> dtables;     // n_inst x n_surf device records
+    std::vector<double> dpool;
+    std::vector<uint64_t> ray_start; // n_inst + 1
+    double* tables_dev = nullptr;
+    double* pool_dev = nullptr;
+    uint64_t* ray_start_dev = nullptr;
+    int block = rt::kBlock, waves = 0, grid_mult = 2;
+    int tiles_block = 0;             // workgroup size the tile arrays below are built for
+    uint64_t n_tiles = 0;
+    uint64_t* tile_start_dev = nullptr;
+    double* moments_dev = nullptr;
+};
+
+struct rt_hip_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void free_batch(rt_hip_batch* b) {
+    if (!b) return;
+    (void)hipFree(b->tables_dev);
+    (void)hipFree(b->pool_dev);
+    (void)hipFree(b->ray_start_dev);
+    (void)hipFree(b->tile_start_dev);
+    (void)hipFree(b->moments_dev);
+    delete b;
+}
+
+// device records of every instance (one shared pool) and the kernel signature
+int build_batch_tables(rt_hip_batch* b, const double* tables, const double* pool, size_t pool_len) {
+    const size_t rec = (size_t)b->n_surf * RT_SURF_STRIDE;
+    const size_t drec = (size_t)b->n_surf * rt::DEV_STRIDE;
+    b->dtables.assign((size_t)b->n_inst * drec, 0.0);
+    std::string base;
+    std::vector<double> uni;
+    for (int i = 0; i < b->n_inst; ++i) {
+        rt_hip_scheme tmp;
+        tmp.n_surf = b->n_surf;
+        tmp.table.assign(tables + i * rec, tables + (i + 1) * rec);
+        if (pool_len) tmp.pool.assign(pool, pool + pool_len);
+        build_device_tables(&tmp);
+        std::copy(tmp.dtable.begin(), tmp.dtable.end(), b->dtables.begin() + i * drec);
+        // structure without the rotations and translations must be the same for all
+        std::vector<double> st = tmp.dtable;
+        for (int s = 0; s < b->n_surf; ++s)
+            for (int j = 0; j < 3; ++j) {
+                st[s * rt::DEV_STRIDE + RT_S_ANG + j] = 0.0;
+                st[s * rt::DEV_STRIDE + RT_S_DIST + j] = 0.0;
+            }
+        std::string sig = rt_jit_signature(st.data(), b->n_surf);
+        if (i == 0) {
+            base = sig;
+            b->dpool = tmp.dpool;
+            b->pool_len = tmp.dpool.size();
+            uni = tmp.dtable;
+        } else if (sig != base) {
+            return set_error(RT_ERR_BAD_VALUE, "instance %d differs in structure from instance 0", i);
+        }
+        // union of the rotations and translations present in any instance: a zero angle or
+        // offset applied as a rotation or translation leaves finite values unchanged
+        for (int s = 0; s < b->n_surf; ++s)
+            for (int j = 0; j < 3; ++j) {
+                double* u = uni.data() + s * rt::DEV_STRIDE;
+                const double* d = tmp.dtable.data() + s * rt::DEV_STRIDE;
+                if (d[RT_S_ANG + j] != 0.0) u[RT_S_ANG + j] = 1.0;
+                if (d[RT_S_DIST + j] != 0.0) u[RT_S_DIST + j] = 1.0;
+            }
+    }
+    b->signature = rt_jit_signature(uni.data(), b->n_surf);
+    return RT_OK;
+}
+
+// tile arrays for workgroups of b->block lanes
+int prepare_tiles(rt_hip_batch* b) {
+    if (b->tiles_block == b->block) return RT_OK;
+    std::vector<uint64_t> ts(b->n_inst + 1, 0);
+    for (int i = 0; i < b->n_inst; ++i) {
+        uint64_t n = b->ray_start[i + 1] - b->ray_start[i];
+        ts[i + 1] = ts[i] + (n + b->block - 1) / b->block;
+    }
+    (void)hipFree(b->tile_start_dev);
+    (void)hipFree(b->moments_dev);
+    b->tile_start_dev = nullptr;
+    b->moments_dev = nullptr;
+    b->tiles_block = 0;
+    b->n_tiles = ts[b->n_inst];
+    HIP_TRY(hipMalloc(&b->tile_start_dev, ts.size() * sizeof(uint64_t)), RT_ERR_BAD_ALLOC);
+    HIP_TRY(hipMalloc(&b->moments_dev, (b->n_tiles ? b->n_tiles : 1) * 5 * sizeof(double)), RT_ERR_BAD_ALLOC);
+    HIP_TRY(hipMemcpy(b->tile_start_dev, ts.data(), ts.size() * sizeof(uint64_t), hipMemcpyHostToDevice),
+            RT_ERR_FAILED);
+    b->tiles_block = b->block;
+    return RT_OK;
+}
+
+}  // namespace
+
+extern "C" int rt_hip_batch_create(const double* tables, int n_inst, int n_surf, const double* spline_pool,
+                                   size_t pool_len, const uint64_t* ray_counts, rt_hip_batch** out) {
+    if (!out) return set_error(RT_ERR_BAD_VALUE, "out is NULL");
+    *out = nullptr;
+    if (!tables || !ray_counts || n_inst < 1) return set_error(RT_ERR_BAD_VALUE, "bad batch arguments");
+    const size_t rec = (size_t)n_surf * RT_SURF_STRIDE;
+    for (int i = 0; i < n_inst; ++i) {
+        int rc = rt_hip_validate_scheme(tables + i * rec, n_surf, spline_pool, pool_len);
+        if (rc != RT_OK) return rc;
+        if (ray_counts[i] > (1ull << 40)) return set_error(RT_ERR_BAD_VALUE, "instance %d: too many rays", i);
+    }
+    rt_hip_batch* b = new (std::nothrow) rt_hip_batch();
+    if (!b) return set_error(RT_ERR_BAD_ALLOC, "out of host memory");
+    b->n_inst = n_inst;
+    b->n_surf = n_surf;
+    b->ray_start.assign(n_inst + 1, 0);
+    for (int i = 0; i < n_inst; ++i) b->ray_start[i + 1] = b->ray_start[i] + ray_counts[i];
+    int rc = build_batch_tables(b, tables, spline_pool, pool_len);
+    if (rc == RT_OK) rc = ensure_device();
+    if (rc != RT_OK) { free_batch(b); return rc; }
+    b->device = g_dev.device;
+    size_t tb = b->dtables.size() * sizeof(double);
+    size_t pb = (b->dpool.empty() ? 1 : b->dpool.size()) * sizeof(double);
+    size_t rb = b->ray_start.size() * sizeof(uint64_t);
+    if (hipMalloc(&b->tables_dev, tb) != hipSuccess || hipMalloc(&b->pool_dev, pb) != hipSuccess ||
+        hipMalloc(&b->ray_start_dev, rb) != hipSuccess) {
+        free_batch(b);
+        return set_error(RT_ERR_BAD_ALLOC, "hipMalloc of batch tables failed");
+    }
+    if (hipMemcpy(b->tables_dev, b->dtables.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
+        (!b->dpool.empty() &&
+         hipMemcpy(b->pool_dev, b->dpool.data(), b->dpool.size() * sizeof(double), hipMemcpyHostToDevice) !=
+             hipSuccess) ||
+        hipMemcpy(b->ray_start_dev, b->ray_start.data(), rb, hipMemcpyHostToDevice) != hipSuccess) {
+        free_batch(b);
+        return set_error(RT_ERR_FAILED, "upload of batch tables failed");
+    }
+    rc = prepare_tiles(b);
+    if (rc != RT_OK) { free_batch(b); return rc; }
+    *out = b;
+    return RT_OK;
+}
+
+extern "C" int rt_hip_batch_update(rt_hip_batch* b, const double* tables, const double* spline_pool,
+                                   size_t pool_len) {
+    if (!b || !tables) return set_error(RT_ERR_BAD_VALUE, "bad batch arguments");
+    const size_t rec = (size_t)b->n_surf * RT_SURF_STRIDE;
+    for (int i = 0; i < b->n_inst; ++i) {
+        int rc = rt_hip_validate_scheme(tables + i * rec, b->n_surf, spline_pool, pool_len);
+        if (rc != RT_OK) return rc;
+    }
+    std::string sig = b->signature;
+    size_t old_pool = b->pool_len;
+    int rc = build_batch_tables(b, tables, spline_pool, pool_len);
+    if (rc != RT_OK) return rc;
+    if (b->signature != sig || b->pool_len != old_pool)
+        return set_error(RT_ERR_BAD_VALUE, "updated tables change the batch structure; create a new batch");
+    HIP_TRY(hipMemcpy(b->tables_dev, b->dtables.data(), b->dtables.size() * sizeof(double), hipMemcpyHostToDevice),
+            RT_ERR_FAILED);
+    if (!b->dpool.empty())
+        HIP_TRY(hipMemcpy(b->pool_dev, b->dpool.data(), b->dpool.size() * sizeof(double), hipMemcpyHostToDevice),
+                RT_ERR_FAILED);
+    return RT_OK;
+}
+
+extern "C" int rt_hip_batch_destroy(rt_hip_batch* b) {
+    free_batch(b);
+    return RT_OK;
+}
+
+extern "C" int rt_hip_batch_info(const rt_hip_batch* b, uint64_t* total_rays, uint64_t* n_tiles) {
+    if (!b) return set_error(RT_ERR_BAD_VALUE, "batch is NULL");
+    if (total_rays) *total_rays = b->ray_start[b->n_inst];
+    if (n_tiles) *n_tiles = b->n_tiles;
+    return RT_OK;
+}
+
+extern "C" int rt_hip_batch_set_launch(rt_hip_batch* b, int block, int waves, int grid_mult) {
+    if (!b) return set_error(RT_ERR_BAD_VALUE, "batch is NULL");
+    if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
+        return set_error(RT_ERR_BAD_VALUE, "block must be 64, 128, 256, 512 or 1024 lanes");
+    if (waves < 0 || waves > 8) return set_error(RT_ERR_BAD_VALUE, "waves must be in [0, 8]");
+    if (grid_mult < 1 || grid_mult > 16) return set_error(RT_ERR_BAD_VALUE, "grid_mult must be in [1, 16]");
+    b->block = block;
+    b->waves = waves;
+    b->grid_mult = grid_mult;
+    return prepare_tiles(b);
+}
+
+extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_desc, uint64_t seed, int flags,
+                                  double* X, double* Y, double* Z, double* cX, double* cY, double* cZ,
+                                  double* lambda, signed char* flag, double* spots,
+                                  unsigned long long* stats, void* stream) {
+    rt_hip_batch* b = const_cast<rt_hip_batch*>(bc);
+    if (!b) return set_error(RT_ERR_BAD_VALUE, "batch is NULL");
+    int rc = validate_beam(beam_desc);
+    if (rc != RT_OK) return rc;
+    if ((flags & RT_TRACE_STORE) && (!X || !Y || !Z || !cX || !cY || !cZ || !lambda || !flag))
+        return set_error(RT_ERR_BAD_VALUE, "RT_TRACE_STORE needs all ray buffers");
+    if (flags & (RT_TRACE_HIST | RT_TRACE_FLUX))
+        return set_error(RT_ERR_BAD_VALUE, "batched traces reduce spots, not CCD frames");
+    rc = ensure_device();
+    if (rc != RT_OK) return rc;
+    rc = prepare_tiles(b);
+    if (rc != RT_OK) return rc;
+    if (b->n_tiles == 0) return RT_OK;
+
+    TraceArgs a;
+    memset(&a, 0, sizeof a);
+    memcpy(a.beam.v, beam_desc, sizeof a.beam.v);
+    beam_uniforms(a);
+    a.table = b->tables_dev;
+    a.pool = b->pool_dev;
+    a.n_surf = b->n_surf;
+    a.pool_len = (int)b->pool_len;
+    a.flags = flags | RT_TRACE_GENERATE;
+    a.seed = seed;
+    a.n = b->ray_start[b->n_inst];
+    if (flags & RT_TRACE_STORE) {
+        a.X = X; a.Y = Y; a.Z = Z; a.cX = cX; a.cY = cY; a.cZ = cZ; a.L = lambda; a.F = flag;
+    }
+    a.stats = stats;
+    a.tile_start = b->tile_start_dev;
+    a.ray_start = b->ray_start_dev;
+    a.tile_moments = b->moments_dev;
+    a.n_inst = (uint64_t)b->n_inst;
+    a.n_tiles = b->n_tiles;
+    a.inst_stride = (uint64_t)b->n_surf * rt::DEV_STRIDE;
+
+    const int cus = rt_device_cus();
+    const size_t lds = b->pool_len * sizeof(double);
+    hipStream_t st = (hipStream_t)stream;
+    const int beam_mode = (int)a.beam.v[RT_B_TYPE];
+    hipFunction_t fn = nullptr;
+    if (rt_jit_enabled())
+        fn = rt_jit_kernel(b->signature, beam_mode, false, b->block, b->waves, b->device, true);
+    if (fn) {
+        int resident = 0;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, b->block, lds) != hipSuccess ||
+            resident < 1)
+            resident = 1;
+        uint64_t cap = (uint64_t)cus * resident * b->grid_mult;
+        unsigned grid = (unsigned)(b->n_tiles < cap ? b->n_tiles : cap);
+        void* params[] = {&a};
+        HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, b->block, 1, 1, (unsigned)lds, st, params, nullptr),
+                RT_ERR_FAILED);
+        g_last_kernel = "generated";
+    } else {
+        if (b->block != kBlock) {
+            // the generic kernel is built for kBlock lanes: retile for it
+            b->block = kBlock;
+            rc = prepare_tiles(b);
+            if (rc != RT_OK) return rc;
+            a.tile_start = b->tile_start_dev;
+            a.tile_moments = b->moments_dev;
+            a.n_tiles = b->n_tiles;
+        }
+        uint64_t cap = (uint64_t)cus * 8;
+        unsigned grid = (unsigned)(b->n_tiles < cap ? b->n_tiles : cap);
+        auto kern = beam_mode == RT_BEAM_PARALLEL ? rt_trace_batch_kernel<RT_BEAM_PARALLEL>
+                  : beam_mode == RT_BEAM_CONIC    ? rt_trace_batch_kernel<RT_BEAM_CONIC>
+                                                  : rt_trace_batch_kernel<RT_BEAM_GAUSS>;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, a);
+        g_last_kernel = "generic";
+    }
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    if (spots) {
+        unsigned g = (unsigned)((b->n_inst + 255) / 256);
+        hipLaunchKernelGGL(rt_batch_spots_kernel, dim3(g), dim3(256), 0, st, (const double*)b->moments_dev,
+                           (const uint64_t*)b->tile_start_dev, (uint64_t)b->n_inst, spots);
+        HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    }
+    return RT_OK;
+}
+
+// A batched trace with fixed arguments recorded once as a hipGraph (trace and
+// spot kernels), replayed by rt_hip_graph_launch.  Table contents may change
+// between replays (rt_hip_batch_update); the stream must not be the null stream.
+extern "C" int rt_hip_batch_graph_create(const rt_hip_batch* b, const double* beam_desc, uint64_t seed, int flags,
+                                         double* X, double* Y, double* Z, double* cX, double* cY, double* cZ,
+                                         double* lambda, signed char* flag, double* spots,
+                                         unsigned long long* stats, void* stream, rt_hip_graph** out) {
+    if (!out) return set_error(RT_ERR_BAD_VALUE, "out is NULL");
+    *out = nullptr;
+    if (!b) return set_error(RT_ERR_BAD_VALUE, "batch is NULL");
+    if (!stream) return set_error(RT_ERR_BAD_VALUE, "graph capture needs a created stream");
+    hipStream_t st = (hipStream_t)stream;
+    // compile and load the kernel before the capture
+    if (rt_jit_enabled()) {
+        int rc = validate_beam(beam_desc);
+        if (rc != RT_OK) return rc;
+        (void)rt_jit_kernel(b->signature, (int)beam_desc[RT_B_TYPE], false, b->block, b->waves, b->device, true);
+    }
+    rt_hip_graph* g = new (std::nothrow) rt_hip_graph();
+    if (!g) return set_error(RT_ERR_BAD_ALLOC, "out of host memory");
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), RT_ERR_FAILED);
+    int rc = rt_hip_batch_trace(b, beam_desc, seed, flags, X, Y, Z, cX, cY, cZ, lambda, flag, spots, stats, stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc != RT_OK || e != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        delete g;
+        return rc != RT_OK ? rc : set_error(RT_ERR_FAILED, "graph capture: %s", hipGetErrorString(e));
+    }
+    g->graph = graph;
+    if (hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        delete g;
+        return set_error(RT_ERR_FAILED, "hipGraphInstantiate failed");
+    }
+    *out = g;
+    return RT_OK;
+}
+
+extern "C" int rt_hip_graph_launch(rt_hip_graph* g, void* stream) {
+    if (!g || !g->exec) return set_error(RT_ERR_BAD_VALUE, "graph is NULL");
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream), RT_ERR_FAILED);
+    return RT_OK;
+}
+
+extern "C" int rt_hip_graph_destroy(rt_hip_graph* g) {
+    if (!g) return RT_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
     return RT_OK;
 }

@@ -46,6 +46,17 @@ def lib():
         L.rt_hip_scheme_get_launch.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.rt_hip_autotune.argtypes = [_vp, _dp, C.c_uint64, C.c_size_t, C.c_int, C.POINTER(C.c_double), _vp]
         L.rt_hip_last_kernel.argtypes = [C.c_char_p, C.c_size_t]
+        L.rt_hip_batch_create.argtypes = [_dp, C.c_int, C.c_int, _dp, C.c_size_t, C.POINTER(C.c_uint64),
+                                          C.POINTER(_vp)]
+        L.rt_hip_batch_update.argtypes = [_vp, _dp, _dp, C.c_size_t]
+        L.rt_hip_batch_destroy.argtypes = [_vp]
+        L.rt_hip_batch_info.argtypes = [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.rt_hip_batch_set_launch.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+        L.rt_hip_batch_trace.argtypes = [_vp, _dp, C.c_uint64, C.c_int] + [_vp] * 8 + [_vp, _vp, _vp]
+        L.rt_hip_batch_graph_create.argtypes = ([_vp, _dp, C.c_uint64, C.c_int] + [_vp] * 8 +
+                                                [_vp, _vp, _vp, C.POINTER(_vp)])
+        L.rt_hip_graph_launch.argtypes = [_vp, _vp]
+        L.rt_hip_graph_destroy.argtypes = [_vp]
         L.rt_hip_jit_check.argtypes = [_dp, C.c_int, _dp, C.c_size_t, C.c_int, C.c_char_p, C.c_size_t,
                                        C.POINTER(C.c_size_t)]
         # per-stage entry points: host arrays (records, angles, beam descriptions) as
@@ -195,3 +206,103 @@ def jit_check(scheme: "S.Scheme", beam_mode: int):
 
 def init(device: int = 0):
     check(lib().rt_hip_init(device), "rt_hip_init")
+
+
+SPOT_STRIDE = 6     # RT_SPOT_STRIDE: n, centroid x, y, variance x, y, RMS radius
+
+
+def pack_batch(instances):
+    """(n_inst x n_surf x RT_SURF_STRIDE tables, shared pool) of schemes of one structure"""
+    pool = np.ascontiguousarray(instances[0].pool, dtype=np.float64)
+    for s in instances[1:]:
+        if not np.array_equal(s.pool, pool):
+            raise ValueError("batched instances must share one spline pool")
+    tables = np.ascontiguousarray(np.stack([np.asarray(s.table, dtype=np.float64) for s in instances]))
+    return tables, pool
+
+
+class DeviceBatch:
+    """Instances of one scheme structure on the device (rt_hip_batch)."""
+
+    def __init__(self, instances, ray_counts):
+        self.instances = instances
+        self.n_inst = len(instances)
+        self.n_surf = instances[0].n_surf
+        self._t, self._p = pack_batch(instances)
+        self.counts = np.ascontiguousarray(ray_counts, dtype=np.uint64)
+        h = _vp()
+        check(lib().rt_hip_batch_create(_dptr(self._t), self.n_inst, self.n_surf, _dptr(self._p), len(self._p),
+                                        self.counts.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(h)),
+              "rt_hip_batch_create")
+        self.handle = h
+        self.total = int(self.counts.sum())
+
+    def close(self):
+        if self.handle:
+            lib().rt_hip_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def update(self, instances):
+        t, p = pack_batch(instances)
+        self._t, self._p = t, p
+        check(lib().rt_hip_batch_update(self.handle, _dptr(t), _dptr(p), len(p)), "rt_hip_batch_update")
+
+    def set_launch(self, block=256, waves=0, grid_mult=2):
+        check(lib().rt_hip_batch_set_launch(self.handle, block, waves, grid_mult), "rt_hip_batch_set_launch")
+
+    def info(self):
+        n, t = C.c_uint64(), C.c_uint64()
+        check(lib().rt_hip_batch_info(self.handle, C.byref(n), C.byref(t)), "rt_hip_batch_info")
+        return n.value, t.value
+
+    def _args(self, beam, rays, flag):
+        b = np.ascontiguousarray((beam if beam is not None else self.instances[0].beam).pack())
+        ptrs = [None] * 8
+        if rays is not None:
+            assert rays.shape == (7, self.total) and rays.is_contiguous()
+            ptrs = [rays[k].data_ptr() for k in range(7)] + [_tp(flag)]
+        return b, ptrs
+
+    def trace(self, *, seed=0, flags=0, rays=None, flag=None, spots=None, stats=None, beam=None, stream=None):
+        """rt_hip_batch_trace; rays (7, total) and flag are filled with RT_TRACE_STORE,
+        spots (n_inst, SPOT_STRIDE) float64 receives the per-instance spots."""
+        import torch
+        b, ptrs = self._args(beam, rays, flag)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self._beam = b
+        check(lib().rt_hip_batch_trace(self.handle, _dptr(b), seed, flags, *ptrs, _tp(spots), _tp(stats), stream),
+              "rt_hip_batch_trace")
+
+    def graph(self, *, stream, seed=0, flags=0, rays=None, flag=None, spots=None, stats=None, beam=None):
+        """Record the batched trace as a hipGraph on `stream` (a torch.cuda.Stream)."""
+        b, ptrs = self._args(beam, rays, flag)
+        g = _vp()
+        check(lib().rt_hip_batch_graph_create(self.handle, _dptr(b), seed, flags, *ptrs, _tp(spots), _tp(stats),
+                                              stream.cuda_stream, C.byref(g)), "rt_hip_batch_graph_create")
+        return Graph(g, b)
+
+
+class Graph:
+    def __init__(self, handle, keep):
+        self.handle, self._keep = handle, keep
+
+    def launch(self, stream):
+        check(lib().rt_hip_graph_launch(self.handle, stream.cuda_stream), "rt_hip_graph_launch")
+
+    def close(self):
+        if self.handle:
+            lib().rt_hip_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -23,6 +23,7 @@ degrees; scheme.cpp:406-664), with geometry frozen as literal numbers.
 """
 from __future__ import annotations
 
+import dataclasses
 import math
 
 import numpy as np
@@ -196,3 +197,33 @@ def echelle(n_per_lambda: int = 4883, nlambda: int = 2048) -> Scheme:
                      distance=(0.0, 0.0, ECH_CCD_DIST), ccd=ccd, qe=qe, spectrum=1.0,
                      comment="CCD 4096x4096 15um"))
     return Scheme(s, beam, name="echelle")
+
+
+def tolerance_instances(base: Scheme, n_inst: int, seed: int = 0, decenter_mm: float = 0.020,
+                        tilt_arcsec: float = 30.0, curvature_rel: float = 1e-4,
+                        min_rays: int = 10_000, max_rays: int = 1_000_000):
+    """Monte-Carlo tolerance instances of `base`: every surface decentred by
+    N(0, decenter_mm) in x and y, tilted by N(0, tilt_arcsec) about y and x,
+    curvatures (c and, for torics, cR) scaled by 1 + N(0, curvature_rel); ray
+    counts log-uniform in [min_rays, max_rays].  Returns (schemes, counts);
+    the instances share the base's tables (spline objects), so they pack to
+    one spline pool."""
+    rng = np.random.default_rng(seed)
+    out, counts = [], []
+    for _ in range(n_inst):
+        surfs = []
+        for srf in base.surfaces:
+            dx, dy = rng.normal(0.0, decenter_mm, 2)
+            ty, tx = rng.normal(0.0, tilt_arcsec / 3600.0, 2)
+            dist = (srf.distance[0] + dx, srf.distance[1] + dy, srf.distance[2])
+            ang = (srf.angles[0] + ty, srf.angles[1] + tx, srf.angles[2])
+            p = list(srf.params)
+            if srf.cls in (RT_CONIC, RT_TORIC) and p[0] != 0.0:
+                p[0] *= 1.0 + rng.normal(0.0, curvature_rel)
+            if srf.cls == RT_TORIC:
+                n = int(p[2])
+                p[3 + n] *= 1.0 + rng.normal(0.0, curvature_rel)
+            surfs.append(dataclasses.replace(srf, distance=dist, angles=ang, params=tuple(p)))
+        out.append(Scheme(surfs, base.beam, name=f"{base.name}-tol"))
+        counts.append(int(round(math.exp(rng.uniform(math.log(min_rays), math.log(max_rays))))))
+    return out, counts

@@ -185,3 +185,20 @@ def test_stage_intersect_accepts_scheme_records():
     rc = L.rt_hip_stage_intersect(bad.ctypes.data_as(C.POINTER(C.c_double)), 0, *[dummy.ctypes.data] * 6,
                                   flag.ctypes.data, None, None, None)
     assert rc == 4 and "aspheric" in lib.last_error()
+
+
+def test_batched_kernel_compiles_and_batches_are_validated():
+    rc, size, text = lib.jit_check(schemes.echelle(1, 4), S.RT_BEAM_GAUSS | 0x100)
+    assert rc == 0, text[-2000:]
+    assert "trace_batch_body" in text
+    L = lib.lib()
+    a = schemes.echelle(1, 4)
+    b = schemes.czerny_turner(1, 5)
+    b.table = np.resize(b.table, len(a.table))           # same size, other structure
+    tables = np.ascontiguousarray(np.stack([a.table, b.table]))
+    counts = np.array([10, 10], dtype=np.uint64)
+    h = C.c_void_p()
+    rc = L.rt_hip_batch_create(tables.ctypes.data_as(C.POINTER(C.c_double)), 2, a.n_surf,
+                               np.ascontiguousarray(a.pool).ctypes.data_as(C.POINTER(C.c_double)), len(a.pool),
+                               counts.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(h))
+    assert rc == 4 and "structure" in lib.last_error()
