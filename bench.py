@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--nlambda", type=int, default=2048)
     ap.add_argument("--cpu-sample", type=int, default=0, help="rays traced by the CPU baseline (0: the whole workload)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=0, metavar="N_INST",
+                    help="measure the batched workload instead: N_INST Monte-Carlo tolerance instances of "
+                         "the echelle, ray counts log-uniform in [1e4, 1e6], one persistent launch per step")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -73,6 +76,8 @@ def main():
     from rt_amd import scheme as S
 
     lib.init(0)
+    if args.batch:
+        return bench_batch(args)
     sch = schemes.echelle(args.n_per_lambda, args.nlambda)
     n = sch.beam.n_rays
     ds = lib.DeviceScheme(sch)
@@ -159,6 +164,50 @@ def main():
                                "kind": "port",
                                "sample": (f"the whole workload ({m} rays)" if m == n else f"first {m} rays of the same beam") + ", oracle fused trace, one pass"}
     print(json.dumps(out))
+
+
+def bench_batch(args):
+    """Batched forward: tolerance instances of the echelle (decentres N(0, 20 um),
+    tilts N(0, 30 arcsec), curvatures 1 + N(0, 1e-4)), uneven ray counts, spots
+    reduced per instance; one rt_hip_batch_trace per step."""
+    import torch
+    from rt_amd import lib, schemes
+    from rt_amd import scheme as S
+    base = schemes.echelle()
+    inst, counts = schemes.tolerance_instances(base, args.batch, seed=1, min_rays=10_000, max_rays=1_000_000)
+    db = lib.DeviceBatch(inst, counts)
+    db.set_launch(256, 6, 2)
+    spots = torch.zeros((len(inst), lib.SPOT_STRIDE), dtype=torch.float64, device="cuda")
+    stats = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream()
+    for k in range(args.warmup):
+        db.trace(seed=1000 + k, spots=spots, stats=stats)
+    torch.cuda.synchronize()
+    stats.zero_()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for k in range(args.steps):
+        db.trace(seed=k * 4096, spots=spots, stats=stats)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = stats.cpu().numpy().view(np.uint64)
+    ray_surf = int(sum(int(st[S.ST_SURF + 4 * s]) for s in range(base.n_surf)))
+    sp = spots.cpu().numpy()
+    print(json.dumps({
+        "metric": f"ray-surface hits/s FP64 on 1 MI355X, batched ({len(inst)} echelle tolerance instances)",
+        "value": ray_surf / wall, "unit": "ray*surface/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic (Philox beams, seed + instance)",
+        "config": {"workload": "echelle-tolerance-batch", "instances": len(inst), "rays": int(sum(counts)),
+                   "min_rays": int(min(counts)), "max_rays": int(max(counts))},
+        "detail": {"kernel_ms": ev0.elapsed_time(ev1) / args.steps,
+                   "ray_surface_per_step": ray_surf // args.steps,
+                   "median_rms_spot_um": float(np.median(sp[:, 5]) * 1e3),
+                   "rays_per_s": int(sum(counts)) * args.steps / wall}}))
 
 
 if __name__ == "__main__":

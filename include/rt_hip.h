@@ -123,6 +123,7 @@ enum rt_stat_field {
     RT_ST_CCD_OFFCHIP = 3,   /* alive rays whose pixel falls outside the frame    */
     RT_ST_CCD_EDGE = 4,      /* counted rays within 1e-9 px of a pixel edge       */
     RT_ST_SPLINE_OOB = 5,    /* rays killed by a spline argument out of range     */
+    RT_ST_ADJ_SKIP = 6,      /* rays the adjoint could not differentiate          */
     RT_ST_SURF = 16,         /* per surface s: 4 counters at RT_ST_SURF + 4*s:    */
     /*   +0 rays alive entering the intersection stage (the ray.surface count)     */
     /*   +1 Newton iterations summed over rays                                    */
@@ -319,6 +320,25 @@ int rt_hip_batch_graph_create(const rt_hip_batch *batch, const double *beam_desc
                               unsigned long long *stats, void *stream, rt_hip_graph **out);
 int rt_hip_graph_launch(rt_hip_graph *graph, void *stream);
 int rt_hip_graph_destroy(rt_hip_graph *graph);
+
+/* ---- adjoint ------------------------------------------------------------ */
+
+/* Gradient of a loss on the surviving rays' final positions with respect to
+   scheme parameters.  The beam (beam_desc, seed, rays 0 .. n_rays-1) is traced
+   forward and the surface chain reversed per ray, recomputing each surface
+   from its entry state.  The loss enters through group_seeds (device,
+   n_groups x 3): for a ray r alive at the end, of group g = r mod n_groups,
+   with final position (X, Y), dLoss/dX = ax_g + b_g X and dLoss/dY =
+   ay_g + b_g Y (centroids, RMS radii and squared deviations of them are of
+   this form).  Parameters: param_surface[j], param_field[j] (host) name
+   RT_S_C, RT_S_K, RT_S_A + i, RT_S_CR, RT_S_DIST + j, RT_S_ANG + j (radians,
+   the record's value) or RT_S_GRCONST (um) of a surface; grad (device,
+   n_params) receives dLoss/dparameter, reduced in a fixed order.  Plane
+   gratings only.  No counterpart in the reference (it has no derivatives). */
+int rt_hip_adjoint(const rt_hip_scheme *scheme, const double *beam_desc, uint64_t seed, size_t n_rays,
+                   const double *group_seeds, size_t n_groups, const int *param_surface,
+                   const int *param_field, int n_params, double *grad, unsigned long long *stats,
+                   void *stream);
 
 /* "generated" or "generic": the kernel the calling thread's last trace used. */
 int rt_hip_last_kernel(char *buf, size_t len);

@@ -1,0 +1,584 @@
+// rt_adjoint.h -- reverse-mode derivatives of one surface of the trace.
+//
+// surface_adjoint() takes a ray's state on entry to a surface (before the
+// frame transform) and the adjoint of its state after the surface, recomputes
+// the surface's stages (transform, normalize, intersection, action,
+// normalize; Scheme::run order, common-lib/scheme.cpp:983-1046) in plain FP64
+// from the entry state, and runs them backwards: it returns the adjoint of the
+// entry state and adds the derivatives with respect to the surface's
+// parameters (curvature, conic constant, aspheric coefficients, toric cR,
+// decentres, tilts, grating constant) to gp[].  The loss is a function of the
+// surviving rays' positions; vignetting, the choice of root and the Newton
+// iteration count are held fixed, as they are piecewise constant.
+//
+// Intersections: the plane and the closed-form conic (quadric) roots through
+// the implicit function F(p + s c) = 0; the iterative conic and toric
+// intersections (rt_cuda_engine_surface.cu:257-333, 414-492) return the
+// penultimate Newton iterate, so their derivative is taken through the
+// iterations themselves, s_{k+1} = s_k - F(s_k)/F'(s_k), with the iteration
+// count of the forward pass.
+//
+// Everything here is plain arithmetic on doubles (host and device), so it
+// also builds for the host, where it is checked against finite differences.
+#pragma once
+
+#include "rt_hip.h"
+
+#if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+#define RT_HD __host__ __device__ inline
+#else
+#include <math.h>
+#define RT_HD static inline
+#endif
+
+namespace rt {
+namespace adj {
+
+// parameter slots of one surface
+enum ParamSlot {
+    PG_C = 0,                          // curvature c
+    PG_K = 1,                          // conic constant k
+    PG_A = 2,                          // aspheric a_1 .. a_RT_MAX_POLY (coefficients of r^2 .. r^2N)
+    PG_CR = PG_A + RT_MAX_POLY,        // toric cR
+    PG_DIST = PG_CR + 1,               // decentres d_x, d_y, d_z
+    PG_ANG = PG_DIST + 3,              // tilts about Y, X, Z (radians, as in the record)
+    PG_D = PG_ANG + 3,                 // grating constant (um)
+    PG_N = PG_D + 1
+};
+
+// public record field -> slot (-1: not differentiable)
+RT_HD int slot_of_field(int field) {
+    if (field == RT_S_C) return PG_C;
+    if (field == RT_S_K) return PG_K;
+    if (field >= RT_S_A && field < RT_S_A + RT_MAX_POLY) return PG_A + (field - RT_S_A);
+    if (field == RT_S_CR) return PG_CR;
+    if (field >= RT_S_DIST && field < RT_S_DIST + 3) return PG_DIST + (field - RT_S_DIST);
+    if (field >= RT_S_ANG && field < RT_S_ANG + 3) return PG_ANG + (field - RT_S_ANG);
+    if (field == RT_S_GRCONST) return PG_D;
+    return -1;
+}
+
+struct St {
+    double X, Y, Z, cX, cY, cZ;
+};
+
+enum { AK_PLANE = 0, AK_QUAD = 1, AK_CONIC_NEWTON = 2, AK_TORIC_NEWTON = 3, AK_NONE = 4 };
+
+RT_HD int ikind(const double* p) {
+    const int c = (int)p[RT_S_CLASS], n = (int)p[RT_S_NPOLY];
+    if (c == RT_PLANE || (c == RT_CONIC && n == 0 && p[RT_S_C] == 0.0)) return AK_PLANE;
+    if (c == RT_CONIC) return n == 0 ? AK_QUAD : AK_CONIC_NEWTON;
+    if (c == RT_TORIC) return AK_TORIC_NEWTON;
+    return AK_NONE;
+}
+
+// ------------------------------------------------------------ polynomials
+// P(x) = sum a_i x^i, Q = P'(x) = sum i a_i x^(i-1), R = Q'(x)
+struct Poly {
+    double P, Q, R;
+};
+
+RT_HD Poly poly(const double* p, double x) {
+    const int N = (int)p[RT_S_NPOLY];
+    Poly o = {0.0, 0.0, 0.0};
+    double xi = 1.0;                   // x^(i-1)
+    double xm = 0.0;                   // x^(i-2)
+    for (int i = 1; i <= N; ++i) {
+        const double a = p[RT_S_A + i - 1];
+        o.P += a * xi * x;
+        o.Q += i * a * xi;
+        if (i >= 2) o.R += i * (i - 1) * a * xm;
+        xm = xi;
+        xi *= x;
+    }
+    return o;
+}
+
+// d/d a_i of P and of Q at x, accumulated with weights wP, wQ
+RT_HD void poly_param_adj(const double* p, double x, double wP, double wQ, double* gp) {
+    const int N = (int)p[RT_S_NPOLY];
+    double xi = 1.0;
+    for (int i = 1; i <= N; ++i) {
+        gp[PG_A + i - 1] += wP * xi * x + wQ * i * xi;
+        xi *= x;
+    }
+}
+
+// ---------------------------------------------------- conic profile pieces
+// With t = 1 - (k+1) c^2 x, q = sqrt(t) (x = r^2, or Y^2 for a toric):
+//   S(x) = c x / (1 + q)   sag of the conic, dS/dx = c / (2q)
+//   E(x) = c / q + 2 Q(x)  the factor of the conic normal, -dF/dX = E X
+struct Conic {
+    double q, S, E, dSdx, dEdx, dSdc, dSdk, dEdc, dEdk;
+    Poly pl;
+};
+
+RT_HD Conic conic_at(const double* p, double x) {
+    const double c = p[RT_S_C], k1 = p[RT_S_K] + 1.0;
+    Conic o;
+    const double K = k1 * c * c;
+    const double t = 1.0 - K * x;
+    o.q = sqrt(t);
+    const double q = o.q, q3 = t * q, opq = 1.0 + q;
+    o.pl = poly(p, x);
+    o.S = c * x / opq;
+    o.E = c / q + 2.0 * o.pl.Q;
+    o.dSdx = c / (2.0 * q);
+    o.dEdx = c * K / (2.0 * q3) + 2.0 * o.pl.R;
+    o.dSdc = x / opq + k1 * c * c * x * x / (q * opq * opq);
+    o.dSdk = c * c * c * x * x / (2.0 * q * opq * opq);
+    o.dEdc = 1.0 / q + k1 * c * c * x / q3;
+    o.dEdk = c * c * c * x / (2.0 * q3);
+    return o;
+}
+
+// adjoint of (S + P) and of E at x, with weights wS (on S + P: the full sag) and wE
+RT_HD double conic_adj(const double* p, const Conic& k, double x, double wS, double wE, double* gp) {
+    gp[PG_C] += wS * k.dSdc + wE * k.dEdc;
+    gp[PG_K] += wS * k.dSdk + wE * k.dEdk;
+    poly_param_adj(p, x, wS, 2.0 * wE, gp);
+    return wS * (k.dSdx + k.pl.Q) + wE * k.dEdx;      // adjoint of x
+}
+
+// --------------------------------------------------------------- transform
+RT_HD void rot2(double s, double c, double& v1, double& v2) {
+    double e = v1 * c - v2 * s;
+    v2 = v1 * s + v2 * c;
+    v1 = e;
+}
+
+RT_HD void fwd_transform(const double* p, St& r) {
+    const double* d = p + RT_S_DIST;
+    if ((d[2] != 0.0) || (d[0] != 0.0) || (d[1] != 0.0)) {
+        r.X -= d[0]; r.Y -= d[1]; r.Z -= d[2];
+    }
+    if (p[RT_S_ANG + 0] != 0.0) { rot2(p[RT_S_SIN + 0], p[RT_S_COS + 0], r.X, r.Z); rot2(p[RT_S_SIN + 0], p[RT_S_COS + 0], r.cX, r.cZ); }
+    if (p[RT_S_ANG + 1] != 0.0) { rot2(p[RT_S_SIN + 1], p[RT_S_COS + 1], r.Y, r.Z); rot2(p[RT_S_SIN + 1], p[RT_S_COS + 1], r.cY, r.cZ); }
+    if (p[RT_S_ANG + 2] != 0.0) { rot2(p[RT_S_SIN + 2], p[RT_S_COS + 2], r.X, r.Y); rot2(p[RT_S_SIN + 2], p[RT_S_COS + 2], r.cX, r.cY); }
+}
+
+// one rotation backwards: outputs (v1, v2) become inputs, adjoints (w1, w2)
+// likewise; the derivative with respect to the angle is added to *ga.  A zero
+// angle (skipped forward) still has its derivative, the rotation's at 0.
+RT_HD void rot2_adj(double s, double c, double& v1, double& v2, double& w1, double& w2, double* ga) {
+    *ga += -w1 * v2 + w2 * v1;
+    double i1 = c * v1 + s * v2, i2 = -s * v1 + c * v2;
+    double a1 = c * w1 + s * w2, a2 = -s * w1 + c * w2;
+    v1 = i1; v2 = i2; w1 = a1; w2 = a2;
+}
+
+// r: state after the transform (becomes the state before it); a: its adjoint
+RT_HD void adj_transform(const double* p, St& r, St& a, double* gp) {
+    const double s2 = p[RT_S_ANG + 2] != 0.0 ? p[RT_S_SIN + 2] : 0.0, c2 = p[RT_S_ANG + 2] != 0.0 ? p[RT_S_COS + 2] : 1.0;
+    const double s1 = p[RT_S_ANG + 1] != 0.0 ? p[RT_S_SIN + 1] : 0.0, c1 = p[RT_S_ANG + 1] != 0.0 ? p[RT_S_COS + 1] : 1.0;
+    const double s0 = p[RT_S_ANG + 0] != 0.0 ? p[RT_S_SIN + 0] : 0.0, c0 = p[RT_S_ANG + 0] != 0.0 ? p[RT_S_COS + 0] : 1.0;
+    double ga2 = 0.0, ga1 = 0.0, ga0 = 0.0;
+    rot2_adj(s2, c2, r.X, r.Y, a.X, a.Y, &ga2);
+    rot2_adj(s2, c2, r.cX, r.cY, a.cX, a.cY, &ga2);
+    rot2_adj(s1, c1, r.Y, r.Z, a.Y, a.Z, &ga1);
+    rot2_adj(s1, c1, r.cY, r.cZ, a.cY, a.cZ, &ga1);
+    rot2_adj(s0, c0, r.X, r.Z, a.X, a.Z, &ga0);
+    rot2_adj(s0, c0, r.cX, r.cZ, a.cX, a.cZ, &ga0);
+    gp[PG_ANG + 0] += ga0;
+    gp[PG_ANG + 1] += ga1;
+    gp[PG_ANG + 2] += ga2;
+    gp[PG_DIST + 0] -= a.X;
+    gp[PG_DIST + 1] -= a.Y;
+    gp[PG_DIST + 2] -= a.Z;
+    r.X += p[RT_S_DIST + 0]; r.Y += p[RT_S_DIST + 1]; r.Z += p[RT_S_DIST + 2];
+}
+
+// --------------------------------------------------------------- normalize
+RT_HD void fwd_normalize(St& r) {
+    const double n = sqrt(r.cX * r.cX + r.cY * r.cY + r.cZ * r.cZ);
+    r.cX /= n; r.cY /= n; r.cZ /= n;
+}
+
+// c: direction before normalizing; a: adjoint of the normalized direction (in/out)
+RT_HD void adj_normalize(double cX, double cY, double cZ, double& aX, double& aY, double& aZ) {
+    const double n = sqrt(cX * cX + cY * cY + cZ * cZ);
+    const double uX = cX / n, uY = cY / n, uZ = cZ / n;
+    const double d = uX * aX + uY * aY + uZ * aZ;
+    aX = (aX - uX * d) / n;
+    aY = (aY - uY * d) / n;
+    aZ = (aZ - uZ * d) / n;
+}
+
+// ------------------------------------------------------------ intersection
+constexpr int kMaxNewtonAdj = 32;   // iterations differentiated exactly
+
+struct NewtonSeq {
+    int n;                          // iterations of the forward pass
+    double s0;
+    double s[kMaxNewtonAdj];        // s^(0) .. s^(n-1); the point is at s^(n-1)
+};
+
+// F and dF/ds of the iterative intersections at parameter s along the ray
+RT_HD void newton_FG(const double* p, int kind, const St& in, double s0, double s, double& F, double& G) {
+    const double u = s + s0;
+    const double X = in.X + u * in.cX, Y = in.Y + u * in.cY, Z = s * in.cZ;
+    if (kind == AK_CONIC_NEWTON) {
+        const Conic k = conic_at(p, X * X + Y * Y);
+        F = Z - k.S - k.pl.P;
+        G = -k.E * X * in.cX - k.E * Y * in.cY + in.cZ;
+    } else {
+        const double cR = p[RT_S_CR];
+        const Conic k = conic_at(p, Y * Y);
+        const double Fy = k.S + k.pl.P;
+        const double dFy = k.E * Y;
+        F = Z - Fy - 0.5 * cR * (X * X + Z * Z - Fy * Fy);
+        G = -cR * X * in.cX + (cR * Fy - 1.0) * dFy * in.cY + (1.0 - cR * Z) * in.cZ;
+    }
+}
+
+// forward intersection from `in`; n_fwd > 0 replays that many Newton
+// iterations, 0 iterates to the reference's convergence test.  false: lost
+RT_HD bool fwd_intersect(const double* p, const St& in, int n_fwd, St& out, NewtonSeq& q) {
+    const int kind = ikind(p);
+    out = in;
+    q.n = 0;
+    q.s0 = 0.0;
+    if (kind == AK_PLANE) {
+        const double s0 = -in.Z / in.cZ;
+        if (s0 < 0.0) return false;
+        out.X = in.X + in.cX * s0; out.Y = in.Y + in.cY * s0; out.Z = 0.0;
+        return true;
+    }
+    if (kind == AK_QUAD) {
+        const double c = p[RT_S_C], k1 = p[RT_S_K] + 1.0, ic = 1.0 / c;
+        const double A = k1 * in.cZ * in.cZ + in.cX * in.cX + in.cY * in.cY;
+        const double B = 2.0 * (k1 * in.Z * in.cZ + in.X * in.cX + in.Y * in.cY - in.cZ * ic);
+        const double C = k1 * in.Z * in.Z + in.X * in.X + in.Y * in.Y - 2.0 * in.Z * ic;
+        const double D = B * B - 4.0 * A * C;
+        if (D < 0.0) return false;
+        const double sq = sqrt(D), sgnB = (B < 0.0) ? -1.0 : 1.0, sigma = (c > 0) ? -1.0 : 1.0;
+        const double qq = -0.5 * (B + sgnB * sq);
+        double s;
+        if (sigma == sgnB) s = (qq != 0.0) ? C / qq : (-B + sigma * sq) / (2.0 * A);
+        else s = qq / A;
+        out.X = in.X + s * in.cX; out.Y = in.Y + s * in.cY; out.Z = in.Z + s * in.cZ;
+        return true;
+    }
+    if (kind == AK_CONIC_NEWTON || kind == AK_TORIC_NEWTON) {
+        q.s0 = -in.Z / in.cZ;
+        if (!(q.s0 >= 0)) return false;
+        double s1, s2 = 0.0;
+        int it = 0;
+        do {
+            s1 = s2;
+            if (it < kMaxNewtonAdj) q.s[it] = s1;
+            double F, G;
+            newton_FG(p, kind, in, q.s0, s1, F, G);
+            if (fabs(G) < 1.0e-7) return false;
+            s2 = s1 - F / G;
+            ++it;
+            if (n_fwd > 0 && it >= n_fwd) break;
+        } while (n_fwd > 0 || fabs(s2 - s1) >= 1.0e-7);
+        if (it > kMaxNewtonAdj) return false;
+        q.n = it;
+        const double u = s1 + q.s0;
+        out.X = in.X + u * in.cX; out.Y = in.Y + u * in.cY; out.Z = s1 * in.cZ;
+        return true;
+    }
+    return true;
+}
+
+// adjoint of the point at parameter s (X = X0 + (s + s0) cX, Y = .., Z = s cZ)
+RT_HD void point_adj(const St& in, double s0, double s, double aX, double aY, double aZ, St& ain, double& as,
+                     double& as0) {
+    const double u = s + s0;
+    const double au = aX * in.cX + aY * in.cY;
+    as += au + aZ * in.cZ;
+    as0 += au;
+    ain.X += aX;
+    ain.Y += aY;
+    ain.cX += aX * u;
+    ain.cY += aY * u;
+    ain.cZ += aZ * s;
+}
+
+// adjoint of F, G (newton_FG) at parameter s: point adjoint out (aX, aY, aZ),
+// direct direction adjoints into ain, parameters into gp
+RT_HD void newton_FG_adj(const double* p, int kind, const St& in, double s0, double s, double aF, double aG,
+                         double& aX, double& aY, double& aZ, St& ain, double* gp) {
+    const double u = s + s0;
+    const double X = in.X + u * in.cX, Y = in.Y + u * in.cY, Z = s * in.cZ;
+    aX = aY = aZ = 0.0;
+    if (kind == AK_CONIC_NEWTON) {
+        const double r2 = X * X + Y * Y;
+        const Conic k = conic_at(p, r2);
+        const double W = X * in.cX + Y * in.cY;
+        // G = -E W + cZ
+        const double aE = -aG * W, aW = -aG * k.E;
+        ain.cZ += aG;
+        aX += aW * in.cX; ain.cX += aW * X;
+        aY += aW * in.cY; ain.cY += aW * Y;
+        // F = Z - (S + P)
+        aZ += aF;
+        const double ar2 = conic_adj(p, k, r2, -aF, aE, gp);
+        aX += 2.0 * X * ar2;
+        aY += 2.0 * Y * ar2;
+    } else {
+        const double cR = p[RT_S_CR];
+        const double Y2 = Y * Y;
+        const Conic k = conic_at(p, Y2);
+        const double Fy = k.S + k.pl.P, dFy = k.E * Y;
+        // G = -cR X cX + (cR Fy - 1) dFy cY + (1 - cR Z) cZ
+        aX += -aG * cR * in.cX; ain.cX += -aG * cR * X;
+        double aFy = aG * cR * dFy * in.cY;
+        double adFy = aG * (cR * Fy - 1.0) * in.cY;
+        ain.cY += aG * (cR * Fy - 1.0) * dFy;
+        aZ += -aG * cR * in.cZ; ain.cZ += aG * (1.0 - cR * Z);
+        gp[PG_CR] += aG * (-X * in.cX + Fy * dFy * in.cY - Z * in.cZ);
+        // F = Z - Fy - cR/2 (X^2 + Z^2 - Fy^2)
+        aZ += aF * (1.0 - cR * Z);
+        aX += aF * (-cR * X);
+        aFy += aF * (-1.0 + cR * Fy);
+        gp[PG_CR] += aF * (-0.5 * (X * X + Z * Z - Fy * Fy));
+        // Fy(Y) = S + P at Y^2; dFy(Y) = E Y
+        aY += aFy * dFy + adFy * (k.E + 2.0 * Y2 * k.dEdx);
+        const double aY2 = conic_adj(p, k, Y2, aFy, adFy * Y, gp);
+        (void)aY2;   // the Y^2 chain of Fy and E is already in dFy and dE/dY above
+    }
+}
+
+// in: state before the intersection; out: the intersection point; a: adjoint of
+// `out` (becomes the adjoint of `in`)
+RT_HD void adj_intersect(const double* p, const St& in, const St& out, const NewtonSeq& q, St& a, double* gp) {
+    const int kind = ikind(p);
+    St ain = {0, 0, 0, a.cX, a.cY, a.cZ};
+    if (kind == AK_PLANE) {
+        const double s0 = -in.Z / in.cZ;
+        const double as0 = a.X * in.cX + a.Y * in.cY;
+        ain.X = a.X; ain.Y = a.Y;
+        ain.cX += a.X * s0; ain.cY += a.Y * s0;
+        ain.Z = -as0 / in.cZ;
+        ain.cZ += as0 * in.Z / (in.cZ * in.cZ);
+    } else if (kind == AK_QUAD) {
+        const double c = p[RT_S_C], k1 = p[RT_S_K] + 1.0;
+        const double s = ((out.X - in.X) * in.cX + (out.Y - in.Y) * in.cY + (out.Z - in.Z) * in.cZ) /
+                         (in.cX * in.cX + in.cY * in.cY + in.cZ * in.cZ);
+        const double gX = 2.0 * out.X, gY = 2.0 * out.Y, gZ = 2.0 * k1 * out.Z - 2.0 / c;
+        const double gc = gX * in.cX + gY * in.cY + gZ * in.cZ;
+        const double f = (a.X * in.cX + a.Y * in.cY + a.Z * in.cZ) / gc;
+        ain.X = a.X - f * gX; ain.Y = a.Y - f * gY; ain.Z = a.Z - f * gZ;
+        ain.cX += s * (a.X - f * gX); ain.cY += s * (a.Y - f * gY); ain.cZ += s * (a.Z - f * gZ);
+        gp[PG_K] += -f * out.Z * out.Z;
+        gp[PG_C] += -f * 2.0 * out.Z / (c * c);
+    } else if (kind == AK_CONIC_NEWTON || kind == AK_TORIC_NEWTON) {
+        double as = 0.0, as0 = 0.0;
+        const int n = q.n;
+        point_adj(in, q.s0, q.s[n - 1], a.X, a.Y, a.Z, ain, as, as0);
+        for (int k = n - 2; k >= 0; --k) {
+            // s^(k+1) = s^(k) - F/G at s^(k)
+            double F, G;
+            newton_FG(p, kind, in, q.s0, q.s[k], F, G);
+            const double aF = -as / G, aG = as * F / (G * G);
+            double pX, pY, pZ;
+            newton_FG_adj(p, kind, in, q.s0, q.s[k], aF, aG, pX, pY, pZ, ain, gp);
+            point_adj(in, q.s0, q.s[k], pX, pY, pZ, ain, as, as0);   // as keeps the direct term
+        }
+        // s0 = -Z / cZ
+        ain.Z += -as0 / in.cZ;
+        ain.cZ += as0 * in.Z / (in.cZ * in.cZ);
+    } else {
+        ain.X = a.X; ain.Y = a.Y; ain.Z = a.Z;
+    }
+    a = ain;
+}
+
+// ------------------------------------------------------------------ normals
+// Surface normal (unnormalised, the engine's convention) at a point and the
+// adjoint of a normal back to the point and the parameters.
+struct Normal {
+    double x, y, z;
+};
+
+RT_HD Normal normal_at(const double* p, double X, double Y, double Z) {
+    const int cls = (int)p[RT_S_CLASS];
+    if (cls == RT_CONIC) {
+        const Conic k = conic_at(p, X * X + Y * Y);
+        return Normal{-k.E * X, -k.E * Y, 1.0};
+    }
+    if (cls == RT_TORIC) {
+        const double cR = p[RT_S_CR];
+        const Conic k = conic_at(p, Y * Y);
+        const double Fy = k.S + k.pl.P, dFy = k.E * Y;
+        return Normal{-cR * X, (cR * Fy - 1.0) * dFy, 1.0 - cR * Z};
+    }
+    return Normal{0.0, 0.0, 1.0};
+}
+
+RT_HD void normal_adj(const double* p, double X, double Y, double Z, const Normal& aN, St& apt, double* gp) {
+    const int cls = (int)p[RT_S_CLASS];
+    if (cls == RT_CONIC) {
+        const double r2 = X * X + Y * Y;
+        const Conic k = conic_at(p, r2);
+        const double aE = -(aN.x * X + aN.y * Y);
+        apt.X += -k.E * aN.x;
+        apt.Y += -k.E * aN.y;
+        const double ar2 = conic_adj(p, k, r2, 0.0, aE, gp);
+        apt.X += 2.0 * X * ar2;
+        apt.Y += 2.0 * Y * ar2;
+    } else if (cls == RT_TORIC) {
+        const double cR = p[RT_S_CR];
+        const double Y2 = Y * Y;
+        const Conic k = conic_at(p, Y2);
+        const double Fy = k.S + k.pl.P, dFy = k.E * Y;
+        apt.X += -cR * aN.x;
+        apt.Z += -cR * aN.z;
+        gp[PG_CR] += -X * aN.x + Fy * dFy * aN.y - Z * aN.z;
+        const double aFy = aN.y * cR * dFy, adFy = aN.y * (cR * Fy - 1.0);
+        apt.Y += aFy * dFy + adFy * (k.E + 2.0 * Y2 * k.dEdx);
+        (void)conic_adj(p, k, Y2, aFy, adFy * Y, gp);
+    }
+}
+
+// ------------------------------------------------------------------ actions
+// The action's output direction (before the normalisation that follows);
+// false for total internal reflection or an evanescent order.
+RT_HD bool fwd_action(const double* p, const St& r, double mu, double L, double& oX, double& oY, double& oZ) {
+    const int type = (int)p[RT_S_TYPE], cls = (int)p[RT_S_CLASS];
+    oX = r.cX; oY = r.cY; oZ = r.cZ;
+    if (type == RT_MIRROR) {
+        if (cls == RT_PLANE) { oZ = -r.cZ; return true; }
+        const Normal N = normal_at(p, r.X, r.Y, r.Z);
+        const double a2 = 2.0 * (r.cX * N.x + r.cY * N.y + r.cZ * N.z) / (N.x * N.x + N.y * N.y + N.z * N.z);
+        oX = r.cX - a2 * N.x; oY = r.cY - a2 * N.y; oZ = r.cZ - a2 * N.z;
+        return true;
+    }
+    if (type == RT_LENS) {
+        if (cls == RT_PLANE) {
+            const double a = mu * r.cZ, b = mu * mu - 1.0;
+            if (b > a * a) return false;
+            const double G = -a + sqrt(a * a - b);
+            oX = mu * r.cX; oY = mu * r.cY; oZ = mu * r.cZ + G;
+            return true;
+        }
+        const Normal N = normal_at(p, r.X, r.Y, r.Z);
+        const double M = N.x * N.x + N.y * N.y + N.z * N.z;
+        const double a = mu * (r.cX * N.x + r.cY * N.y + r.cZ * N.z) / M, b = (mu * mu - 1.0) / M;
+        if (b > a * a) return false;
+        const double G = -a + sqrt(a * a - b);
+        oX = mu * r.cX + G * N.x; oY = mu * r.cY + G * N.y; oZ = mu * r.cZ + G * N.z;
+        return true;
+    }
+    if (type == RT_GRATING) {
+        if (cls != RT_PLANE) return false;                 // curved gratings: no adjoint
+        if ((int)p[RT_S_GRTYPE] == RT_REFLECTIVE) {
+            const double a = r.cZ, b = L * L - 2.0 * L * r.cX;
+            if (b > a * a) return false;
+            const double G = -a - sqrt(a * a - b);
+            oX = r.cX - L; oZ = r.cZ + G;
+        } else {
+            const double a = mu * r.cZ, b = mu * mu - 1.0 + L * L - 2.0 * mu * L * r.cX;
+            if (b > a * a) return false;
+            const double G = -a + sqrt(a * a - b);
+            oX = mu * r.cX - L; oY = mu * r.cY; oZ = mu * r.cZ + G;
+        }
+        return true;
+    }
+    return true;
+}
+
+// r: state on the surface before the action; (aX, aY, aZ): adjoint of the
+// action's output direction; a: receives the adjoint of r (its point part is
+// added to, its direction part set); aL: adjoint of L
+RT_HD bool adj_action(const double* p, const St& r, double mu, double L, double aX, double aY, double aZ, St& a,
+                      double& aL, double* gp) {
+    const int type = (int)p[RT_S_TYPE], cls = (int)p[RT_S_CLASS];
+    aL = 0.0;
+    if (type == RT_MIRROR) {
+        if (cls == RT_PLANE) { a.cX = aX; a.cY = aY; a.cZ = -aZ; return true; }
+        const Normal N = normal_at(p, r.X, r.Y, r.Z);
+        const double D = r.cX * N.x + r.cY * N.y + r.cZ * N.z, M = N.x * N.x + N.y * N.y + N.z * N.z;
+        const double aoN = aX * N.x + aY * N.y + aZ * N.z;
+        const double aD = -2.0 / M * aoN, aM = 2.0 * D / (M * M) * aoN;
+        Normal aN = {-2.0 * D / M * aX + aD * r.cX + 2.0 * aM * N.x, -2.0 * D / M * aY + aD * r.cY + 2.0 * aM * N.y,
+                     -2.0 * D / M * aZ + aD * r.cZ + 2.0 * aM * N.z};
+        a.cX = aX + aD * N.x; a.cY = aY + aD * N.y; a.cZ = aZ + aD * N.z;
+        normal_adj(p, r.X, r.Y, r.Z, aN, a, gp);
+        return true;
+    }
+    if (type == RT_LENS) {
+        if (cls == RT_PLANE) {
+            const double av = mu * r.cZ, b = mu * mu - 1.0, sq = sqrt(av * av - b);
+            const double aa = aZ * (-1.0 + av / sq);
+            a.cX = mu * aX; a.cY = mu * aY; a.cZ = mu * aZ + aa * mu;
+            return true;
+        }
+        const Normal N = normal_at(p, r.X, r.Y, r.Z);
+        const double M = N.x * N.x + N.y * N.y + N.z * N.z;
+        const double D = r.cX * N.x + r.cY * N.y + r.cZ * N.z;
+        const double av = mu * D / M, b = (mu * mu - 1.0) / M, sq = sqrt(av * av - b), G = -av + sq;
+        const double aG = aX * N.x + aY * N.y + aZ * N.z;
+        const double aa = aG * (-1.0 + av / sq), ab = -aG / (2.0 * sq);
+        const double aD = aa * mu / M, aM = -aa * av / M - ab * b / M;
+        Normal aN = {G * aX + aD * r.cX + 2.0 * aM * N.x, G * aY + aD * r.cY + 2.0 * aM * N.y,
+                     G * aZ + aD * r.cZ + 2.0 * aM * N.z};
+        a.cX = mu * aX + aD * N.x; a.cY = mu * aY + aD * N.y; a.cZ = mu * aZ + aD * N.z;
+        normal_adj(p, r.X, r.Y, r.Z, aN, a, gp);
+        return true;
+    }
+    if (type == RT_GRATING) {
+        if (cls != RT_PLANE) return false;
+        if ((int)p[RT_S_GRTYPE] == RT_REFLECTIVE) {
+            const double av = r.cZ, b = L * L - 2.0 * L * r.cX, sq = sqrt(av * av - b);
+            const double aG = aZ, aa = aG * (-1.0 - av / sq), ab = aG / (2.0 * sq);
+            a.cX = aX + ab * (-2.0 * L);
+            a.cY = aY;
+            a.cZ = aZ + aa;
+            aL = -aX + ab * (2.0 * L - 2.0 * r.cX);
+        } else {
+            const double av = mu * r.cZ, b = mu * mu - 1.0 + L * L - 2.0 * mu * L * r.cX, sq = sqrt(av * av - b);
+            const double aG = aZ, aa = aG * (-1.0 + av / sq), ab = -aG / (2.0 * sq);
+            a.cX = mu * aX + ab * (-2.0 * mu * L);
+            a.cY = mu * aY;
+            a.cZ = mu * aZ + aa * mu;
+            aL = -aX + ab * (2.0 * L - 2.0 * mu * r.cX);
+        }
+        return true;
+    }
+    a.cX = aX; a.cY = aY; a.cZ = aZ;
+    return true;
+}
+
+// ---------------------------------------------------------------- surface
+// entry: ray state on entry to the surface (previous frame); n_fwd: Newton
+// iterations of the forward pass (0: recompute to convergence); mu = n1/n2
+// and L = m lambda / n2 / d of the forward pass (lenses, gratings); a: adjoint
+// of the state after the surface (in), of the entry state (out).  false when
+// the surface has no adjoint here (curved gratings) or the replayed ray is lost.
+RT_HD bool surface_adjoint(const double* p, const St& entry, int n_fwd, double mu, double L, St& a, double* gp) {
+    const int type = (int)p[RT_S_TYPE];
+    St sT = entry;
+    fwd_transform(p, sT);
+    St sN = sT;
+    fwd_normalize(sN);
+    if (type == RT_AUXTYPE) {
+        adj_normalize(sT.cX, sT.cY, sT.cZ, a.cX, a.cY, a.cZ);
+        adj_transform(p, sT, a, gp);
+        return true;
+    }
+    St sI;
+    NewtonSeq q;
+    if (!fwd_intersect(p, sN, n_fwd, sI, q)) return false;
+    double oX, oY, oZ;
+    if (!fwd_action(p, sI, mu, L, oX, oY, oZ)) return false;
+    // normalisation after the action
+    adj_normalize(oX, oY, oZ, a.cX, a.cY, a.cZ);
+    // action: point adjoint accumulates, direction adjoint replaced
+    St aI = {a.X, a.Y, a.Z, 0.0, 0.0, 0.0};
+    double aL = 0.0;
+    if (!adj_action(p, sI, mu, L, a.cX, a.cY, a.cZ, aI, aL, gp)) return false;
+    if (type == RT_GRATING && L != 0.0) gp[PG_D] += aL * (-L / p[RT_S_GRCONST]);
+    // intersection, first normalisation, transform
+    adj_intersect(p, sN, sI, q, aI, gp);
+    adj_normalize(sT.cX, sT.cY, sT.cZ, aI.cX, aI.cY, aI.cZ);
+    adj_transform(p, sT, aI, gp);
+    a = aI;
+    return true;
+}
+
+}  // namespace adj
+}  // namespace rt

@@ -784,6 +784,7 @@ struct TraceCtx {
     bool do_hist, do_flux;
     unsigned n_oob, n_hit, n_off, n_edge;
     double n_prev;                // n2 of the last action (reused where the next n1 is the same table)
+    unsigned last_it;             // Newton iterations of the last intersection (adjoint tape)
 };
 
 __device__ __forceinline__ void lds_add(unsigned* counter, unsigned v) {
@@ -838,6 +839,7 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
         alive = intersect_ray(d, r, it);                         // :1002
         lost_int = !alive;
     }
+    c.last_it = it;
     if (alive) {
         alive = mask_ray(d, r.X, r.Y);                           // :1014
         if (alive) {                                             // :1035

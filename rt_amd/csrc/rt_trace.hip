@@ -900,3 +900,15 @@ extern "C" int rt_hip_graph_destroy(rt_hip_graph* g) {
     delete g;
     return RT_OK;
 }
+
+// device tables of a scheme, for the other translation units of the library
+extern "C" int rt_hip_scheme_device_tables(const rt_hip_scheme* s, const double** table_dev, const double** pool_dev,
+                                           int* n_surf, size_t* pool_len, const double** host_table) {
+    if (!s) return set_error(RT_ERR_BAD_VALUE, "scheme is NULL");
+    *table_dev = s->table_dev;
+    *pool_dev = s->pool_dev;
+    *n_surf = s->n_surf;
+    *pool_len = s->pool_len;
+    *host_table = s->table.data();
+    return RT_OK;
+}

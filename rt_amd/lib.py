@@ -55,6 +55,8 @@ def lib():
         L.rt_hip_batch_trace.argtypes = [_vp, _dp, C.c_uint64, C.c_int] + [_vp] * 8 + [_vp, _vp, _vp]
         L.rt_hip_batch_graph_create.argtypes = ([_vp, _dp, C.c_uint64, C.c_int] + [_vp] * 8 +
                                                 [_vp, _vp, _vp, C.POINTER(_vp)])
+        L.rt_hip_adjoint.argtypes = [_vp, _dp, C.c_uint64, C.c_size_t, _vp, C.c_size_t,
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, _vp, _vp, _vp]
         L.rt_hip_graph_launch.argtypes = [_vp, _vp]
         L.rt_hip_graph_destroy.argtypes = [_vp]
         L.rt_hip_jit_check.argtypes = [_dp, C.c_int, _dp, C.c_size_t, C.c_int, C.c_char_p, C.c_size_t,
@@ -157,6 +159,29 @@ class DeviceScheme:
         rc = lib().rt_hip_trace(self.handle, _dptr(b), seed, begin, n, *ptrs, _tp(flag), flags,
                                 _tp(hist), _tp(flux), _tp(stats), _tp(dump), _tp(dump_flag), stream)
         check(rc, "rt_hip_trace")
+
+    def adjoint(self, group_seeds, param_surface, param_field, *, seed=0, n=None, grad=None, stats=None,
+                beam=None, stream=None):
+        """rt_hip_adjoint: dLoss/dparameters for a loss seeded per ray group by
+        group_seeds (CUDA float64 tensor, n_groups x 3: ax, ay, b).  Returns grad
+        (CUDA float64 tensor, one value per parameter)."""
+        import torch
+        b = np.ascontiguousarray((beam if beam is not None else self.scheme.beam).pack())
+        if n is None:
+            n = self.scheme.beam.n_rays
+        ps = np.ascontiguousarray(param_surface, dtype=np.int32)
+        pf = np.ascontiguousarray(param_field, dtype=np.int32)
+        assert len(ps) == len(pf)
+        assert group_seeds.dtype == torch.float64 and group_seeds.is_cuda and group_seeds.is_contiguous()
+        if grad is None:
+            grad = torch.zeros(len(ps), dtype=torch.float64, device=group_seeds.device)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        ip = C.POINTER(C.c_int)
+        check(lib().rt_hip_adjoint(self.handle, _dptr(b), seed, n, group_seeds.data_ptr(), group_seeds.shape[0],
+                                   ps.ctypes.data_as(ip), pf.ctypes.data_as(ip), len(ps), _tp(grad), _tp(stats),
+                                   stream), "rt_hip_adjoint")
+        return grad
 
     def set_launch(self, block: int = 256, waves: int = 0, grid_mult: int = 2):
         """Launch of the generated kernel (rt_hip_scheme_set_launch)."""

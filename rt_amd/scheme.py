@@ -51,7 +51,7 @@ BEAM_STRIDE = 16
 
 RT_TRACE_GENERATE, RT_TRACE_STORE, RT_TRACE_HIST, RT_TRACE_FLUX = 1, 2, 4, 8
 
-ST_RAYS, ST_GOOD, ST_CCD_HITS, ST_CCD_OFFCHIP, ST_CCD_EDGE, ST_SPLINE_OOB = range(6)
+ST_RAYS, ST_GOOD, ST_CCD_HITS, ST_CCD_OFFCHIP, ST_CCD_EDGE, ST_SPLINE_OOB, ST_ADJ_SKIP = range(7)
 ST_SURF = 16
 STATS_LEN = ST_SURF + 4 * RT_MAX_SURF
 

@@ -201,18 +201,23 @@ def echelle(n_per_lambda: int = 4883, nlambda: int = 2048) -> Scheme:
 
 def tolerance_instances(base: Scheme, n_inst: int, seed: int = 0, decenter_mm: float = 0.020,
                         tilt_arcsec: float = 30.0, curvature_rel: float = 1e-4,
-                        min_rays: int = 10_000, max_rays: int = 1_000_000):
-    """Monte-Carlo tolerance instances of `base`: every surface decentred by
-    N(0, decenter_mm) in x and y, tilted by N(0, tilt_arcsec) about y and x,
-    curvatures (c and, for torics, cR) scaled by 1 + N(0, curvature_rel); ray
-    counts log-uniform in [min_rays, max_rays].  Returns (schemes, counts);
-    the instances share the base's tables (spline objects), so they pack to
-    one spline pool."""
+                        min_rays: int = 10_000, max_rays: int = 1_000_000, first: int = None):
+    """Monte-Carlo tolerance instances of `base`: every surface from `first`
+    on decentred by N(0, decenter_mm) in x and y, tilted by N(0, tilt_arcsec)
+    about y and x, curvatures (c and, for torics, cR) scaled by
+    1 + N(0, curvature_rel); ray counts log-uniform in [min_rays, max_rays].
+    `first` defaults to the surface after the last stop (the slit of the
+    echelle: the optics that only illuminate the slit are not toleranced).
+    Returns (schemes, counts); the instances share the base's tables (spline
+    objects), so they pack to one spline pool."""
     rng = np.random.default_rng(seed)
+    if first is None:
+        stops = [i for i, s in enumerate(base.surfaces) if s.type == RT_STOP]
+        first = stops[-1] + 1 if stops else 0
     out, counts = [], []
     for _ in range(n_inst):
-        surfs = []
-        for srf in base.surfaces:
+        surfs = list(base.surfaces[:first])
+        for srf in base.surfaces[first:]:
             dx, dy = rng.normal(0.0, decenter_mm, 2)
             ty, tx = rng.normal(0.0, tilt_arcsec / 3600.0, 2)
             dist = (srf.distance[0] + dx, srf.distance[1] + dy, srf.distance[2])

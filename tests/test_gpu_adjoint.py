@@ -1,0 +1,138 @@
+"""The adjoint kernel (rt_hip_adjoint) and the differentiable spot loss.
+
+* GPU adjoint against the host build of the same derivative formulas reversed
+  over the oracle's forward states (tests/native): 1e-9 relative.
+* Gradient of the spot loss (sum over wavelengths of RMS radius plus the
+  squared distance of the centroid from a dispersion line) over the echelle's
+  48 design parameters against central finite differences with h = 1e-7
+  relative (1e-7 absolute for parameters at 0), of the GPU forward and of the
+  oracle: |adjoint - FD| <= 1e-6 |FD| + the FD's rounding noise (8 eps |loss| /
+  2h), for parameters whose FD runs keep the same surviving rays and Newton
+  iteration counts (checked; at least 40 of the 48 must qualify).
+* The torch.autograd.Function returns the same gradient.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from rt_amd import diff
+from rt_amd import scheme as S
+from rt_amd import schemes
+from oracle import oracle as O
+
+from test_adjoint_cpu import chain_adjoint, host  # noqa: F401  (fixture)
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_adjoint(hip, sch, seed, seeds_np, ps, pf):
+    ds = hip.DeviceScheme(sch)
+    seeds = torch.from_numpy(np.ascontiguousarray(seeds_np)).cuda()
+    st = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
+    print(f"[adjoint] {sch.name} n={sch.beam.n_rays} params={len(ps)}", flush=True)
+    g = ds.adjoint(seeds, ps, pf, seed=seed, stats=st)
+    torch.cuda.synchronize()
+    ds.close()
+    return g.cpu().numpy(), st.cpu().numpy().view(np.uint64)
+
+
+def test_gpu_adjoint_matches_host_chain(hip, host):  # noqa: F811
+    sch = schemes.echelle(n_per_lambda=3, nlambda=64)
+    seed = 5
+    G = 64
+    rng = np.random.default_rng(1)
+    seeds = np.stack([rng.normal(size=G), rng.normal(size=G), rng.normal(size=G) * 0.1], axis=1)
+    ref = O.trace(sch, seed=seed)
+    gidx = np.arange(sch.beam.n_rays) % G
+    u = seeds[gidx, 0] + seeds[gidx, 2] * ref.rays[0]
+    v = seeds[gidx, 1] + seeds[gidx, 2] * ref.rays[1]
+    H, _ = chain_adjoint(host, sch, seed, u, v)
+    spec = diff.echelle_design_params(sch)
+    ps = [p.surface for p in spec.params]
+    pf = [p.field for p in spec.params]
+    g, st = gpu_adjoint(hip, sch, seed, seeds, ps, pf)
+    assert int(st[S.ST_ADJ_SKIP]) == 0
+    assert int(st[S.ST_GOOD]) == int(np.sum(ref.flag == 1))
+    h = np.array([H[s, host.adj_host_slot(f)] for s, f in zip(ps, pf)])
+    scale = np.maximum(np.abs(h), 1e-12 * np.max(np.abs(h)))
+    err = np.abs(g - h) / scale
+    assert np.max(err) <= 1e-9, (int(np.argmax(err)), float(np.max(err)))
+
+
+def _oracle_loss(sch, seed, n_groups, target, weight):
+    r = O.trace(sch, seed=seed)
+    ok = r.flag == 1
+    g = np.arange(sch.beam.n_rays) % n_groups
+    loss = 0.0
+    for k in range(n_groups):
+        m = ok & (g == k)
+        if not m.any():
+            continue
+        x, y = r.rays[0, m], r.rays[1, m]
+        cx, cy = x.mean(), y.mean()
+        rms = math.sqrt(np.mean((x - cx) ** 2 + (y - cy) ** 2))
+        loss += rms + weight * ((cx - target[k, 0]) ** 2 + (cy - target[k, 1]) ** 2)
+    return loss, r
+
+
+def _counters(st, n_surf):
+    return np.array([int(st[S.ST_SURF + 4 * s + k]) for s in range(n_surf) for k in (1, 2, 3)])
+
+
+def test_spot_loss_gradient_matches_finite_differences(hip):
+    base = schemes.echelle(n_per_lambda=48, nlambda=64)
+    spec = diff.echelle_design_params(base)
+    n_groups, seed, weight = 64, 9, 0.5
+    v0 = spec.values(base)
+    # target: the least-squares dispersion line through the nominal centroids, in wavelength
+    loss0 = diff.SpotLoss(base, spec, n_groups=n_groups, seed=seed)
+    cnt, cx, cy, rms = (t.cpu().numpy() for t in loss0.spots(v0))
+    lam = np.array([O.gen_rays(base.beam.pack(), seed, k, 1)[6, 0] for k in range(n_groups)])
+    tx = np.polyval(np.polyfit(lam, cx, 1), lam)
+    ty = np.polyval(np.polyfit(lam, cy, 1), lam)
+    target = np.stack([tx, ty], axis=1)
+    L = diff.SpotLoss(base, spec, n_groups=n_groups, target=target, weight=weight, seed=seed)
+    val, grad = L.value_and_grad(v0)
+    oval, oref = _oracle_loss(spec.apply(base, v0), seed, n_groups, target, weight)
+    assert abs(val - oval) <= 1e-11 * abs(oval)
+    base_cnt = _counters(oref.stats, base.n_surf)
+    checked = 0
+    for j, p in enumerate(spec.params):
+        x = v0[j]
+        h = 1e-7 * abs(x) if x != 0.0 else 1e-7
+        vp, vm = v0.copy(), v0.copy()
+        vp[j] += h
+        vm[j] -= h
+        lp, rp = _oracle_loss(spec.apply(base, vp), seed, n_groups, target, weight)
+        lm, rm = _oracle_loss(spec.apply(base, vm), seed, n_groups, target, weight)
+        same = (np.array_equal(rp.flag, oref.flag) and np.array_equal(rm.flag, oref.flag) and
+                np.array_equal(_counters(rp.stats, base.n_surf), base_cnt) and
+                np.array_equal(_counters(rm.stats, base.n_surf), base_cnt))
+        if not same:
+            print(f"[fd] parameter {j} ({p.surface}, {p.field}): rays or iterations change, skipped")
+            continue
+        noise = 8 * np.finfo(float).eps * abs(val) / (2 * h)
+        fd_cpu = (lp - lm) / (2 * h)
+        gp_, _ = L.value_and_grad(vp, grad=False)
+        gm_, _ = L.value_and_grad(vm, grad=False)
+        fd_gpu = (gp_ - gm_) / (2 * h)
+        print(f"[fd] {j:2d} s{p.surface} f{p.field} adj {grad[j]:+.9e} fd_cpu {fd_cpu:+.9e} fd_gpu {fd_gpu:+.9e}")
+        assert abs(grad[j] - fd_cpu) <= 1e-6 * abs(fd_cpu) + noise, (j, grad[j], fd_cpu, noise)
+        assert abs(grad[j] - fd_gpu) <= 1e-6 * abs(fd_gpu) + noise, (j, grad[j], fd_gpu, noise)
+        checked += 1
+    assert checked >= 40, checked
+
+
+def test_autograd_function(hip):
+    base = schemes.echelle(n_per_lambda=16, nlambda=64)
+    spec = diff.echelle_design_params(base)
+    L = diff.SpotLoss(base, spec, n_groups=64, seed=2)
+    v0 = spec.values(base)
+    _, grad = L.value_and_grad(v0)
+    SchemeLoss = diff.scheme_loss_function()
+    x = torch.tensor(v0, dtype=torch.float64, requires_grad=True)
+    loss = SchemeLoss.apply(x, L)
+    loss.backward()
+    assert np.array_equal(x.grad.numpy(), grad)
