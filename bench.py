@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--batch", type=int, default=0, metavar="N_INST",
                     help="measure the batched workload instead: N_INST Monte-Carlo tolerance instances of "
                          "the echelle, ray counts log-uniform in [1e4, 1e6], one persistent launch per step")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="measure the adjoint instead: gradient of the spot loss over the echelle's 48 design "
+                         "parameters at 1e6 rays over 64 wavelengths, as a time ratio to the plain forward trace")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -78,6 +81,8 @@ def main():
     lib.init(0)
     if args.batch:
         return bench_batch(args)
+    if args.adjoint:
+        return bench_adjoint(args)
     sch = schemes.echelle(args.n_per_lambda, args.nlambda)
     n = sch.beam.n_rays
     ds = lib.DeviceScheme(sch)
@@ -164,6 +169,63 @@ def main():
                                "kind": "port",
                                "sample": (f"the whole workload ({m} rays)" if m == n else f"first {m} rays of the same beam") + ", oracle fused trace, one pass"}
     print(json.dumps(out))
+
+
+def bench_adjoint(args):
+    """Adjoint/forward time ratio on the differentiable configuration: the
+    echelle at 15625 rays x 64 wavelengths (1e6 rays), loss = sum over
+    wavelengths of the RMS spot radius + 0.5 x squared centroid deviation from
+    the nominal dispersion line, gradient over 48 design parameters.  Times:
+    the plain forward trace (beam generated, nothing stored), the loss forward
+    (rays stored, spots reduced), the adjoint launch; ratio = (loss forward +
+    adjoint) / plain forward, per the same rays and surfaces."""
+    import torch
+    from rt_amd import diff, lib, schemes
+    from rt_amd import scheme as S
+    base = schemes.echelle(n_per_lambda=15625, nlambda=64)
+    spec = diff.echelle_design_params(base)
+    v0 = spec.values(base)
+    n = base.beam.n_rays
+    ds = lib.DeviceScheme(base)
+    rays = torch.zeros((7, n), dtype=torch.float64, device="cuda")
+    flag = torch.zeros(n, dtype=torch.int8, device="cuda")
+    ps = np.array([p.surface for p in spec.params], dtype=np.int32)
+    pf = np.array([p.field for p in spec.params], dtype=np.int32)
+    seeds = torch.zeros((64, 3), dtype=torch.float64, device="cuda")
+
+    def loss_forward():
+        ds.trace(rays, flag, seed=5, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE)
+        cnt, cx, cy, rms = diff.group_spots(rays, flag, 64)
+        safe = torch.clamp(cnt, min=1.0)
+        b = 1.0 / (safe * rms)
+        seeds[:, 0] = -b * cx
+        seeds[:, 1] = -b * cy
+        seeds[:, 2] = b
+        return rms
+
+    def timed(fn, k):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(k):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    plain = lambda: ds.trace(None, None, seed=5, n=n, flags=S.RT_TRACE_GENERATE)
+    for _ in range(args.warmup):
+        plain(); loss_forward(); ds.adjoint(seeds, ps, pf, seed=5)
+    t_plain = timed(plain, args.steps)
+    t_loss = timed(loss_forward, args.steps)
+    t_adj = timed(lambda: ds.adjoint(seeds, ps, pf, seed=5), args.steps)
+    print(json.dumps({
+        "metric": "adjoint/forward time ratio (echelle, 1e6 rays, 48 parameters)",
+        "value": (t_loss + t_adj) / t_plain, "unit": "x", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": False, "dtype": "f64",
+        "data": "synthetic (Philox beam)",
+        "config": {"workload": "echelle-adjoint", "rays": n, "wavelengths": 64, "parameters": len(spec)},
+        "detail": {"forward_ms": t_plain, "loss_forward_ms": t_loss, "adjoint_ms": t_adj}}))
 
 
 def bench_batch(args):

@@ -25,14 +25,43 @@
 #include "rt_hip.h"
 
 #if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+#include "rt_device.h"     // the refined reciprocal / reciprocal square root of the device build
 #define RT_HD __host__ __device__ inline
+#define RT_UNROLL _Pragma("unroll")
 #else
 #include <math.h>
 #define RT_HD static inline
+#define RT_UNROLL
 #endif
 
 namespace rt {
 namespace adj {
+
+// reciprocal and reciprocal square root: the refined hardware estimates of
+// rt_device.h on the device, IEEE on the host
+RT_HD double drcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rt::rcp(x);
+#else
+    return 1.0 / x;
+#endif
+}
+
+RT_HD double drsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rt::rsqrt(x);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+
+RT_HD double dsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rt::fsqrt(x);
+#else
+    return sqrt(x);
+#endif
+}
 
 // parameter slots of one surface
 enum ParamSlot {
@@ -118,17 +147,18 @@ RT_HD Conic conic_at(const double* p, double x) {
     Conic o;
     const double K = k1 * c * c;
     const double t = 1.0 - K * x;
-    o.q = sqrt(t);
-    const double q = o.q, q3 = t * q, opq = 1.0 + q;
+    const double rq = drsqrt(t);               // 1/q
+    o.q = t * rq;
+    const double rq3 = rq * rq * rq, iopq = drcp(1.0 + o.q), iopq2 = iopq * iopq;
     o.pl = poly(p, x);
-    o.S = c * x / opq;
-    o.E = c / q + 2.0 * o.pl.Q;
-    o.dSdx = c / (2.0 * q);
-    o.dEdx = c * K / (2.0 * q3) + 2.0 * o.pl.R;
-    o.dSdc = x / opq + k1 * c * c * x * x / (q * opq * opq);
-    o.dSdk = c * c * c * x * x / (2.0 * q * opq * opq);
-    o.dEdc = 1.0 / q + k1 * c * c * x / q3;
-    o.dEdk = c * c * c * x / (2.0 * q3);
+    o.S = c * x * iopq;
+    o.E = c * rq + 2.0 * o.pl.Q;
+    o.dSdx = 0.5 * c * rq;
+    o.dEdx = 0.5 * c * K * rq3 + 2.0 * o.pl.R;
+    o.dSdc = x * iopq + k1 * c * c * x * x * rq * iopq2;
+    o.dSdk = 0.5 * c * c * c * x * x * rq * iopq2;
+    o.dEdc = rq + k1 * c * c * x * rq3;
+    o.dEdk = 0.5 * c * c * c * x * rq3;
     return o;
 }
 
@@ -190,22 +220,23 @@ RT_HD void adj_transform(const double* p, St& r, St& a, double* gp) {
 
 // --------------------------------------------------------------- normalize
 RT_HD void fwd_normalize(St& r) {
-    const double n = sqrt(r.cX * r.cX + r.cY * r.cY + r.cZ * r.cZ);
-    r.cX /= n; r.cY /= n; r.cZ /= n;
+    const double in = drsqrt(r.cX * r.cX + r.cY * r.cY + r.cZ * r.cZ);
+    r.cX *= in; r.cY *= in; r.cZ *= in;
 }
 
 // c: direction before normalizing; a: adjoint of the normalized direction (in/out)
 RT_HD void adj_normalize(double cX, double cY, double cZ, double& aX, double& aY, double& aZ) {
-    const double n = sqrt(cX * cX + cY * cY + cZ * cZ);
-    const double uX = cX / n, uY = cY / n, uZ = cZ / n;
+    const double in = drsqrt(cX * cX + cY * cY + cZ * cZ);
+    const double uX = cX * in, uY = cY * in, uZ = cZ * in;
     const double d = uX * aX + uY * aY + uZ * aZ;
-    aX = (aX - uX * d) / n;
-    aY = (aY - uY * d) / n;
-    aZ = (aZ - uZ * d) / n;
+    aX = (aX - uX * d) * in;
+    aY = (aY - uY * d) * in;
+    aZ = (aZ - uZ * d) * in;
 }
 
 // ------------------------------------------------------------ intersection
-constexpr int kMaxNewtonAdj = 32;   // iterations differentiated exactly
+constexpr int kMaxNewtonAdj = 8;    // iterations differentiated exactly (the loops below are unrolled
+                                    // over this bound, so the iterates stay in registers)
 
 struct NewtonSeq {
     int n;                          // iterations of the forward pass
@@ -239,42 +270,46 @@ RT_HD bool fwd_intersect(const double* p, const St& in, int n_fwd, St& out, Newt
     q.n = 0;
     q.s0 = 0.0;
     if (kind == AK_PLANE) {
-        const double s0 = -in.Z / in.cZ;
+        const double s0 = -in.Z * drcp(in.cZ);
         if (s0 < 0.0) return false;
         out.X = in.X + in.cX * s0; out.Y = in.Y + in.cY * s0; out.Z = 0.0;
         return true;
     }
     if (kind == AK_QUAD) {
-        const double c = p[RT_S_C], k1 = p[RT_S_K] + 1.0, ic = 1.0 / c;
+        const double c = p[RT_S_C], k1 = p[RT_S_K] + 1.0, ic = drcp(c);
         const double A = k1 * in.cZ * in.cZ + in.cX * in.cX + in.cY * in.cY;
         const double B = 2.0 * (k1 * in.Z * in.cZ + in.X * in.cX + in.Y * in.cY - in.cZ * ic);
         const double C = k1 * in.Z * in.Z + in.X * in.X + in.Y * in.Y - 2.0 * in.Z * ic;
         const double D = B * B - 4.0 * A * C;
         if (D < 0.0) return false;
-        const double sq = sqrt(D), sgnB = (B < 0.0) ? -1.0 : 1.0, sigma = (c > 0) ? -1.0 : 1.0;
+        const double sq = dsqrt(D), sgnB = (B < 0.0) ? -1.0 : 1.0, sigma = (c > 0) ? -1.0 : 1.0;
         const double qq = -0.5 * (B + sgnB * sq);
         double s;
-        if (sigma == sgnB) s = (qq != 0.0) ? C / qq : (-B + sigma * sq) / (2.0 * A);
-        else s = qq / A;
+        if (sigma == sgnB) s = (qq != 0.0) ? C * drcp(qq) : (-B + sigma * sq) * drcp(2.0 * A);
+        else s = qq * drcp(A);
         out.X = in.X + s * in.cX; out.Y = in.Y + s * in.cY; out.Z = in.Z + s * in.cZ;
         return true;
     }
     if (kind == AK_CONIC_NEWTON || kind == AK_TORIC_NEWTON) {
-        q.s0 = -in.Z / in.cZ;
+        q.s0 = -in.Z * drcp(in.cZ);
         if (!(q.s0 >= 0)) return false;
-        double s1, s2 = 0.0;
+        double s1 = 0.0, s2 = 0.0;
         int it = 0;
-        do {
-            s1 = s2;
-            if (it < kMaxNewtonAdj) q.s[it] = s1;
-            double F, G;
-            newton_FG(p, kind, in, q.s0, s1, F, G);
-            if (fabs(G) < 1.0e-7) return false;
-            s2 = s1 - F / G;
-            ++it;
-            if (n_fwd > 0 && it >= n_fwd) break;
-        } while (n_fwd > 0 || fabs(s2 - s1) >= 1.0e-7);
-        if (it > kMaxNewtonAdj) return false;
+        bool done = false;
+RT_UNROLL
+        for (int k = 0; k < kMaxNewtonAdj; ++k) {
+            if (!done) {
+                s1 = s2;
+                q.s[k] = s1;
+                double F, G;
+                newton_FG(p, kind, in, q.s0, s1, F, G);
+                if (fabs(G) < 1.0e-7) return false;
+                s2 = s1 - F * drcp(G);
+                it = k + 1;
+                done = n_fwd > 0 ? it >= n_fwd : !(fabs(s2 - s1) >= 1.0e-7);
+            }
+        }
+        if (!done) return false;            // more iterations than are differentiated here
         q.n = it;
         const double u = s1 + q.s0;
         out.X = in.X + u * in.cX; out.Y = in.Y + u * in.cY; out.Z = s1 * in.cZ;
@@ -348,39 +383,47 @@ RT_HD void adj_intersect(const double* p, const St& in, const St& out, const New
     const int kind = ikind(p);
     St ain = {0, 0, 0, a.cX, a.cY, a.cZ};
     if (kind == AK_PLANE) {
-        const double s0 = -in.Z / in.cZ;
+        const double icZ = drcp(in.cZ);
+        const double s0 = -in.Z * icZ;
         const double as0 = a.X * in.cX + a.Y * in.cY;
         ain.X = a.X; ain.Y = a.Y;
         ain.cX += a.X * s0; ain.cY += a.Y * s0;
-        ain.Z = -as0 / in.cZ;
-        ain.cZ += as0 * in.Z / (in.cZ * in.cZ);
+        ain.Z = -as0 * icZ;
+        ain.cZ += as0 * in.Z * icZ * icZ;
     } else if (kind == AK_QUAD) {
-        const double c = p[RT_S_C], k1 = p[RT_S_K] + 1.0;
-        const double s = ((out.X - in.X) * in.cX + (out.Y - in.Y) * in.cY + (out.Z - in.Z) * in.cZ) /
-                         (in.cX * in.cX + in.cY * in.cY + in.cZ * in.cZ);
-        const double gX = 2.0 * out.X, gY = 2.0 * out.Y, gZ = 2.0 * k1 * out.Z - 2.0 / c;
+        const double c = p[RT_S_C], k1 = p[RT_S_K] + 1.0, ic = drcp(c);
+        const double s = ((out.X - in.X) * in.cX + (out.Y - in.Y) * in.cY + (out.Z - in.Z) * in.cZ) *
+                         drcp(in.cX * in.cX + in.cY * in.cY + in.cZ * in.cZ);
+        const double gX = 2.0 * out.X, gY = 2.0 * out.Y, gZ = 2.0 * k1 * out.Z - 2.0 * ic;
         const double gc = gX * in.cX + gY * in.cY + gZ * in.cZ;
-        const double f = (a.X * in.cX + a.Y * in.cY + a.Z * in.cZ) / gc;
+        const double f = (a.X * in.cX + a.Y * in.cY + a.Z * in.cZ) * drcp(gc);
         ain.X = a.X - f * gX; ain.Y = a.Y - f * gY; ain.Z = a.Z - f * gZ;
         ain.cX += s * (a.X - f * gX); ain.cY += s * (a.Y - f * gY); ain.cZ += s * (a.Z - f * gZ);
         gp[PG_K] += -f * out.Z * out.Z;
-        gp[PG_C] += -f * 2.0 * out.Z / (c * c);
+        gp[PG_C] += -f * 2.0 * out.Z * ic * ic;
     } else if (kind == AK_CONIC_NEWTON || kind == AK_TORIC_NEWTON) {
         double as = 0.0, as0 = 0.0;
         const int n = q.n;
-        point_adj(in, q.s0, q.s[n - 1], a.X, a.Y, a.Z, ain, as, as0);
-        for (int k = n - 2; k >= 0; --k) {
-            // s^(k+1) = s^(k) - F/G at s^(k)
-            double F, G;
-            newton_FG(p, kind, in, q.s0, q.s[k], F, G);
-            const double aF = -as / G, aG = as * F / (G * G);
-            double pX, pY, pZ;
-            newton_FG_adj(p, kind, in, q.s0, q.s[k], aF, aG, pX, pY, pZ, ain, gp);
-            point_adj(in, q.s0, q.s[k], pX, pY, pZ, ain, as, as0);   // as keeps the direct term
+RT_UNROLL
+        for (int k = kMaxNewtonAdj - 1; k >= 0; --k) {
+            if (k == n - 1) {
+                // the returned point, at the last iterate evaluated
+                point_adj(in, q.s0, q.s[k], a.X, a.Y, a.Z, ain, as, as0);
+            } else if (k < n - 1) {
+                // s^(k+1) = s^(k) - F/G at s^(k)
+                double F, G;
+                newton_FG(p, kind, in, q.s0, q.s[k], F, G);
+                const double iG = drcp(G);
+                const double aF = -as * iG, aG = as * F * iG * iG;
+                double pX, pY, pZ;
+                newton_FG_adj(p, kind, in, q.s0, q.s[k], aF, aG, pX, pY, pZ, ain, gp);
+                point_adj(in, q.s0, q.s[k], pX, pY, pZ, ain, as, as0);   // as keeps the direct term
+            }
         }
         // s0 = -Z / cZ
-        ain.Z += -as0 / in.cZ;
-        ain.cZ += as0 * in.Z / (in.cZ * in.cZ);
+        const double icZ = drcp(in.cZ);
+        ain.Z += -as0 * icZ;
+        ain.cZ += as0 * in.Z * icZ * icZ;
     } else {
         ain.X = a.X; ain.Y = a.Y; ain.Z = a.Z;
     }
@@ -443,7 +486,7 @@ RT_HD bool fwd_action(const double* p, const St& r, double mu, double L, double&
     if (type == RT_MIRROR) {
         if (cls == RT_PLANE) { oZ = -r.cZ; return true; }
         const Normal N = normal_at(p, r.X, r.Y, r.Z);
-        const double a2 = 2.0 * (r.cX * N.x + r.cY * N.y + r.cZ * N.z) / (N.x * N.x + N.y * N.y + N.z * N.z);
+        const double a2 = 2.0 * (r.cX * N.x + r.cY * N.y + r.cZ * N.z) * drcp(N.x * N.x + N.y * N.y + N.z * N.z);
         oX = r.cX - a2 * N.x; oY = r.cY - a2 * N.y; oZ = r.cZ - a2 * N.z;
         return true;
     }
@@ -451,15 +494,15 @@ RT_HD bool fwd_action(const double* p, const St& r, double mu, double L, double&
         if (cls == RT_PLANE) {
             const double a = mu * r.cZ, b = mu * mu - 1.0;
             if (b > a * a) return false;
-            const double G = -a + sqrt(a * a - b);
+            const double G = -a + dsqrt(a * a - b);
             oX = mu * r.cX; oY = mu * r.cY; oZ = mu * r.cZ + G;
             return true;
         }
         const Normal N = normal_at(p, r.X, r.Y, r.Z);
-        const double M = N.x * N.x + N.y * N.y + N.z * N.z;
-        const double a = mu * (r.cX * N.x + r.cY * N.y + r.cZ * N.z) / M, b = (mu * mu - 1.0) / M;
+        const double iM = drcp(N.x * N.x + N.y * N.y + N.z * N.z);
+        const double a = mu * (r.cX * N.x + r.cY * N.y + r.cZ * N.z) * iM, b = (mu * mu - 1.0) * iM;
         if (b > a * a) return false;
-        const double G = -a + sqrt(a * a - b);
+        const double G = -a + dsqrt(a * a - b);
         oX = mu * r.cX + G * N.x; oY = mu * r.cY + G * N.y; oZ = mu * r.cZ + G * N.z;
         return true;
     }
@@ -468,12 +511,12 @@ RT_HD bool fwd_action(const double* p, const St& r, double mu, double L, double&
         if ((int)p[RT_S_GRTYPE] == RT_REFLECTIVE) {
             const double a = r.cZ, b = L * L - 2.0 * L * r.cX;
             if (b > a * a) return false;
-            const double G = -a - sqrt(a * a - b);
+            const double G = -a - dsqrt(a * a - b);
             oX = r.cX - L; oZ = r.cZ + G;
         } else {
             const double a = mu * r.cZ, b = mu * mu - 1.0 + L * L - 2.0 * mu * L * r.cX;
             if (b > a * a) return false;
-            const double G = -a + sqrt(a * a - b);
+            const double G = -a + dsqrt(a * a - b);
             oX = mu * r.cX - L; oY = mu * r.cY; oZ = mu * r.cZ + G;
         }
         return true;
@@ -493,27 +536,30 @@ RT_HD bool adj_action(const double* p, const St& r, double mu, double L, double 
         const Normal N = normal_at(p, r.X, r.Y, r.Z);
         const double D = r.cX * N.x + r.cY * N.y + r.cZ * N.z, M = N.x * N.x + N.y * N.y + N.z * N.z;
         const double aoN = aX * N.x + aY * N.y + aZ * N.z;
-        const double aD = -2.0 / M * aoN, aM = 2.0 * D / (M * M) * aoN;
-        Normal aN = {-2.0 * D / M * aX + aD * r.cX + 2.0 * aM * N.x, -2.0 * D / M * aY + aD * r.cY + 2.0 * aM * N.y,
-                     -2.0 * D / M * aZ + aD * r.cZ + 2.0 * aM * N.z};
+        const double iM = drcp(M), q2 = -2.0 * D * iM;
+        const double aD = -2.0 * iM * aoN, aM = 2.0 * D * iM * iM * aoN;
+        Normal aN = {q2 * aX + aD * r.cX + 2.0 * aM * N.x, q2 * aY + aD * r.cY + 2.0 * aM * N.y,
+                     q2 * aZ + aD * r.cZ + 2.0 * aM * N.z};
         a.cX = aX + aD * N.x; a.cY = aY + aD * N.y; a.cZ = aZ + aD * N.z;
         normal_adj(p, r.X, r.Y, r.Z, aN, a, gp);
         return true;
     }
     if (type == RT_LENS) {
         if (cls == RT_PLANE) {
-            const double av = mu * r.cZ, b = mu * mu - 1.0, sq = sqrt(av * av - b);
-            const double aa = aZ * (-1.0 + av / sq);
+            const double av = mu * r.cZ, b = mu * mu - 1.0, sq = dsqrt(av * av - b);
+            const double aa = aZ * (-1.0 + av * drcp(sq));
             a.cX = mu * aX; a.cY = mu * aY; a.cZ = mu * aZ + aa * mu;
             return true;
         }
         const Normal N = normal_at(p, r.X, r.Y, r.Z);
         const double M = N.x * N.x + N.y * N.y + N.z * N.z;
         const double D = r.cX * N.x + r.cY * N.y + r.cZ * N.z;
-        const double av = mu * D / M, b = (mu * mu - 1.0) / M, sq = sqrt(av * av - b), G = -av + sq;
+        const double iM = drcp(M);
+        const double av = mu * D * iM, b = (mu * mu - 1.0) * iM, sq = dsqrt(av * av - b), G = -av + sq;
+        const double isq = drcp(sq);
         const double aG = aX * N.x + aY * N.y + aZ * N.z;
-        const double aa = aG * (-1.0 + av / sq), ab = -aG / (2.0 * sq);
-        const double aD = aa * mu / M, aM = -aa * av / M - ab * b / M;
+        const double aa = aG * (-1.0 + av * isq), ab = -0.5 * aG * isq;
+        const double aD = aa * mu * iM, aM = (-aa * av - ab * b) * iM;
         Normal aN = {G * aX + aD * r.cX + 2.0 * aM * N.x, G * aY + aD * r.cY + 2.0 * aM * N.y,
                      G * aZ + aD * r.cZ + 2.0 * aM * N.z};
         a.cX = mu * aX + aD * N.x; a.cY = mu * aY + aD * N.y; a.cZ = mu * aZ + aD * N.z;
@@ -523,15 +569,16 @@ RT_HD bool adj_action(const double* p, const St& r, double mu, double L, double 
     if (type == RT_GRATING) {
         if (cls != RT_PLANE) return false;
         if ((int)p[RT_S_GRTYPE] == RT_REFLECTIVE) {
-            const double av = r.cZ, b = L * L - 2.0 * L * r.cX, sq = sqrt(av * av - b);
-            const double aG = aZ, aa = aG * (-1.0 - av / sq), ab = aG / (2.0 * sq);
+            const double av = r.cZ, b = L * L - 2.0 * L * r.cX, isq = drcp(dsqrt(av * av - b));
+            const double aG = aZ, aa = aG * (-1.0 - av * isq), ab = 0.5 * aG * isq;
             a.cX = aX + ab * (-2.0 * L);
             a.cY = aY;
             a.cZ = aZ + aa;
             aL = -aX + ab * (2.0 * L - 2.0 * r.cX);
         } else {
-            const double av = mu * r.cZ, b = mu * mu - 1.0 + L * L - 2.0 * mu * L * r.cX, sq = sqrt(av * av - b);
-            const double aG = aZ, aa = aG * (-1.0 + av / sq), ab = -aG / (2.0 * sq);
+            const double av = mu * r.cZ, b = mu * mu - 1.0 + L * L - 2.0 * mu * L * r.cX;
+            const double isq = drcp(dsqrt(av * av - b));
+            const double aG = aZ, aa = aG * (-1.0 + av * isq), ab = -0.5 * aG * isq;
             a.cX = mu * aX + ab * (-2.0 * mu * L);
             a.cY = mu * aY;
             a.cZ = mu * aZ + aa * mu;
@@ -571,7 +618,7 @@ RT_HD bool surface_adjoint(const double* p, const St& entry, int n_fwd, double m
     St aI = {a.X, a.Y, a.Z, 0.0, 0.0, 0.0};
     double aL = 0.0;
     if (!adj_action(p, sI, mu, L, a.cX, a.cY, a.cZ, aI, aL, gp)) return false;
-    if (type == RT_GRATING && L != 0.0) gp[PG_D] += aL * (-L / p[RT_S_GRCONST]);
+    if (type == RT_GRATING && L != 0.0) gp[PG_D] += aL * (-L * drcp(p[RT_S_GRCONST]));
     // intersection, first normalisation, transform
     adj_intersect(p, sN, sI, q, aI, gp);
     adj_normalize(sT.cX, sT.cY, sT.cZ, aI.cX, aI.cY, aI.cZ);

@@ -17,7 +17,6 @@
 #include <string.h>
 
 #include <algorithm>
-#include <numeric>
 #include <vector>
 
 #include "rt_adjoint.h"
@@ -39,9 +38,7 @@ struct AdjArgs {
     int n_surf, pool_len, n_par, pad;
     uint64_t seed, n, n_groups;
     const double* seeds;                 // n_groups x 3: ax, ay, b
-    const int* pstart;                   // n_surf + 1: parameters of surface s are [pstart[s], pstart[s+1])
-    const int* porder;                   // their index in the caller's list
-    const int* pslot;                    // their rt::adj::ParamSlot
+    const int* prow;                     // n_surf x PG_N: the caller's index of (surface, slot), or -1
     double* tape;                        // (n_surf * kTape) x lanes
     double* gacc;                        // n_par x lanes
     unsigned long long* stats;
@@ -118,8 +115,14 @@ __global__ __launch_bounds__(kAdjBlock) void rt_adjoint_kernel(AdjArgs a) {
             for (int k = 0; k < rt::adj::PG_N; ++k) gp[k] = 0.0;
             ok = rt::adj::surface_adjoint(p, entry, nit, mu, L, ad, gp);
             if (!ok) break;
-            for (int k = a.pstart[s]; k < a.pstart[s + 1]; ++k)
-                a.gacc[(uint64_t)a.porder[k] * G + lane] += gp[a.pslot[k]];
+            // the slot loop is unrolled, so gp stays in registers; the row table is uniform
+            const __attribute__((address_space(4))) int* rows =
+                (const __attribute__((address_space(4))) int*)a.prow + s * rt::adj::PG_N;
+#pragma unroll
+            for (int sl = 0; sl < rt::adj::PG_N; ++sl) {
+                const int j = rows[sl];
+                if (j >= 0) a.gacc[(uint64_t)j * G + lane] += gp[sl];
+            }
         }
         if (!ok) ++skipped;
     }
@@ -196,9 +199,8 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     }
     if (n_params < 0 || n_params > 4096 || (n_params > 0 && (!param_surface || !param_field || !grad)))
         return rt_set_error(RT_ERR_BAD_VALUE, "bad parameter list");
-    // parameters grouped by surface: pstart[n_surf+1], porder[n], pslot[n]
-    std::vector<int> idx(n_params);
-    std::iota(idx.begin(), idx.end(), 0);
+    // row table: the caller's index of each (surface, slot), -1 where not asked for
+    std::vector<int> host((size_t)n_surf * rt::adj::PG_N, -1);
     for (int j = 0; j < n_params; ++j) {
         if (param_surface[j] < 0 || param_surface[j] >= n_surf)
             return rt_set_error(RT_ERR_BAD_VALUE, "parameter %d: surface %d out of range", j, param_surface[j]);
@@ -208,18 +210,10 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
         if (slot >= rt::adj::PG_A && slot < rt::adj::PG_CR &&
             slot - rt::adj::PG_A >= (int)host_table[(size_t)param_surface[j] * RT_SURF_STRIDE + RT_S_NPOLY])
             return rt_set_error(RT_ERR_BAD_VALUE, "parameter %d: aspheric term beyond the surface's N", j);
+        int& row = host[(size_t)param_surface[j] * rt::adj::PG_N + slot];
+        if (row >= 0) return rt_set_error(RT_ERR_BAD_VALUE, "parameter %d repeats parameter %d", j, row);
+        row = j;
     }
-    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return param_surface[x] < param_surface[y]; });
-    std::vector<int> host(n_surf + 1 + 2 * (size_t)n_params, 0);
-    int* pstart = host.data();
-    int* porder = pstart + n_surf + 1;
-    int* pslot = porder + n_params;
-    for (int k = 0; k < n_params; ++k) {
-        porder[k] = idx[k];
-        pslot[k] = rt::adj::slot_of_field(param_field[idx[k]]);
-        pstart[param_surface[idx[k]] + 1]++;
-    }
-    for (int s = 0; s < n_surf; ++s) pstart[s + 1] += pstart[s];
     if (n_rays == 0) {
         if (n_params) HIP_TRY(hipMemsetAsync(grad, 0, n_params * sizeof(double), (hipStream_t)stream), RT_ERR_FAILED);
         return RT_OK;
@@ -278,9 +272,7 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     hipStream_t st = (hipStream_t)stream;
     HIP_TRY(hipMemcpyAsync(B.params, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice, st),
             RT_ERR_FAILED);
-    a.pstart = B.params;
-    a.porder = B.params + n_surf + 1;
-    a.pslot = a.porder + n_params;
+    a.prow = B.params;
     a.tape = B.tape;
     a.gacc = B.gacc;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kAdjBlock), lds, st, a);
