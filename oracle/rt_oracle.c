@@ -907,3 +907,35 @@ int oracle_grating_energy_distr(size_t n, const double *lambda, long order,
     }
     return RT_OK;
 }
+
+/* ------------------------------------------------- per-range beam generators */
+
+int oracle_beam_positions(const double *beam, uint64_t seed, size_t begin, size_t n,
+                          double *X, double *Y, double *Z)
+{
+    for (size_t i = 0; i < n; ++i) {
+        ray_t r;
+        gen_position(beam, seed, begin + i, &r);
+        X[i] = r.X; Y[i] = r.Y; Z[i] = r.Z;
+    }
+    return RT_OK;
+}
+
+int oracle_beam_directions(const double *beam, uint64_t seed, size_t begin, size_t n,
+                           const double *X, const double *Y, double *cX, double *cY, double *cZ)
+{
+    for (size_t i = 0; i < n; ++i) {
+        ray_t r;
+        r.X = X ? X[i] : 0.0;
+        r.Y = Y ? Y[i] : 0.0;
+        gen_direction(beam, seed, begin + i, &r);
+        cX[i] = r.cX; cY[i] = r.cY; cZ[i] = r.cZ;
+    }
+    return RT_OK;
+}
+
+int oracle_beam_wavelengths(const double *beam, uint64_t seed, size_t begin, size_t n, double *lambda)
+{
+    for (size_t i = 0; i < n; ++i) lambda[i] = gen_lambda(beam, seed, begin + i);
+    return RT_OK;
+}

@@ -46,6 +46,14 @@ int oracle_trace(const double *table, int n_surf, const double *pool,
                  uint32_t *hist, double *flux, uint64_t *stats,
                  double *dump, signed char *dump_flag, int nthreads);
 
+/* per-range beam generators (the engine's beam_random, beam_*_cosins and
+   *_range_beam split by component) */
+int oracle_beam_positions(const double *beam, uint64_t seed, size_t begin, size_t n,
+                          double *X, double *Y, double *Z);
+int oracle_beam_directions(const double *beam, uint64_t seed, size_t begin, size_t n,
+                           const double *X, const double *Y, double *cX, double *cY, double *cZ);
+int oracle_beam_wavelengths(const double *beam, uint64_t seed, size_t begin, size_t n, double *lambda);
+
 /* ---- per-stage restatements of the rt_engine_func.h entry points ---- */
 int oracle_beam_transform(size_t n, const double *angles, const double *dist,
                           double *X, double *Y, double *Z,
