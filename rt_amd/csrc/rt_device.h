@@ -290,8 +290,11 @@ __device__ __forceinline__ void gen_direction(cptr B, uint64_t seed, uint64_t r,
 }
 
 // wavelengths: beam.cpp:425-484; uniform_range_beam_kernel (beam.cu:701-710,
-// global index), random_range_beam_kernel (beam.cu:801-812)
+// global index), random_range_beam_kernel (beam.cu:801-812).  Products and
+// sums rounded separately (no FMA contraction), so every wavelength is the
+// correctly rounded value of the written expression, bit for bit.
 __device__ __forceinline__ double gen_lambda(cptr B, uint64_t seed, uint64_t r) {
+#pragma clang fp contract(off)
     int distr = (int)B[RT_B_RANGE_DISTR];
     double r0 = B[RT_B_RANGE + 0], r1 = B[RT_B_RANGE + 1];
     uint64_t nper = (uint64_t)B[RT_B_NPER];
