@@ -30,12 +30,15 @@ TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "fused_trace_echelle_hbm_traffi
 HOST_CPU_SHARE = 16         # host threads one GPU's job may use
 
 
-def measured_traffic():
-    """HBM bytes per launch of the fused trace on this workload, from the
-    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+def measured_traffic(block, waves):
+    """HBM bytes per launch of the fused trace on this workload under the launch
+    (block, waves per SIMD), from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (copy-calibrated; lines 'traffic_bytes_per_launch[B,W] = T'), or
+    None when that launch was not profiled."""
+    key = f"traffic_bytes_per_launch[{block},{waves}]"
     try:
         for line in open(TRAFFIC_SUMMARY):
-            if line.startswith("traffic_bytes_per_launch"):
+            if line.startswith(key):
                 return float(line.split("=")[1])
     except OSError:
         pass
@@ -51,6 +54,8 @@ def main():
     ap.add_argument("--nlambda", type=int, default=2048)
     ap.add_argument("--cpu-sample", type=int, default=0, help="rays traced by the CPU baseline (0: the whole workload)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--launch", default="", metavar="BLOCK,WAVES,GRID_MULT",
+                    help="launch configuration of the generated kernel instead of the autotuner's choice")
     ap.add_argument("--batch", type=int, default=0, metavar="N_INST",
                     help="measure the batched workload instead: N_INST Monte-Carlo tolerance instances of "
                          "the echelle, ray counts log-uniform in [1e4, 1e6], one persistent launch per step")
@@ -96,9 +101,13 @@ def main():
     flags = S.RT_TRACE_GENERATE | S.RT_TRACE_STORE | S.RT_TRACE_HIST | S.RT_TRACE_FLUX
     stream = torch.cuda.current_stream()
 
-    # launch configuration of the generated kernel chosen by the autotuner on a
-    # 2e6-ray sample of the same beam (setup, outside the timed region)
-    tuned_ms = ds.autotune(n_sample=min(n, 2_000_000), flags=flags, seed=999)
+    # launch configuration of the generated kernel chosen by the autotuner on the
+    # same workload with another seed (setup, outside the timed region)
+    if args.launch:
+        ds.set_launch(*(int(v) for v in args.launch.split(",")))
+        tuned_ms = None
+    else:
+        tuned_ms = ds.autotune(n_sample=n, flags=flags, seed=999)
     block, waves, grid_mult = ds.get_launch()
 
     def step(k):
@@ -148,7 +157,7 @@ def main():
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
-                     "traffic": measured_traffic()},
+                     "traffic": measured_traffic(block, waves)},
         "detail": {"kernel_ms": kernel_ms, "launch": {"block": block, "waves_per_simd": waves,
                                                       "grid_mult": grid_mult, "autotune_sample_ms": tuned_ms},
                    "ray_surface_per_step": ray_surf // args.steps,

@@ -271,9 +271,11 @@ int rt_hip_scheme_set_launch(rt_hip_scheme *scheme, int block, int waves, int gr
 int rt_hip_scheme_get_launch(const rt_hip_scheme *scheme, int *block, int *waves, int *grid_mult);
 
 /* Autotuner: traces n_sample rays of the beam (generated from seed, with
-   `flags`) under each candidate launch (block 128/256, waves 0/5/6, grid
-   multiple 1/2), keeps the fastest in the scheme and returns its time in
-   *best_ms.  Allocates its sample buffers; call it once after
+   `flags`) under each candidate launch (block 128/256, waves 0/5/6/7/8,
+   grid multiple 1/2/4/8/16; register allocations that spill to scratch are
+   skipped when a spill-free one exists), re-times the four fastest in
+   interleaved rounds, keeps the one with the lowest median in the scheme and
+   returns that median in *best_ms.  Allocates its sample buffers; call it once after
    rt_hip_scheme_create, outside any timed or captured region. */
 int rt_hip_autotune(rt_hip_scheme *scheme, const double *beam_desc, uint64_t seed, size_t n_sample,
                     int flags, double *best_ms, void *stream);

@@ -30,6 +30,7 @@
 #include <string.h>
 
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "rt_device.h"
@@ -538,29 +539,76 @@ extern "C" int rt_hip_autotune(rt_hip_scheme* sch, const double* beam_desc, uint
         return set_error(RT_ERR_FAILED, "hipEventCreate");
     }
     const int blocks[] = {128, 256};
-    const int wavess[] = {0, 5, 6};
-    const int mults[] = {1, 2};
-    double best = 1e300;
-    int bb = sch->block, bw = sch->waves, bm = sch->grid_mult;
+    const int wavess[] = {0, 5, 6, 7, 8};
+    const int mults[] = {1, 2, 4, 8, 16};
+    struct Cand { int b, w, m; float ms; };
+    std::vector<Cand> cands;
+    const int bb0 = sch->block, bw0 = sch->waves, bm0 = sch->grid_mult;
+    auto timed = [&](const Cand& c, float& ms) -> int {
+        sch->block = c.b; sch->waves = c.w; sch->grid_mult = c.m;
+        (void)hipEventRecord(e0, st);
+        int r = run();
+        (void)hipEventRecord(e1, st);
+        if (r != RT_OK) return r;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            return set_error(RT_ERR_FAILED, "autotune timing");
+        return RT_OK;
+    };
+    // screening: every candidate, best of 2 after a compiling/warming launch.
+    // Register allocations that spill to scratch memory are skipped when a
+    // spill-free one exists: their time is within run-to-run noise of the
+    // spill-free launches, and the spills add memory traffic.
     rc = RT_OK;
+    const int beam_mode = (int)beam_desc[RT_B_TYPE];
+    bool any_clean = false;
+    std::vector<Cand> spilling;
     for (int b : blocks)
         for (int w : wavess)
             for (int m : mults) {
+                if (rc != RT_OK) break;
+                Cand c{b, w, m, 1e30f};
                 sch->block = b; sch->waves = w; sch->grid_mult = m;
                 if ((rc = run()) != RT_OK) break;                      // first launch compiles
                 if (hipStreamSynchronize(st) != hipSuccess) { rc = set_error(RT_ERR_FAILED, "autotune"); break; }
-                float ms = 1e30f;
-                for (int k = 0; k < 3 && rc == RT_OK; ++k) {
-                    (void)hipEventRecord(e0, st);
-                    rc = run();
-                    (void)hipEventRecord(e1, st);
-                    if (hipEventSynchronize(e1) != hipSuccess) rc = set_error(RT_ERR_FAILED, "autotune");
+                int local = 0;
+                hipFunction_t fn =
+                    rt_jit_enabled() ? rt_jit_kernel(sch->signature, beam_mode, false, b, w, sch->device) : nullptr;
+                if (fn && hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fn) != hipSuccess)
+                    local = 0;
+                for (int k = 0; k < 2 && rc == RT_OK; ++k) {
                     float t = 0.0f;
-                    if (rc == RT_OK && hipEventElapsedTime(&t, e0, e1) == hipSuccess && t < ms) ms = t;
+                    rc = timed(c, t);
+                    if (rc == RT_OK && t < c.ms) c.ms = t;
                 }
-                if (rc != RT_OK) break;
-                if (ms < best) { best = ms; bb = b; bw = w; bm = m; }
+                if (local > 0) {
+                    spilling.push_back(c);
+                } else {
+                    cands.push_back(c);
+                    any_clean = true;
+                }
             }
+    if (!any_clean) cands = spilling;
+    // the clock of a sustained FP64 load drifts between launches: the four best
+    // are re-timed in interleaved rounds and ranked by their median
+    int bb = bb0, bw = bw0, bm = bm0;
+    double best = 1e300;
+    if (rc == RT_OK && !cands.empty()) {
+        std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+        const size_t top = cands.size() < 4 ? cands.size() : 4;
+        const int rounds = 7;
+        std::vector<std::vector<float>> t(top);
+        for (int r = 0; r < rounds && rc == RT_OK; ++r)
+            for (size_t i = 0; i < top && rc == RT_OK; ++i) {
+                float ms = 0.0f;
+                rc = timed(cands[i], ms);
+                t[i].push_back(ms);
+            }
+        for (size_t i = 0; i < top && rc == RT_OK; ++i) {
+            std::sort(t[i].begin(), t[i].end());
+            const double med = t[i][t[i].size() / 2];
+            if (med < best) { best = med; bb = cands[i].b; bw = cands[i].w; bm = cands[i].m; }
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     sch->block = bb; sch->waves = bw; sch->grid_mult = bm;
