@@ -11,13 +11,15 @@
 //
 // usage: ref_pipeline <scheme.txt> <out prefix>
 //   writes <prefix>.dat (Beam::Save), <prefix>.ccd (FP64 image, if the last
-//   surface is a detector with a QE table), <prefix>.qe (efficiency of every
+//   surface is a detector with a CCD frame), <prefix>.qe (efficiency of every
 //   grating and of the detector on the beam's wavelength grid), and prints
-//   "info", "ngood" lines.
+//   "info", "trace_seconds" (wall time of Beam::Create and the surface loop)
+//   and "ngood" lines.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <chrono>
 #include <fstream>
 #include <stdexcept>
 #include <iostream>
@@ -186,6 +188,7 @@ int main(int argc, char** argv) {
         vector<Surface*> scheme;
         for (const SurfDesc& d : descs) scheme.push_back(make_surface(d));
 
+        const auto t0 = std::chrono::steady_clock::now();
         beam->Create(bparams, bcenter, brange, composite, nlambda);
         // Scheme::run per-surface loop (scheme.cpp:983-1046), one wavelength range
         for (size_t i = 0; i < scheme.size(); ++i) {
@@ -199,6 +202,8 @@ int main(int argc, char** argv) {
                 beam->Normalize();
             }
         }
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        printf("trace_seconds %.6f\n", secs);
         beam->Save(prefix + ".dat");
         printf("ngood %zu of %zu\n", beam->GetNumberOfGoodRays(), beam->GetTotalNumberOfRays());
 
