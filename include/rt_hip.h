@@ -342,15 +342,33 @@ int rt_hip_adjoint(const rt_hip_scheme *scheme, const double *beam_desc, uint64_
                    const int *param_field, int n_params, double *grad, unsigned long long *stats,
                    void *stream);
 
+/* Spots of a traced beam per ray group (ray r belongs to group r mod
+   n_groups): spots (device, n_groups x RT_SPOT_STRIDE) receives, from the
+   rays with flag 1, n, centroid x, y, variance x, y and RMS radius
+   sqrt(var x + var y), summed in a fixed order (bit-reproducible).  X, Y,
+   flag: device arrays of n rays.  No counterpart in the reference; it feeds
+   the spot loss of the adjoint. */
+int rt_hip_group_spots(const double *X, const double *Y, const signed char *flag, size_t n, size_t n_groups,
+                       double *spots, void *stream);
+
+/* The spot loss over groups with live rays, sum of rms + weight x squared
+   centroid offset from target (device, n_groups x 2, or NULL for 0), into
+   *loss (device), and the adjoint seeds of rt_hip_adjoint (device, n_groups
+   x 3: ax, ay, b with dloss/dX_r = ax + b X_r, dloss/dY_r = ay + b Y_r),
+   from rt_hip_group_spots' output; fixed order. */
+int rt_hip_spot_loss(const double *spots, size_t n_groups, const double *target, double weight, double *seeds,
+                     double *loss, void *stream);
+
 /* "generated" or "generic": the kernel the calling thread's last trace used. */
 int rt_hip_last_kernel(char *buf, size_t len);
 
 /* Generate and compile (without loading, no device needed) the kernel of a
    packed scheme for beam mode (-1: rays given, else RT_BEAM_*, or'ed with
-   RT_JIT_BATCH for the batched kernel); `text` receives the generated source
+   RT_JIT_BATCH for the batched kernel, or with RT_JIT_ADJOINT for the adjoint
+   kernel with every slot of every surface asked for); `text` receives the generated source
    and the compiler log, *code_size the code object size.  RT_OK when it
    compiles. */
-enum { RT_JIT_BATCH = 0x100 };
+enum { RT_JIT_BATCH = 0x100, RT_JIT_ADJOINT = 0x200 };
 int rt_hip_jit_check(const double *surf_table, int n_surf, const double *spline_pool,
                      size_t pool_len, int beam_mode, char *text, size_t len, size_t *code_size);
 

@@ -27,10 +27,12 @@
 #if defined(__HIPCC__) || defined(__HIPCC_RTC__)
 #include "rt_device.h"     // the refined reciprocal / reciprocal square root of the device build
 #define RT_HD __host__ __device__ inline
+#define RT_HDM __host__ __device__
 #define RT_UNROLL _Pragma("unroll")
 #else
 #include <math.h>
 #define RT_HD static inline
+#define RT_HDM
 #define RT_UNROLL
 #endif
 
@@ -93,13 +95,30 @@ struct St {
 
 enum { AK_PLANE = 0, AK_QUAD = 1, AK_CONIC_NEWTON = 2, AK_TORIC_NEWTON = 3, AK_NONE = 4 };
 
-RT_HD int ikind(const double* p) {
-    const int c = (int)p[RT_S_CLASS], n = (int)p[RT_S_NPOLY];
-    if (c == RT_PLANE || (c == RT_CONIC && n == 0 && p[RT_S_C] == 0.0)) return AK_PLANE;
-    if (c == RT_CONIC) return n == 0 ? AK_QUAD : AK_CONIC_NEWTON;
-    if (c == RT_TORIC) return AK_TORIC_NEWTON;
-    return AK_NONE;
-}
+// The structure of a surface (class, type, number of aspheric terms, grating
+// type, intersection kind, which rotations are applied) comes from a record
+// view: DynRec reads it from the record at run time (host, generic kernel);
+// the generated kernels pass rt::StatSurf<...> (rt_device.h), whose structure
+// is compile-time, so every branch below on it folds away and the parameter
+// slots are fixed indices.  Values are always read through .p.
+struct DynRec {
+    const double* p;
+    RT_HDM int cls() const { return (int)p[RT_S_CLASS]; }
+    RT_HDM int type() const { return (int)p[RT_S_TYPE]; }
+    RT_HDM int npoly() const { return (int)p[RT_S_NPOLY]; }
+    RT_HDM int grtype() const { return (int)p[RT_S_GRTYPE]; }
+    RT_HDM bool rot(int j) const { return p[RT_S_ANG + j] != 0.0; }
+    RT_HDM bool trans() const {
+        return (p[RT_S_DIST + 2] != 0.0) || (p[RT_S_DIST + 0] != 0.0) || (p[RT_S_DIST + 1] != 0.0);
+    }
+    RT_HDM int ikind() const {
+        const int c = cls(), n = npoly();
+        if (c == RT_PLANE || (c == RT_CONIC && n == 0 && p[RT_S_C] == 0.0)) return AK_PLANE;
+        if (c == RT_CONIC) return n == 0 ? AK_QUAD : AK_CONIC_NEWTON;
+        if (c == RT_TORIC) return AK_TORIC_NEWTON;
+        return AK_NONE;
+    }
+};
 
 // ------------------------------------------------------------ polynomials
 // P(x) = sum a_i x^i, Q = P'(x) = sum i a_i x^(i-1), R = Q'(x)
@@ -107,8 +126,10 @@ struct Poly {
     double P, Q, R;
 };
 
-RT_HD Poly poly(const double* p, double x) {
-    const int N = (int)p[RT_S_NPOLY];
+template <class Rec>
+RT_HD Poly poly(const Rec& d, double x) {
+    const auto p = d.p;
+    const int N = d.npoly();
     Poly o = {0.0, 0.0, 0.0};
     double xi = 1.0;                   // x^(i-1)
     double xm = 0.0;                   // x^(i-2)
@@ -124,8 +145,9 @@ RT_HD Poly poly(const double* p, double x) {
 }
 
 // d/d a_i of P and of Q at x, accumulated with weights wP, wQ
-RT_HD void poly_param_adj(const double* p, double x, double wP, double wQ, double* gp) {
-    const int N = (int)p[RT_S_NPOLY];
+template <class Rec>
+RT_HD void poly_param_adj(const Rec& d, double x, double wP, double wQ, double* gp) {
+    const int N = d.npoly();
     double xi = 1.0;
     for (int i = 1; i <= N; ++i) {
         gp[PG_A + i - 1] += wP * xi * x + wQ * i * xi;
@@ -142,7 +164,9 @@ struct Conic {
     Poly pl;
 };
 
-RT_HD Conic conic_at(const double* p, double x) {
+template <class Rec>
+RT_HD Conic conic_at(const Rec& d, double x) {
+    const auto p = d.p;
     const double c = p[RT_S_C], k1 = p[RT_S_K] + 1.0;
     Conic o;
     const double K = k1 * c * c;
@@ -150,7 +174,7 @@ RT_HD Conic conic_at(const double* p, double x) {
     const double rq = drsqrt(t);               // 1/q
     o.q = t * rq;
     const double rq3 = rq * rq * rq, iopq = drcp(1.0 + o.q), iopq2 = iopq * iopq;
-    o.pl = poly(p, x);
+    o.pl = poly(d, x);
     o.S = c * x * iopq;
     o.E = c * rq + 2.0 * o.pl.Q;
     o.dSdx = 0.5 * c * rq;
@@ -163,10 +187,11 @@ RT_HD Conic conic_at(const double* p, double x) {
 }
 
 // adjoint of (S + P) and of E at x, with weights wS (on S + P: the full sag) and wE
-RT_HD double conic_adj(const double* p, const Conic& k, double x, double wS, double wE, double* gp) {
+template <class Rec>
+RT_HD double conic_adj(const Rec& d, const Conic& k, double x, double wS, double wE, double* gp) {
     gp[PG_C] += wS * k.dSdc + wE * k.dEdc;
     gp[PG_K] += wS * k.dSdk + wE * k.dEdk;
-    poly_param_adj(p, x, wS, 2.0 * wE, gp);
+    poly_param_adj(d, x, wS, 2.0 * wE, gp);
     return wS * (k.dSdx + k.pl.Q) + wE * k.dEdx;      // adjoint of x
 }
 
@@ -177,14 +202,15 @@ RT_HD void rot2(double s, double c, double& v1, double& v2) {
     v1 = e;
 }
 
-RT_HD void fwd_transform(const double* p, St& r) {
-    const double* d = p + RT_S_DIST;
-    if ((d[2] != 0.0) || (d[0] != 0.0) || (d[1] != 0.0)) {
-        r.X -= d[0]; r.Y -= d[1]; r.Z -= d[2];
+template <class Rec>
+RT_HD void fwd_transform(const Rec& d, St& r) {
+    const auto p = d.p;
+    if (d.trans()) {
+        r.X -= p[RT_S_DIST + 0]; r.Y -= p[RT_S_DIST + 1]; r.Z -= p[RT_S_DIST + 2];
     }
-    if (p[RT_S_ANG + 0] != 0.0) { rot2(p[RT_S_SIN + 0], p[RT_S_COS + 0], r.X, r.Z); rot2(p[RT_S_SIN + 0], p[RT_S_COS + 0], r.cX, r.cZ); }
-    if (p[RT_S_ANG + 1] != 0.0) { rot2(p[RT_S_SIN + 1], p[RT_S_COS + 1], r.Y, r.Z); rot2(p[RT_S_SIN + 1], p[RT_S_COS + 1], r.cY, r.cZ); }
-    if (p[RT_S_ANG + 2] != 0.0) { rot2(p[RT_S_SIN + 2], p[RT_S_COS + 2], r.X, r.Y); rot2(p[RT_S_SIN + 2], p[RT_S_COS + 2], r.cX, r.cY); }
+    if (d.rot(0)) { rot2(p[RT_S_SIN + 0], p[RT_S_COS + 0], r.X, r.Z); rot2(p[RT_S_SIN + 0], p[RT_S_COS + 0], r.cX, r.cZ); }
+    if (d.rot(1)) { rot2(p[RT_S_SIN + 1], p[RT_S_COS + 1], r.Y, r.Z); rot2(p[RT_S_SIN + 1], p[RT_S_COS + 1], r.cY, r.cZ); }
+    if (d.rot(2)) { rot2(p[RT_S_SIN + 2], p[RT_S_COS + 2], r.X, r.Y); rot2(p[RT_S_SIN + 2], p[RT_S_COS + 2], r.cX, r.cY); }
 }
 
 // one rotation backwards: outputs (v1, v2) become inputs, adjoints (w1, w2)
@@ -198,10 +224,12 @@ RT_HD void rot2_adj(double s, double c, double& v1, double& v2, double& w1, doub
 }
 
 // r: state after the transform (becomes the state before it); a: its adjoint
-RT_HD void adj_transform(const double* p, St& r, St& a, double* gp) {
-    const double s2 = p[RT_S_ANG + 2] != 0.0 ? p[RT_S_SIN + 2] : 0.0, c2 = p[RT_S_ANG + 2] != 0.0 ? p[RT_S_COS + 2] : 1.0;
-    const double s1 = p[RT_S_ANG + 1] != 0.0 ? p[RT_S_SIN + 1] : 0.0, c1 = p[RT_S_ANG + 1] != 0.0 ? p[RT_S_COS + 1] : 1.0;
-    const double s0 = p[RT_S_ANG + 0] != 0.0 ? p[RT_S_SIN + 0] : 0.0, c0 = p[RT_S_ANG + 0] != 0.0 ? p[RT_S_COS + 0] : 1.0;
+template <class Rec>
+RT_HD void adj_transform(const Rec& d, St& r, St& a, double* gp) {
+    const auto p = d.p;
+    const double s2 = d.rot(2) ? p[RT_S_SIN + 2] : 0.0, c2 = d.rot(2) ? p[RT_S_COS + 2] : 1.0;
+    const double s1 = d.rot(1) ? p[RT_S_SIN + 1] : 0.0, c1 = d.rot(1) ? p[RT_S_COS + 1] : 1.0;
+    const double s0 = d.rot(0) ? p[RT_S_SIN + 0] : 0.0, c0 = d.rot(0) ? p[RT_S_COS + 0] : 1.0;
     double ga2 = 0.0, ga1 = 0.0, ga0 = 0.0;
     rot2_adj(s2, c2, r.X, r.Y, a.X, a.Y, &ga2);
     rot2_adj(s2, c2, r.cX, r.cY, a.cX, a.cY, &ga2);
@@ -245,16 +273,18 @@ struct NewtonSeq {
 };
 
 // F and dF/ds of the iterative intersections at parameter s along the ray
-RT_HD void newton_FG(const double* p, int kind, const St& in, double s0, double s, double& F, double& G) {
+template <class Rec>
+RT_HD void newton_FG(const Rec& d, int kind, const St& in, double s0, double s, double& F, double& G) {
+    const auto p = d.p;
     const double u = s + s0;
     const double X = in.X + u * in.cX, Y = in.Y + u * in.cY, Z = s * in.cZ;
     if (kind == AK_CONIC_NEWTON) {
-        const Conic k = conic_at(p, X * X + Y * Y);
+        const Conic k = conic_at(d, X * X + Y * Y);
         F = Z - k.S - k.pl.P;
         G = -k.E * X * in.cX - k.E * Y * in.cY + in.cZ;
     } else {
         const double cR = p[RT_S_CR];
-        const Conic k = conic_at(p, Y * Y);
+        const Conic k = conic_at(d, Y * Y);
         const double Fy = k.S + k.pl.P;
         const double dFy = k.E * Y;
         F = Z - Fy - 0.5 * cR * (X * X + Z * Z - Fy * Fy);
@@ -264,8 +294,10 @@ RT_HD void newton_FG(const double* p, int kind, const St& in, double s0, double 
 
 // forward intersection from `in`; n_fwd > 0 replays that many Newton
 // iterations, 0 iterates to the reference's convergence test.  false: lost
-RT_HD bool fwd_intersect(const double* p, const St& in, int n_fwd, St& out, NewtonSeq& q) {
-    const int kind = ikind(p);
+template <class Rec>
+RT_HD bool fwd_intersect(const Rec& d, const St& in, int n_fwd, St& out, NewtonSeq& q) {
+    const auto p = d.p;
+    const int kind = d.ikind();
     out = in;
     q.n = 0;
     q.s0 = 0.0;
@@ -302,7 +334,7 @@ RT_UNROLL
                 s1 = s2;
                 q.s[k] = s1;
                 double F, G;
-                newton_FG(p, kind, in, q.s0, s1, F, G);
+                newton_FG(d, kind, in, q.s0, s1, F, G);
                 if (fabs(G) < 1.0e-7) return false;
                 s2 = s1 - F * drcp(G);
                 it = k + 1;
@@ -334,14 +366,16 @@ RT_HD void point_adj(const St& in, double s0, double s, double aX, double aY, do
 
 // adjoint of F, G (newton_FG) at parameter s: point adjoint out (aX, aY, aZ),
 // direct direction adjoints into ain, parameters into gp
-RT_HD void newton_FG_adj(const double* p, int kind, const St& in, double s0, double s, double aF, double aG,
+template <class Rec>
+RT_HD void newton_FG_adj(const Rec& d, int kind, const St& in, double s0, double s, double aF, double aG,
                          double& aX, double& aY, double& aZ, St& ain, double* gp) {
+    const auto p = d.p;
     const double u = s + s0;
     const double X = in.X + u * in.cX, Y = in.Y + u * in.cY, Z = s * in.cZ;
     aX = aY = aZ = 0.0;
     if (kind == AK_CONIC_NEWTON) {
         const double r2 = X * X + Y * Y;
-        const Conic k = conic_at(p, r2);
+        const Conic k = conic_at(d, r2);
         const double W = X * in.cX + Y * in.cY;
         // G = -E W + cZ
         const double aE = -aG * W, aW = -aG * k.E;
@@ -350,13 +384,13 @@ RT_HD void newton_FG_adj(const double* p, int kind, const St& in, double s0, dou
         aY += aW * in.cY; ain.cY += aW * Y;
         // F = Z - (S + P)
         aZ += aF;
-        const double ar2 = conic_adj(p, k, r2, -aF, aE, gp);
+        const double ar2 = conic_adj(d, k, r2, -aF, aE, gp);
         aX += 2.0 * X * ar2;
         aY += 2.0 * Y * ar2;
     } else {
         const double cR = p[RT_S_CR];
         const double Y2 = Y * Y;
-        const Conic k = conic_at(p, Y2);
+        const Conic k = conic_at(d, Y2);
         const double Fy = k.S + k.pl.P, dFy = k.E * Y;
         // G = -cR X cX + (cR Fy - 1) dFy cY + (1 - cR Z) cZ
         aX += -aG * cR * in.cX; ain.cX += -aG * cR * X;
@@ -372,15 +406,17 @@ RT_HD void newton_FG_adj(const double* p, int kind, const St& in, double s0, dou
         gp[PG_CR] += aF * (-0.5 * (X * X + Z * Z - Fy * Fy));
         // Fy(Y) = S + P at Y^2; dFy(Y) = E Y
         aY += aFy * dFy + adFy * (k.E + 2.0 * Y2 * k.dEdx);
-        const double aY2 = conic_adj(p, k, Y2, aFy, adFy * Y, gp);
+        const double aY2 = conic_adj(d, k, Y2, aFy, adFy * Y, gp);
         (void)aY2;   // the Y^2 chain of Fy and E is already in dFy and dE/dY above
     }
 }
 
 // in: state before the intersection; out: the intersection point; a: adjoint of
 // `out` (becomes the adjoint of `in`)
-RT_HD void adj_intersect(const double* p, const St& in, const St& out, const NewtonSeq& q, St& a, double* gp) {
-    const int kind = ikind(p);
+template <class Rec>
+RT_HD void adj_intersect(const Rec& d, const St& in, const St& out, const NewtonSeq& q, St& a, double* gp) {
+    const auto p = d.p;
+    const int kind = d.ikind();
     St ain = {0, 0, 0, a.cX, a.cY, a.cZ};
     if (kind == AK_PLANE) {
         const double icZ = drcp(in.cZ);
@@ -412,11 +448,11 @@ RT_UNROLL
             } else if (k < n - 1) {
                 // s^(k+1) = s^(k) - F/G at s^(k)
                 double F, G;
-                newton_FG(p, kind, in, q.s0, q.s[k], F, G);
+                newton_FG(d, kind, in, q.s0, q.s[k], F, G);
                 const double iG = drcp(G);
                 const double aF = -as * iG, aG = as * F * iG * iG;
                 double pX, pY, pZ;
-                newton_FG_adj(p, kind, in, q.s0, q.s[k], aF, aG, pX, pY, pZ, ain, gp);
+                newton_FG_adj(d, kind, in, q.s0, q.s[k], aF, aG, pX, pY, pZ, ain, gp);
                 point_adj(in, q.s0, q.s[k], pX, pY, pZ, ain, as, as0);   // as keeps the direct term
             }
         }
@@ -437,55 +473,60 @@ struct Normal {
     double x, y, z;
 };
 
-RT_HD Normal normal_at(const double* p, double X, double Y, double Z) {
-    const int cls = (int)p[RT_S_CLASS];
+template <class Rec>
+RT_HD Normal normal_at(const Rec& d, double X, double Y, double Z) {
+    const auto p = d.p;
+    const int cls = d.cls();
     if (cls == RT_CONIC) {
-        const Conic k = conic_at(p, X * X + Y * Y);
+        const Conic k = conic_at(d, X * X + Y * Y);
         return Normal{-k.E * X, -k.E * Y, 1.0};
     }
     if (cls == RT_TORIC) {
         const double cR = p[RT_S_CR];
-        const Conic k = conic_at(p, Y * Y);
+        const Conic k = conic_at(d, Y * Y);
         const double Fy = k.S + k.pl.P, dFy = k.E * Y;
         return Normal{-cR * X, (cR * Fy - 1.0) * dFy, 1.0 - cR * Z};
     }
     return Normal{0.0, 0.0, 1.0};
 }
 
-RT_HD void normal_adj(const double* p, double X, double Y, double Z, const Normal& aN, St& apt, double* gp) {
-    const int cls = (int)p[RT_S_CLASS];
+template <class Rec>
+RT_HD void normal_adj(const Rec& d, double X, double Y, double Z, const Normal& aN, St& apt, double* gp) {
+    const auto p = d.p;
+    const int cls = d.cls();
     if (cls == RT_CONIC) {
         const double r2 = X * X + Y * Y;
-        const Conic k = conic_at(p, r2);
+        const Conic k = conic_at(d, r2);
         const double aE = -(aN.x * X + aN.y * Y);
         apt.X += -k.E * aN.x;
         apt.Y += -k.E * aN.y;
-        const double ar2 = conic_adj(p, k, r2, 0.0, aE, gp);
+        const double ar2 = conic_adj(d, k, r2, 0.0, aE, gp);
         apt.X += 2.0 * X * ar2;
         apt.Y += 2.0 * Y * ar2;
     } else if (cls == RT_TORIC) {
         const double cR = p[RT_S_CR];
         const double Y2 = Y * Y;
-        const Conic k = conic_at(p, Y2);
+        const Conic k = conic_at(d, Y2);
         const double Fy = k.S + k.pl.P, dFy = k.E * Y;
         apt.X += -cR * aN.x;
         apt.Z += -cR * aN.z;
         gp[PG_CR] += -X * aN.x + Fy * dFy * aN.y - Z * aN.z;
         const double aFy = aN.y * cR * dFy, adFy = aN.y * (cR * Fy - 1.0);
         apt.Y += aFy * dFy + adFy * (k.E + 2.0 * Y2 * k.dEdx);
-        (void)conic_adj(p, k, Y2, aFy, adFy * Y, gp);
+        (void)conic_adj(d, k, Y2, aFy, adFy * Y, gp);
     }
 }
 
 // ------------------------------------------------------------------ actions
 // The action's output direction (before the normalisation that follows);
 // false for total internal reflection or an evanescent order.
-RT_HD bool fwd_action(const double* p, const St& r, double mu, double L, double& oX, double& oY, double& oZ) {
-    const int type = (int)p[RT_S_TYPE], cls = (int)p[RT_S_CLASS];
+template <class Rec>
+RT_HD bool fwd_action(const Rec& d, const St& r, double mu, double L, double& oX, double& oY, double& oZ) {
+    const int type = d.type(), cls = d.cls();
     oX = r.cX; oY = r.cY; oZ = r.cZ;
     if (type == RT_MIRROR) {
         if (cls == RT_PLANE) { oZ = -r.cZ; return true; }
-        const Normal N = normal_at(p, r.X, r.Y, r.Z);
+        const Normal N = normal_at(d, r.X, r.Y, r.Z);
         const double a2 = 2.0 * (r.cX * N.x + r.cY * N.y + r.cZ * N.z) * drcp(N.x * N.x + N.y * N.y + N.z * N.z);
         oX = r.cX - a2 * N.x; oY = r.cY - a2 * N.y; oZ = r.cZ - a2 * N.z;
         return true;
@@ -498,7 +539,7 @@ RT_HD bool fwd_action(const double* p, const St& r, double mu, double L, double&
             oX = mu * r.cX; oY = mu * r.cY; oZ = mu * r.cZ + G;
             return true;
         }
-        const Normal N = normal_at(p, r.X, r.Y, r.Z);
+        const Normal N = normal_at(d, r.X, r.Y, r.Z);
         const double iM = drcp(N.x * N.x + N.y * N.y + N.z * N.z);
         const double a = mu * (r.cX * N.x + r.cY * N.y + r.cZ * N.z) * iM, b = (mu * mu - 1.0) * iM;
         if (b > a * a) return false;
@@ -508,7 +549,7 @@ RT_HD bool fwd_action(const double* p, const St& r, double mu, double L, double&
     }
     if (type == RT_GRATING) {
         if (cls != RT_PLANE) return false;                 // curved gratings: no adjoint
-        if ((int)p[RT_S_GRTYPE] == RT_REFLECTIVE) {
+        if (d.grtype() == RT_REFLECTIVE) {
             const double a = r.cZ, b = L * L - 2.0 * L * r.cX;
             if (b > a * a) return false;
             const double G = -a - dsqrt(a * a - b);
@@ -527,13 +568,14 @@ RT_HD bool fwd_action(const double* p, const St& r, double mu, double L, double&
 // r: state on the surface before the action; (aX, aY, aZ): adjoint of the
 // action's output direction; a: receives the adjoint of r (its point part is
 // added to, its direction part set); aL: adjoint of L
-RT_HD bool adj_action(const double* p, const St& r, double mu, double L, double aX, double aY, double aZ, St& a,
+template <class Rec>
+RT_HD bool adj_action(const Rec& d, const St& r, double mu, double L, double aX, double aY, double aZ, St& a,
                       double& aL, double* gp) {
-    const int type = (int)p[RT_S_TYPE], cls = (int)p[RT_S_CLASS];
+    const int type = d.type(), cls = d.cls();
     aL = 0.0;
     if (type == RT_MIRROR) {
         if (cls == RT_PLANE) { a.cX = aX; a.cY = aY; a.cZ = -aZ; return true; }
-        const Normal N = normal_at(p, r.X, r.Y, r.Z);
+        const Normal N = normal_at(d, r.X, r.Y, r.Z);
         const double D = r.cX * N.x + r.cY * N.y + r.cZ * N.z, M = N.x * N.x + N.y * N.y + N.z * N.z;
         const double aoN = aX * N.x + aY * N.y + aZ * N.z;
         const double iM = drcp(M), q2 = -2.0 * D * iM;
@@ -541,7 +583,7 @@ RT_HD bool adj_action(const double* p, const St& r, double mu, double L, double 
         Normal aN = {q2 * aX + aD * r.cX + 2.0 * aM * N.x, q2 * aY + aD * r.cY + 2.0 * aM * N.y,
                      q2 * aZ + aD * r.cZ + 2.0 * aM * N.z};
         a.cX = aX + aD * N.x; a.cY = aY + aD * N.y; a.cZ = aZ + aD * N.z;
-        normal_adj(p, r.X, r.Y, r.Z, aN, a, gp);
+        normal_adj(d, r.X, r.Y, r.Z, aN, a, gp);
         return true;
     }
     if (type == RT_LENS) {
@@ -551,7 +593,7 @@ RT_HD bool adj_action(const double* p, const St& r, double mu, double L, double 
             a.cX = mu * aX; a.cY = mu * aY; a.cZ = mu * aZ + aa * mu;
             return true;
         }
-        const Normal N = normal_at(p, r.X, r.Y, r.Z);
+        const Normal N = normal_at(d, r.X, r.Y, r.Z);
         const double M = N.x * N.x + N.y * N.y + N.z * N.z;
         const double D = r.cX * N.x + r.cY * N.y + r.cZ * N.z;
         const double iM = drcp(M);
@@ -563,12 +605,12 @@ RT_HD bool adj_action(const double* p, const St& r, double mu, double L, double 
         Normal aN = {G * aX + aD * r.cX + 2.0 * aM * N.x, G * aY + aD * r.cY + 2.0 * aM * N.y,
                      G * aZ + aD * r.cZ + 2.0 * aM * N.z};
         a.cX = mu * aX + aD * N.x; a.cY = mu * aY + aD * N.y; a.cZ = mu * aZ + aD * N.z;
-        normal_adj(p, r.X, r.Y, r.Z, aN, a, gp);
+        normal_adj(d, r.X, r.Y, r.Z, aN, a, gp);
         return true;
     }
     if (type == RT_GRATING) {
         if (cls != RT_PLANE) return false;
-        if ((int)p[RT_S_GRTYPE] == RT_REFLECTIVE) {
+        if (d.grtype() == RT_REFLECTIVE) {
             const double av = r.cZ, b = L * L - 2.0 * L * r.cX, isq = drcp(dsqrt(av * av - b));
             const double aG = aZ, aa = aG * (-1.0 - av * isq), ab = 0.5 * aG * isq;
             a.cX = aX + ab * (-2.0 * L);
@@ -596,36 +638,216 @@ RT_HD bool adj_action(const double* p, const St& r, double mu, double L, double 
 // and L = m lambda / n2 / d of the forward pass (lenses, gratings); a: adjoint
 // of the state after the surface (in), of the entry state (out).  false when
 // the surface has no adjoint here (curved gratings) or the replayed ray is lost.
-RT_HD bool surface_adjoint(const double* p, const St& entry, int n_fwd, double mu, double L, St& a, double* gp) {
-    const int type = (int)p[RT_S_TYPE];
+template <class Rec>
+RT_HD bool surface_adjoint(const Rec& d, const St& entry, int n_fwd, double mu, double L, St& a, double* gp) {
+    const auto p = d.p;
+    const int type = d.type();
     St sT = entry;
-    fwd_transform(p, sT);
+    fwd_transform(d, sT);
     St sN = sT;
     fwd_normalize(sN);
     if (type == RT_AUXTYPE) {
         adj_normalize(sT.cX, sT.cY, sT.cZ, a.cX, a.cY, a.cZ);
-        adj_transform(p, sT, a, gp);
+        adj_transform(d, sT, a, gp);
         return true;
     }
     St sI;
     NewtonSeq q;
-    if (!fwd_intersect(p, sN, n_fwd, sI, q)) return false;
+    if (!fwd_intersect(d, sN, n_fwd, sI, q)) return false;
     double oX, oY, oZ;
-    if (!fwd_action(p, sI, mu, L, oX, oY, oZ)) return false;
+    if (!fwd_action(d, sI, mu, L, oX, oY, oZ)) return false;
     // normalisation after the action
     adj_normalize(oX, oY, oZ, a.cX, a.cY, a.cZ);
     // action: point adjoint accumulates, direction adjoint replaced
     St aI = {a.X, a.Y, a.Z, 0.0, 0.0, 0.0};
     double aL = 0.0;
-    if (!adj_action(p, sI, mu, L, a.cX, a.cY, a.cZ, aI, aL, gp)) return false;
+    if (!adj_action(d, sI, mu, L, a.cX, a.cY, a.cZ, aI, aL, gp)) return false;
     if (type == RT_GRATING && L != 0.0) gp[PG_D] += aL * (-L * drcp(p[RT_S_GRCONST]));
     // intersection, first normalisation, transform
-    adj_intersect(p, sN, sI, q, aI, gp);
+    adj_intersect(d, sN, sI, q, aI, gp);
     adj_normalize(sT.cX, sT.cY, sT.cZ, aI.cX, aI.cY, aI.cZ);
-    adj_transform(p, sT, aI, gp);
+    adj_transform(d, sT, aI, gp);
     a = aI;
     return true;
 }
 
 }  // namespace adj
 }  // namespace rt
+
+#if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+// ------------------------------------------------------------------ kernels
+// The adjoint kernel body, shared by the generic kernel (rt_adjoint.hip,
+// runtime surface descriptors) and the kernels generated per scheme structure
+// and parameter list (rt_jit.cpp, compile-time descriptors and gradient rows).
+// Each lane traces its rays forward through the scheme (the trace's own
+// surface steps), keeping the state on entry to every surface and that
+// surface's Newton iteration count in a tape slot of its own (n_surf x 7
+// doubles per lane, reused ray after ray), then walks the surfaces backwards
+// with surface_adjoint.  The loss enters as a seed per ray group
+// g = ray mod n_groups: adjoint of the final (X, Y) = (ax_g + b_g X, ay_g + b_g Y).
+// Parameter derivatives go to per-lane accumulators (n_par x lanes), reduced
+// afterwards in a fixed order.
+namespace rt {
+namespace adj {
+
+constexpr int kTape = 7;                 // X Y Z cX cY cZ, Newton iterations
+
+struct AdjArgs {
+    double beam[RT_BEAM_STRIDE + 3];     // beam fields + BeamUniform (first: kernarg offset 0)
+    const double* table;                 // device records
+    const double* pool;
+    int n_surf, pool_len, n_par, pad;
+    uint64_t seed, n, n_groups;
+    const double* seeds;                 // n_groups x 3: ax, ay, b
+    const int* prow;                     // n_surf x PG_N: the caller's index of (surface, slot), or -1
+    double* tape;                        // (n_surf * kTape) x lanes
+    double* gacc;                        // n_par x lanes
+    unsigned long long* stats;
+};
+
+// Lane-interleaved rows of doubles (row k of lane l at (k * G + l) * 8 bytes)
+// addressed as a buffer resource: the lane's byte offset stays in one VGPR
+// and the row offset k * G * 8 is formed in an SGPR at the access, from a
+// row size the caller re-reads per ray, so the compiler keeps no per-row
+// address across the ray loop (held, the tape and accumulator addresses of
+// a generated kernel would take more registers than a lane has).  Every
+// access stays inside the resource's byte range, and a buffer access outside
+// it would read 0 and write nothing.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr int kBufWord3 = 0x00020000;    // raw 32-bit dword access (gfx9 buffer resource word 3)
+
+struct LaneRows {
+    double* base;                        // wave-uniform
+    int bytes;                           // byte range of the rows (wave-uniform)
+    uint32_t voff;                       // lane * 8
+    uint32_t row;                        // G * 8 (wave-uniform)
+    // the resource and the row offset are wave-uniform; readfirstlane states
+    // it where control flow with per-lane exits hides that from the compiler
+    __device__ __amdgpu_buffer_rsrc_t rsrc() const {
+        const uint64_t bp = (uint64_t)base;
+        const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)bp);
+        const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(bp >> 32));
+        return __builtin_amdgcn_make_buffer_rsrc((double*)((hi << 32) | lo), (short)0,
+                                                 __builtin_amdgcn_readfirstlane(bytes), kBufWord3);
+    }
+    __device__ uint32_t off(int k) const {
+        return (uint32_t)__builtin_amdgcn_readfirstlane((int)((uint32_t)k * row));
+    }
+    __device__ double ld(int k) const {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc(), voff, off(k), 0));
+    }
+    __device__ void st(int k, double v) const {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rsrc(), voff, off(k), 0);
+    }
+    __device__ void add(int k, double v) const { st(k, ld(k) + v); }
+};
+
+__device__ __forceinline__ LaneRows lane_rows(double* base, uint64_t rows, uint64_t G, uint64_t lane) {
+    LaneRows b;
+    b.base = base;
+    b.bytes = (int)(rows * G * sizeof(double));
+    b.voff = (uint32_t)(lane * sizeof(double));
+    uint32_t row = (uint32_t)(G * sizeof(double));
+    row = (uint32_t)__builtin_amdgcn_readfirstlane((int)row);
+    b.row = row;
+    return b;
+}
+
+// forward through one surface: tape of the entry state (rows k0 .. k0+6),
+// the trace's surface step, the Newton count
+template <class SD>
+__device__ __forceinline__ void fwd_step(const SD& d, int s, Ray& r, bool& alive, TraceCtx& c, const LaneRows& t,
+                                         int k0) {
+    if (alive) {
+        t.st(k0 + 0, r.X); t.st(k0 + 1, r.Y); t.st(k0 + 2, r.Z);
+        t.st(k0 + 3, r.cX); t.st(k0 + 4, r.cY); t.st(k0 + 5, r.cZ);
+    }
+    surface_step(d, s, r, alive, c);
+    if (alive) t.st(k0 + 6, (double)c.last_it);
+}
+
+// backward through one surface; acc(gp) adds the asked-for slots to the
+// lane's accumulators.  false: no adjoint (curved grating, replay lost)
+template <class SD, class Acc>
+__device__ __forceinline__ bool bwd_step(const SD& d, const Ray& r, St& ad, const TraceCtx& c, const LaneRows& t,
+                                         int k0, Acc acc) {
+    const auto p = d.p;
+    const St entry = {t.ld(k0 + 0), t.ld(k0 + 1), t.ld(k0 + 2), t.ld(k0 + 3), t.ld(k0 + 4), t.ld(k0 + 5)};
+    const int nit = (int)t.ld(k0 + 6);
+    // mu and L exactly as the forward pass formed them
+    double mu = 0.0, L = 0.0;
+    if (d.type() == RT_LENS || d.type() == RT_GRATING) {
+        int e1 = 0, e2 = 0;
+        const double n1 = fn_eval(c.pool, d.n1spline(), p[RT_S_N1VAL], r.lam, e1);
+        const double n2 = fn_eval(c.pool, d.n2spline(), p[RT_S_N2VAL], r.lam, e2);
+        const double in2 = rcp(n2);
+        mu = n1 * in2;
+        if (d.type() == RT_GRATING) L = grating_order(d, r.lam) * r.lam * in2 * p[D_INV_D];
+    }
+    double gp[PG_N];
+RT_UNROLL
+    for (int k = 0; k < PG_N; ++k) gp[k] = 0.0;
+    if (!surface_adjoint(d, entry, nit, mu, L, ad, gp)) return false;
+    acc(gp);
+    return true;
+}
+
+// forward(r, alive, c, tape); backward(r, ad, c, tape, acc) -> ok, with
+// tape and acc the lane's rows of a.tape (n_surf * kTape) and a.gacc (n_par);
+// both row sizes must fit a buffer resource (checked by the host)
+template <int BEAM, class Fwd, class Bwd>
+__device__ __forceinline__ void adjoint_body(const AdjArgs& a, Fwd forward, Bwd backward) {
+    extern __shared__ double s_pool[];
+    __shared__ unsigned s_stats[RT_STATS_LEN];
+    for (int i = threadIdx.x; i < RT_STATS_LEN; i += blockDim.x) s_stats[i] = 0u;
+    for (int i = threadIdx.x; i < a.pool_len; i += blockDim.x) s_pool[i] = a.pool[i];
+    __syncthreads();
+
+    TraceCtx c;
+    c.pool = s_pool;
+    c.st = s_stats;
+    c.hist = nullptr;
+    c.flux = nullptr;
+    c.do_hist = c.do_flux = false;
+    c.n_oob = c.n_hit = c.n_off = c.n_edge = 0;
+    c.n_prev = 0.0;
+    c.last_it = 0;
+    const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int j = 0; j < a.n_par; ++j) a.gacc[(uint64_t)j * G + lane] = 0.0;
+    unsigned skipped = 0;
+
+    for (uint64_t i = lane; i < a.n; i += G) {
+        cptr tbl = (cptr)a.table;
+        __asm__ volatile("" : "+s"(tbl));
+        c.table = tbl;
+        Ray r;
+        cptr bm = (cptr)__builtin_amdgcn_kernarg_segment_ptr();
+        gen_ray<BEAM>(bm, a.seed, i, r);
+        bool alive = true;
+        forward(r, alive, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane));
+        unsigned long long good = __ballot(alive), all = __ballot(true);
+        if (__lane_id() == 0) {
+            atomicAdd(&s_stats[RT_ST_RAYS], (unsigned)__popcll(all));
+            if (good) atomicAdd(&s_stats[RT_ST_GOOD], (unsigned)__popcll(good));
+        }
+        if (!alive) continue;
+        const uint64_t g = i % a.n_groups;
+        const double bx = a.seeds[3 * g], by = a.seeds[3 * g + 1], b = a.seeds[3 * g + 2];
+        St ad = {bx + b * r.X, by + b * r.Y, 0.0, 0.0, 0.0, 0.0};
+        if (!backward(r, ad, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane),
+                      lane_rows(a.gacc, (uint64_t)(a.n_par > 0 ? a.n_par : 1), G, lane)))
+            ++skipped;
+    }
+    lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
+    lds_add(&s_stats[RT_ST_ADJ_SKIP], skipped);
+    __syncthreads();
+    if (a.stats) {
+        for (int k = threadIdx.x; k < RT_STATS_LEN; k += blockDim.x)
+            if (s_stats[k]) atomicAdd(&a.stats[k], (unsigned long long)s_stats[k]);
+    }
+}
+
+}  // namespace adj
+}  // namespace rt
+#endif

@@ -1,154 +1,97 @@
 // rt_adjoint.hip -- gradient of a loss on the surviving rays' final positions
 // with respect to scheme parameters, by a hand-written adjoint of the trace.
 //
-// One persistent kernel per call.  Each lane traces its rays forward through
-// the scheme (the same per-ray stages as the trace), keeping only the state on
-// entry to every surface and that surface's Newton iteration count in a tape
-// slot of its own (n_surf x 7 doubles per lane, reused ray after ray), then
-// walks the surfaces backwards with rt::adj::surface_adjoint, which recomputes
-// each surface's internals from its entry state.  The loss enters as a seed per
-// ray group g = ray mod n_groups: adjoint of the final position (X, Y) =
-// (ax_g + b_g X, ay_g + b_g Y), which covers spot centroids, RMS radii and any
-// quadratic of them.  Parameter derivatives are summed per lane into slots of
-// their own and reduced afterwards in a fixed order (bit-reproducible).
+// One persistent kernel per call (body: rt::adj::adjoint_body in
+// rt_adjoint.h): forward with a per-lane tape, then the surfaces backwards.
+// The kernel is generated per scheme structure and parameter list (rt_jit.cpp:
+// compile-time surface descriptors, so every structural branch folds away and
+// the gradient slots are registers); the generic kernel below reads the
+// structure at run time and is the fallback.  Per-lane accumulators are
+// reduced afterwards in two fixed-order stages (bit-reproducible).
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "rt_adjoint.h"
 #include "rt_device.h"
 #include "rt_host.h"
+#include "rt_jit.h"
 
 using rt::cptr;
 using rt::Ray;
+using rt::adj::AdjArgs;
+using rt::adj::kTape;
 
 namespace {
 
 constexpr int kAdjBlock = 256;
-constexpr int kTape = 7;                 // X Y Z cX cY cZ, Newton iterations
-
-struct AdjArgs {
-    double beam[RT_BEAM_STRIDE + 3];     // beam fields + BeamUniform (first: kernarg offset 0)
-    const double* table;                 // device records
-    const double* pool;
-    int n_surf, pool_len, n_par, pad;
-    uint64_t seed, n, n_groups;
-    const double* seeds;                 // n_groups x 3: ax, ay, b
-    const int* prow;                     // n_surf x PG_N: the caller's index of (surface, slot), or -1
-    double* tape;                        // (n_surf * kTape) x lanes
-    double* gacc;                        // n_par x lanes
-    unsigned long long* stats;
-};
+constexpr int kReduceChunks = 64;        // first-stage workgroups per parameter
 
 template <int BEAM>
 __global__ __launch_bounds__(kAdjBlock) void rt_adjoint_kernel(AdjArgs a) {
-    extern __shared__ double s_pool[];
-    __shared__ unsigned s_stats[RT_STATS_LEN];
-    for (int i = threadIdx.x; i < RT_STATS_LEN; i += blockDim.x) s_stats[i] = 0u;
-    for (int i = threadIdx.x; i < a.pool_len; i += blockDim.x) s_pool[i] = a.pool[i];
-    __syncthreads();
-
-    rt::TraceCtx c;
-    c.pool = s_pool;
-    c.st = s_stats;
-    c.hist = nullptr;
-    c.flux = nullptr;
-    c.do_hist = c.do_flux = false;
-    c.n_oob = c.n_hit = c.n_off = c.n_edge = 0;
-    c.n_prev = 0.0;
-    c.last_it = 0;
-    const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (int j = 0; j < a.n_par; ++j) a.gacc[(uint64_t)j * G + lane] = 0.0;
-    unsigned skipped = 0;
-
-    for (uint64_t i = lane; i < a.n; i += G) {
-        cptr tbl = (cptr)a.table;
-        __asm__ volatile("" : "+s"(tbl));
-        c.table = tbl;
-        Ray r;
-        cptr bm = (cptr)__builtin_amdgcn_kernarg_segment_ptr();
-        rt::gen_ray<BEAM>(bm, a.seed, i, r);
-        bool alive = true;
-        for (int s = 0; s < a.n_surf; ++s) {
-            double* t = a.tape + (uint64_t)s * kTape * G + lane;
-            if (alive) {
-                t[0] = r.X; t[G] = r.Y; t[2 * G] = r.Z;
-                t[3 * G] = r.cX; t[4 * G] = r.cY; t[5 * G] = r.cZ;
-            }
-            rt::surface_step(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, s, r, alive, c);
-            if (alive) t[6 * G] = (double)c.last_it;
-        }
-        unsigned long long good = __ballot(alive), all = __ballot(true);
-        if (__lane_id() == 0) {
-            atomicAdd(&s_stats[RT_ST_RAYS], (unsigned)__popcll(all));
-            if (good) atomicAdd(&s_stats[RT_ST_GOOD], (unsigned)__popcll(good));
-        }
-        if (!alive) continue;
-        // seed: adjoint of the final position
-        const uint64_t g = i % a.n_groups;
-        const double bx = a.seeds[3 * g], by = a.seeds[3 * g + 1], b = a.seeds[3 * g + 2];
-        rt::adj::St ad = {bx + b * r.X, by + b * r.Y, 0.0, 0.0, 0.0, 0.0};
-        bool ok = true;
-        for (int s = a.n_surf - 1; s >= 0 && ok; --s) {
-            const double* p = a.table + (size_t)s * rt::DEV_STRIDE;
-            const rt::DynSurf d{c.table + (size_t)s * rt::DEV_STRIDE};
-            const double* t = a.tape + (uint64_t)s * kTape * G + lane;
-            const rt::adj::St entry = {t[0], t[G], t[2 * G], t[3 * G], t[4 * G], t[5 * G]};
-            const int nit = (int)t[6 * G];
-            // mu and L exactly as the forward pass formed them
-            double mu = 0.0, L = 0.0;
-            const int type = d.type();
-            if (type == RT_LENS || type == RT_GRATING) {
-                int e1 = 0, e2 = 0;
-                const double n1 = rt::fn_eval(c.pool, d.n1spline(), p[RT_S_N1VAL], r.lam, e1);
-                const double n2 = rt::fn_eval(c.pool, d.n2spline(), p[RT_S_N2VAL], r.lam, e2);
-                const double in2 = rt::rcp(n2);
-                mu = n1 * in2;
-                if (type == RT_GRATING) L = rt::grating_order(d, r.lam) * r.lam * in2 * p[rt::D_INV_D];
-            }
-            double gp[rt::adj::PG_N];
-            for (int k = 0; k < rt::adj::PG_N; ++k) gp[k] = 0.0;
-            ok = rt::adj::surface_adjoint(p, entry, nit, mu, L, ad, gp);
-            if (!ok) break;
-            // the slot loop is unrolled, so gp stays in registers; the row table is uniform
-            const __attribute__((address_space(4))) int* rows =
-                (const __attribute__((address_space(4))) int*)a.prow + s * rt::adj::PG_N;
+    using rt::adj::LaneRows;
+    rt::adj::adjoint_body<BEAM>(
+        a,
+        [&](Ray& r, bool& alive, rt::TraceCtx& c, const LaneRows& t) {
+            for (int s = 0; s < a.n_surf; ++s)
+                rt::adj::fwd_step(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, s, r, alive, c, t, s * kTape);
+        },
+        [&](const Ray& r, rt::adj::St& ad, rt::TraceCtx& c, const LaneRows& t, const LaneRows& g) {
+            bool ok = true;
+            // a wave-uniform loop (a lane whose replay failed skips the rest)
+            for (int s = a.n_surf - 1; s >= 0; --s) {
+                if (!ok) continue;
+                // the row table is uniform: read through the constant address space
+                const __attribute__((address_space(4))) int* rows =
+                    (const __attribute__((address_space(4))) int*)a.prow + s * rt::adj::PG_N;
+                ok = rt::adj::bwd_step(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, r, ad, c, t, s * kTape,
+                                       [&](const double* gp) {
 #pragma unroll
-            for (int sl = 0; sl < rt::adj::PG_N; ++sl) {
-                const int j = rows[sl];
-                if (j >= 0) a.gacc[(uint64_t)j * G + lane] += gp[sl];
+                                           for (int sl = 0; sl < rt::adj::PG_N; ++sl) {
+                                               const int j = rows[sl];
+                                               if (j >= 0) g.add(j, gp[sl]);
+                                           }
+                                       });
             }
-        }
-        if (!ok) ++skipped;
-    }
-    rt::lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
-    rt::lds_add(&s_stats[RT_ST_ADJ_SKIP], skipped);
-    __syncthreads();
-    if (a.stats) {
-        for (int k = threadIdx.x; k < RT_STATS_LEN; k += blockDim.x)
-            if (s_stats[k]) atomicAdd(&a.stats[k], (unsigned long long)s_stats[k]);
-    }
+            return ok;
+        });
 }
 
-// grad[j] = sum over lanes of gacc[j][lane], one workgroup per parameter, in a
-// fixed order
-__global__ __launch_bounds__(256) void rt_adjoint_reduce(const double* gacc, uint64_t G, double* grad) {
+// Fixed-order reduction of the per-lane accumulators: stage 1 sums chunk b of
+// parameter j's lanes (blockIdx = (b, j)) into part[j][b], stage 2 sums the
+// chunks of each parameter; both with the same tree over the workgroup.
+__device__ __forceinline__ double block_sum_256(double v) {
     __shared__ double s[256];
-    const uint64_t j = blockIdx.x;
-    double v = 0.0;
-    for (uint64_t k = threadIdx.x; k < G; k += blockDim.x) v += gacc[j * G + k];
     s[threadIdx.x] = v;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) grad[j] = s[0];
+    return s[0];
+}
+
+__global__ __launch_bounds__(256) void rt_adjoint_reduce_chunks(const double* gacc, uint64_t G, double* part) {
+    const uint64_t j = blockIdx.y, b = blockIdx.x;
+    const uint64_t per = (G + kReduceChunks - 1) / kReduceChunks;
+    const uint64_t lo = b * per, hi = lo + per < G ? lo + per : G;
+    double v = 0.0;
+    for (uint64_t k = lo + threadIdx.x; k < hi; k += blockDim.x) v += gacc[j * G + k];
+    v = block_sum_256(v);
+    if (threadIdx.x == 0) part[j * kReduceChunks + b] = v;
+}
+
+__global__ __launch_bounds__(256) void rt_adjoint_reduce_parts(const double* part, double* grad) {
+    const uint64_t j = blockIdx.x;
+    double v = (int)threadIdx.x < kReduceChunks ? part[j * kReduceChunks + threadIdx.x] : 0.0;
+    v = block_sum_256(v);
+    if (threadIdx.x == 0) grad[j] = v;
 }
 
 struct AdjBuffers {
@@ -157,6 +100,8 @@ struct AdjBuffers {
     size_t tape_len = 0;
     double* gacc = nullptr;
     size_t gacc_len = 0;
+    double* part = nullptr;
+    size_t part_len = 0;
     int* params = nullptr;
     size_t params_len = 0;
 };
@@ -239,15 +184,33 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     a.seeds = group_seeds;
     a.stats = stats;
 
+    // the kernel generated for this structure and parameter list, else the generic one
+    hipFunction_t fn = nullptr;
+    if (rt_jit_enabled()) {
+        // minimum waves per SIMD asked of the generated kernel's register
+        // allocation: 3 (168 VGPRs) measured fastest on the echelle's 48
+        // parameters against 2 and 4 and the compiler's own choice;
+        // RT_HIP_ADJOINT_WAVES overrides (0: the compiler's choice)
+        const char* w = getenv("RT_HIP_ADJOINT_WAVES");
+        fn = rt_jit_adjoint_kernel(rt_scheme_signature(sch), beam_type, host, n_surf, kAdjBlock, w ? atoi(w) : 3,
+                                   rt_scheme_device(sch));
+    }
     auto kern = beam_type == RT_BEAM_PARALLEL ? rt_adjoint_kernel<RT_BEAM_PARALLEL>
               : beam_type == RT_BEAM_CONIC    ? rt_adjoint_kernel<RT_BEAM_CONIC>
                                               : rt_adjoint_kernel<RT_BEAM_GAUSS>;
     const size_t lds = pool_len * sizeof(double);
     int resident = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, kAdjBlock, lds) != hipSuccess || resident < 1)
-        resident = 1;
+    hipError_t oe = fn ? hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, kAdjBlock, lds)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, kAdjBlock, lds);
+    if (oe != hipSuccess || resident < 1) resident = 1;
     uint64_t want = (n_rays + kAdjBlock - 1) / kAdjBlock;
     uint64_t cap = (uint64_t)rt_device_cus() * resident;
+    // the tape (n_surf * kTape rows) and the accumulators (n_params rows) of
+    // G lanes are addressed as buffer resources of at most 2^31 - 1 bytes
+    const uint64_t rows_max = std::max<uint64_t>((uint64_t)n_surf * kTape, (uint64_t)std::max(n_params, 1));
+    const uint64_t lanes_max = 0x7fffffffull / (rows_max * sizeof(double));
+    if (cap > lanes_max / kAdjBlock) cap = lanes_max / kAdjBlock;
+    if (cap < 1) return rt_set_error(RT_ERR_BAD_VALUE, "adjoint: too many parameters");
     const unsigned grid = (unsigned)(want < cap ? want : cap);
     const uint64_t G = (uint64_t)grid * kAdjBlock;
 
@@ -255,12 +218,13 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev), RT_ERR_FAILED);
     if (B.device != dev) {
-        (void)hipFree(B.tape); (void)hipFree(B.gacc); (void)hipFree(B.params);
+        (void)hipFree(B.tape); (void)hipFree(B.gacc); (void)hipFree(B.part); (void)hipFree(B.params);
         B = AdjBuffers();
         B.device = dev;
     }
     if (!grow(B.tape, B.tape_len, (size_t)n_surf * kTape * G) ||
-        !grow(B.gacc, B.gacc_len, (size_t)std::max(n_params, 1) * G))
+        !grow(B.gacc, B.gacc_len, (size_t)std::max(n_params, 1) * G) ||
+        !grow(B.part, B.part_len, (size_t)std::max(n_params, 1) * kReduceChunks))
         return rt_set_error(RT_ERR_BAD_ALLOC, "adjoint buffers");
     if (B.params_len < host.size()) {
         (void)hipFree(B.params);
@@ -275,10 +239,19 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     a.prow = B.params;
     a.tape = B.tape;
     a.gacc = B.gacc;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kAdjBlock), lds, st, a);
+    if (fn) {
+        void* params[] = {&a};
+        HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, kAdjBlock, 1, 1, (unsigned)lds, st, params, nullptr),
+                RT_ERR_FAILED);
+    } else {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kAdjBlock), lds, st, a);
+    }
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     if (n_params > 0) {
-        hipLaunchKernelGGL(rt_adjoint_reduce, dim3(n_params), dim3(256), 0, st, (const double*)B.gacc, G, grad);
+        hipLaunchKernelGGL(rt_adjoint_reduce_chunks, dim3(kReduceChunks, n_params), dim3(256), 0, st,
+                           (const double*)B.gacc, G, B.part);
+        HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+        hipLaunchKernelGGL(rt_adjoint_reduce_parts, dim3(n_params), dim3(256), 0, st, (const double*)B.part, grad);
         HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     }
     // the host copy of the parameter list must outlive the asynchronous upload

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 // Record an error message for rt_hip_last_error and return `code`.
 int rt_set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 
@@ -23,3 +25,8 @@ int rt_device_cus();
 // public fields plus the host-derived constants; spline references are
 // copied unchanged.
 void rt_device_record(const double* pub, double* dev);
+
+// Structural key of a scheme's generated kernels and the device it lives on.
+struct rt_hip_scheme;
+const std::string& rt_scheme_signature(const rt_hip_scheme* s);
+int rt_scheme_device(const rt_hip_scheme* s);

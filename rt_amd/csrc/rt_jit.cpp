@@ -22,7 +22,8 @@
 #include <vector>
 
 #include "rt_device.h"
-#include "rt_embed.inc"   // kEmbedDevice, kEmbedAbi: text of rt_device.h and rt_hip.h
+#include "rt_adjoint.h"
+#include "rt_embed.inc"   // kEmbedDevice, kEmbedAbi, kEmbedAdjoint: text of rt_device.h, rt_hip.h, rt_adjoint.h
 
 namespace {
 
@@ -111,12 +112,52 @@ static std::string generate_source(const std::string& sig, int beam_mode, bool d
     return os.str();
 }
 
+// The adjoint kernel of a scheme structure and parameter list: the forward
+// surface steps and the backward steps as straight-line sequences, each
+// backward step adding exactly the asked-for slots of that surface to the
+// lane's accumulators.
+static std::string generate_adjoint_source(const std::string& sig, int beam_type, const std::vector<int>& rows,
+                                           int n_surf, int block, int waves = 0) {
+    std::vector<std::string> types;
+    size_t pos = 0;
+    while (pos < sig.size()) {
+        size_t end = sig.find(';', pos);
+        types.push_back(sig.substr(pos, end - pos));
+        pos = end + 1;
+    }
+    std::ostringstream os;
+    os << "#include \"rt_adjoint.h\"\n"
+       << "extern \"C\" __global__ __launch_bounds__(" << block;
+    if (waves > 0) os << ", " << waves;
+    os << ") void rt_adjoint_jit(rt::adj::AdjArgs a) {\n"
+       << "  rt::adj::adjoint_body<" << beam_type << ">(a,\n"
+       << "    [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, const rt::adj::LaneRows& t) {\n";
+    for (int s = 0; s < n_surf; ++s)
+        os << "      rt::adj::fwd_step(" << types[s] << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s
+           << ", r, alive, c, t, " << s * rt::adj::kTape << ");\n";
+    os << "    },\n"
+       << "    [&](const rt::Ray& r, rt::adj::St& ad, rt::TraceCtx& c, const rt::adj::LaneRows& t, "
+          "const rt::adj::LaneRows& g) -> bool {\n"
+       << "      bool ok = true;\n";
+    for (int s = n_surf - 1; s >= 0; --s) {
+        os << "      ok = ok && rt::adj::bwd_step(" << types[s] << "{c.table + " << s << " * rt::DEV_STRIDE}, r, ad, c, "
+           << "t, " << s * rt::adj::kTape << ", [&](const double* gp) {";
+        for (int sl = 0; sl < rt::adj::PG_N; ++sl) {
+            const int j = rows[(size_t)s * rt::adj::PG_N + sl];
+            if (j >= 0) os << " g.add(" << j << ", gp[" << sl << "]);";
+        }
+        os << " });\n";
+    }
+    os << "      return ok;\n    });\n}\n";
+    return os.str();
+}
+
 // hiprtc compilation of `src` to a gfx950 code object (no device needed)
 static bool compile_code(const std::string& src, std::vector<char>& code, std::string& log) {
     hiprtcProgram prog;
-    const char* headers[] = {kEmbedDevice, kEmbedAbi};
-    const char* names[] = {"rt_device.h", "rt_hip.h"};
-    if (hiprtcCreateProgram(&prog, src.c_str(), "rt_trace_jit.hip", 2, headers, names) != HIPRTC_SUCCESS) {
+    const char* headers[] = {kEmbedDevice, kEmbedAbi, kEmbedAdjoint};
+    const char* names[] = {"rt_device.h", "rt_hip.h", "rt_adjoint.h"};
+    if (hiprtcCreateProgram(&prog, src.c_str(), "rt_trace_jit.hip", 3, headers, names) != HIPRTC_SUCCESS) {
         log = "hiprtcCreateProgram failed";
         return false;
     }
@@ -142,14 +183,14 @@ static bool compile_code(const std::string& src, std::vector<char>& code, std::s
     return true;
 }
 
-static bool compile(const std::string& src, Entry& e) {
+static bool compile(const std::string& src, Entry& e, const char* name = "rt_trace_jit") {
     std::vector<char> code;
     if (!compile_code(src, code, e.log)) return false;
     if (hipModuleLoadData(&e.module, code.data()) != hipSuccess) {
         e.log += "\nhipModuleLoadData failed";
         return false;
     }
-    if (hipModuleGetFunction(&e.fn, e.module, "rt_trace_jit") != hipSuccess) {
+    if (hipModuleGetFunction(&e.fn, e.module, name) != hipSuccess) {
         e.log += "\nhipModuleGetFunction failed";
         return false;
     }
@@ -174,13 +215,49 @@ hipFunction_t rt_jit_kernel(const std::string& sig, int beam_mode, bool dump, in
     return e.fn;
 }
 
+hipFunction_t rt_jit_adjoint_kernel(const std::string& sig, int beam_type, const std::vector<int>& rows, int n_surf,
+                                    int block, int waves, int device) {
+    std::ostringstream key;
+    key << device << "|adjoint|" << beam_type << "|" << block << "|" << waves << "|";
+    for (size_t k = 0; k < rows.size(); ++k)
+        if (rows[k] >= 0) key << k << ":" << rows[k] << ",";
+    key << "|" << sig;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Entry& e = g_cache[key.str()];
+    if (e.fn) return e.fn;
+    if (e.failed) return nullptr;
+    if (!compile(generate_adjoint_source(sig, beam_type, rows, n_surf, block, waves), e, "rt_adjoint_jit")) {
+        e.failed = true;
+        fprintf(stderr, "rt_hip: generated adjoint kernel failed to build, using the generic kernel:\n%s\n",
+                e.log.c_str());
+        return nullptr;
+    }
+    return e.fn;
+}
+
 // Generate and compile (without loading) the kernel of a device table; the
 // source and the compiler log are returned in `text`.  Host only.
 extern "C" int rt_jit_check_dtable(const double* dtable, int n_surf, int beam_mode, char* text, size_t len,
                                    size_t* code_size) {
-    const bool batch = (beam_mode & RT_JIT_BATCH) != 0;
-    if (batch) beam_mode &= ~RT_JIT_BATCH;
-    std::string src = generate_source(rt_jit_signature(dtable, n_surf), beam_mode, false, rt::kBlock, 0, batch);
+    // a negative mode (rays given) carries no flags
+    const bool batch = beam_mode >= 0 && (beam_mode & RT_JIT_BATCH) != 0;
+    const bool adjoint = beam_mode >= 0 && (beam_mode & RT_JIT_ADJOINT) != 0;
+    if (beam_mode >= 0) beam_mode &= ~(RT_JIT_BATCH | RT_JIT_ADJOINT);
+    std::string src;
+    if (adjoint) {
+        // every slot a surface has (aspheric terms up to its count) asked for
+        std::vector<int> rows((size_t)n_surf * rt::adj::PG_N, -1);
+        int j = 0;
+        for (int s = 0; s < n_surf; ++s) {
+            const int npoly = (int)dtable[(size_t)s * rt::DEV_STRIDE + RT_S_NPOLY];
+            for (int sl = 0; sl < rt::adj::PG_N; ++sl)
+                if (sl < rt::adj::PG_A || sl >= rt::adj::PG_CR || sl - rt::adj::PG_A < npoly)
+                    rows[(size_t)s * rt::adj::PG_N + sl] = j++;
+        }
+        src = generate_adjoint_source(rt_jit_signature(dtable, n_surf), beam_mode, rows, n_surf, rt::kBlock);
+    } else {
+        src = generate_source(rt_jit_signature(dtable, n_surf), beam_mode, false, rt::kBlock, 0, batch);
+    }
     std::vector<char> code;
     std::string log;
     bool ok = compile_code(src, code, log);

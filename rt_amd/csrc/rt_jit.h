@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 // Structural key of a scheme: one StatSurf type per device record.
 std::string rt_jit_signature(const double* dtable, int n_surf);
@@ -19,3 +20,11 @@ bool rt_jit_enabled();
 // launches the generic kernel, which reads the structure from the records).
 hipFunction_t rt_jit_kernel(const std::string& signature, int beam_mode, bool dump, int block, int waves,
                             int device, bool batch = false);
+
+// The adjoint kernel generated for (signature, beam type, gradient rows,
+// lanes per workgroup, minimum waves per SIMD or 0) on `device`: rows holds
+// n_surf x PG_N entries, the caller's parameter index of each (surface, slot)
+// or -1.  Compiled on first use and cached; nullptr on failure (the caller
+// launches the generic adjoint kernel).
+hipFunction_t rt_jit_adjoint_kernel(const std::string& signature, int beam_type, const std::vector<int>& rows,
+                                    int n_surf, int block, int waves, int device);

@@ -112,6 +112,108 @@ __global__ void rt_batch_spots_kernel(const double* tm, const uint64_t* tile_sta
     out[5] = n > 0.0 ? sqrt((qx + qy) / n) : 0.0;
 }
 
+// Spots per ray group (group of ray r = r mod G).  Moments (n, mean x,
+// mean y, M2 x, M2 y) are merged with Chan et al.'s pairwise update.
+struct Mom {
+    double n, mx, my, qx, qy;
+};
+
+__device__ __forceinline__ Mom mom_merge(const Mom& a, const Mom& b) {
+    if (b.n == 0.0) return a;
+    if (a.n == 0.0) return b;
+    const double nn = a.n + b.n, dx = b.mx - a.mx, dy = b.my - a.my, f = b.n / nn;
+    return Mom{nn, a.mx + dx * f, a.my + dy * f, a.qx + b.qx + dx * dx * a.n * f, a.qy + b.qy + dy * dy * a.n * f};
+}
+
+constexpr int kSpotRays = 8;             // rays of one group per thread in stage 1
+
+// stage 1: thread (chunk b, group g) takes the group's kSpotRays rays of the
+// chunk in order (Welford); consecutive threads read consecutive rays
+__global__ void rt_group_moments_kernel(const double* X, const double* Y, const signed char* F, uint64_t n,
+                                        uint64_t G, uint64_t n_chunks, double* part) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_chunks * G) return;
+    const uint64_t b = idx / G, g = idx - b * G;
+    Mom m{0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kSpotRays; ++k) {
+        const uint64_t r = (b * kSpotRays + k) * G + g;
+        if (r < n && F[r] == 1) {
+            const double x = X[r], y = Y[r];
+            m.n += 1.0;
+            const double dx = x - m.mx, dy = y - m.my;
+            m.mx += dx / m.n;
+            m.my += dy / m.n;
+            m.qx += dx * (x - m.mx);
+            m.qy += dy * (y - m.my);
+        }
+    }
+    double* o = part + idx * 5;
+    o[0] = m.n; o[1] = m.mx; o[2] = m.my; o[3] = m.qx; o[4] = m.qy;
+}
+
+// stage 2: one workgroup per group; thread t merges chunks t, t + 256, ... in
+// order, then a fixed tree over the threads
+__global__ __launch_bounds__(256) void rt_group_spots_kernel(const double* part, uint64_t n_chunks, uint64_t G,
+                                                             double* spots) {
+    __shared__ Mom s[256];
+    const uint64_t g = blockIdx.x;
+    Mom m{0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint64_t b = threadIdx.x; b < n_chunks; b += 256) {
+        const double* o = part + (b * G + g) * 5;
+        m = mom_merge(m, Mom{o[0], o[1], o[2], o[3], o[4]});
+    }
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) s[threadIdx.x] = mom_merge(s[threadIdx.x], s[threadIdx.x + k]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const Mom r = s[0];
+        double* out = spots + g * RT_SPOT_STRIDE;
+        out[0] = r.n;
+        out[1] = r.mx;
+        out[2] = r.my;
+        out[3] = r.n > 0.0 ? r.qx / r.n : 0.0;
+        out[4] = r.n > 0.0 ? r.qy / r.n : 0.0;
+        out[5] = r.n > 0.0 ? sqrt((r.qx + r.qy) / r.n) : 0.0;
+    }
+}
+
+// The spot loss of the adjoint and its seeds, one workgroup: per group with
+// live rays, rms + w ((cx - tx)^2 + (cy - ty)^2); seeds (ax, ay, b) with
+// dloss/dX_r = ax + b X_r for the group's rays: b = 1 / (n rms),
+// ax = -b cx + 2 w (cx - tx) / n, likewise for y.  Thread t sums groups
+// t, t + 256, ... in order, then a fixed tree: bit-reproducible.
+__global__ __launch_bounds__(256) void rt_spot_loss_kernel(const double* spots, uint64_t G, const double* target,
+                                                           double w, double* seeds, double* loss) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (uint64_t g = threadIdx.x; g < G; g += 256) {
+        const double* sp = spots + g * RT_SPOT_STRIDE;
+        const double n = sp[0], cx = sp[1], cy = sp[2], rms = sp[5];
+        const double tx = target ? target[2 * g] : 0.0, ty = target ? target[2 * g + 1] : 0.0;
+        double ax = 0.0, ay = 0.0, b = 0.0;
+        if (n > 0.0) {
+            acc += rms + w * ((cx - tx) * (cx - tx) + (cy - ty) * (cy - ty));
+            b = rms > 0.0 ? 1.0 / (n * rms) : 0.0;
+            ax = -b * cx + 2.0 * w * (cx - tx) / n;
+            ay = -b * cy + 2.0 * w * (cy - ty) / n;
+        }
+        seeds[3 * g] = ax;
+        seeds[3 * g + 1] = ay;
+        seeds[3 * g + 2] = b;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = s[0];
+}
+
 // ------------------------------------------------------------------- C ABI
 
 struct rt_hip_scheme {
@@ -950,6 +1052,9 @@ extern "C" int rt_hip_graph_destroy(rt_hip_graph* g) {
 }
 
 // device tables of a scheme, for the other translation units of the library
+const std::string& rt_scheme_signature(const rt_hip_scheme* s) { return s->signature; }
+int rt_scheme_device(const rt_hip_scheme* s) { return s->device; }
+
 extern "C" int rt_hip_scheme_device_tables(const rt_hip_scheme* s, const double** table_dev, const double** pool_dev,
                                            int* n_surf, size_t* pool_len, const double** host_table) {
     if (!s) return set_error(RT_ERR_BAD_VALUE, "scheme is NULL");
@@ -958,5 +1063,53 @@ extern "C" int rt_hip_scheme_device_tables(const rt_hip_scheme* s, const double*
     *n_surf = s->n_surf;
     *pool_len = s->pool_len;
     *host_table = s->table.data();
+    return RT_OK;
+}
+
+// Spots of the alive rays per ray group; partial moments in a per-thread
+// device buffer grown on demand (kept for the life of the thread).
+extern "C" int rt_hip_group_spots(const double* X, const double* Y, const signed char* flag, size_t n,
+                                  size_t n_groups, double* spots, void* stream) {
+    if (!spots || n_groups == 0) return set_error(RT_ERR_BAD_VALUE, "spots NULL or no groups");
+    if (n > 0 && (!X || !Y || !flag)) return set_error(RT_ERR_BAD_VALUE, "NULL ray array");
+    int rc = ensure_device();
+    if (rc != RT_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    // chunks of kSpotRays strides of the groups
+    const uint64_t G = n_groups;
+    const uint64_t strides = (n + G - 1) / G;
+    const uint64_t n_chunks = strides > 0 ? (strides + kSpotRays - 1) / kSpotRays : 1;
+    // per-thread partials, kept for the life of the thread on its device
+    struct Part { double* p = nullptr; size_t cap = 0; int device = -1; };
+    static thread_local Part part;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), RT_ERR_FAILED);
+    const size_t need = n_chunks * G * 5;
+    if (part.device != dev || part.cap < need) {
+        if (part.p && part.device == dev) (void)hipFree(part.p);
+        part = Part();
+        HIP_TRY(hipMalloc(&part.p, need * sizeof(double)), RT_ERR_BAD_ALLOC);
+        part.cap = need;
+        part.device = dev;
+    }
+    // with n == 0 the single chunk is empty and its partials are written as zeros
+    const uint64_t threads = n_chunks * G;
+    hipLaunchKernelGGL(rt_group_moments_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, X, Y,
+                       flag, (uint64_t)n, G, n_chunks, part.p);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    hipLaunchKernelGGL(rt_group_spots_kernel, dim3((unsigned)G), dim3(256), 0, st, (const double*)part.p, n_chunks,
+                       G, spots);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    return RT_OK;
+}
+
+extern "C" int rt_hip_spot_loss(const double* spots, size_t n_groups, const double* target, double weight,
+                                double* seeds, double* loss, void* stream) {
+    if (!spots || !seeds || !loss || n_groups == 0) return set_error(RT_ERR_BAD_VALUE, "NULL argument or no groups");
+    int rc = ensure_device();
+    if (rc != RT_OK) return rc;
+    hipLaunchKernelGGL(rt_spot_loss_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, spots, (uint64_t)n_groups,
+                       target, weight, seeds, loss);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     return RT_OK;
 }

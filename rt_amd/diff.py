@@ -17,7 +17,7 @@ Parameters are (surface index, field) pairs of the packed table
 the units of the scheme description: angles in degrees, distances in mm,
 grating constants in um; ParamSpec.density(s) is a groove density in lines per
 um (1 / grating constant).  SpotLoss.value_and_grad() runs one forward trace
-(rays stored in HBM, spots reduced with torch on the device) and one adjoint
+(rays stored in HBM, spots reduced per group on the device) and one adjoint
 launch; SchemeLoss is the torch.autograd.Function on top.
 """
 from __future__ import annotations
@@ -117,25 +117,10 @@ def _set(srf, p: Param, v: float):
 
 def group_spots(rays, flag, n_groups: int):
     """per group (ray r in group r mod n_groups) of the alive rays: count,
-    centroid x, y and RMS radius (CUDA float64 tensors)"""
-    import torch
-    n = rays.shape[1]
-    g = torch.arange(n, device=rays.device) % n_groups
-    w = (flag == 1).to(torch.float64)
-
-    def gsum(v):
-        # group sums in a fixed order (a reduction over the rows of an n/G x G view);
-        # scattered atomics otherwise
-        if n % n_groups == 0:
-            return v.reshape(n // n_groups, n_groups).sum(dim=0)
-        return torch.zeros(n_groups, dtype=torch.float64, device=v.device).index_add_(0, g, v)
-
-    cnt = gsum(w)
-    safe = torch.clamp(cnt, min=1.0)
-    cx, cy = gsum(w * rays[0]) / safe, gsum(w * rays[1]) / safe
-    dx, dy = rays[0] - cx[g], rays[1] - cy[g]
-    rms = torch.sqrt(gsum(w * (dx * dx + dy * dy)) / safe)
-    return cnt, cx, cy, rms
+    centroid x, y and RMS radius (CUDA float64 tensors), from the fixed-order
+    group reduction of the engine (rt_hip_group_spots)"""
+    sp = lib.group_spots(rays, flag, n_groups)
+    return sp[:, 0], sp[:, 1], sp[:, 2], sp[:, 5]
 
 
 class SpotLoss:
@@ -156,31 +141,26 @@ class SpotLoss:
         rays = torch.zeros((7, n), dtype=torch.float64, device="cuda")
         flag = torch.zeros(n, dtype=torch.int8, device="cuda")
         ds.trace(rays, flag, seed=self.seed, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE)
-        cnt, cx, cy, rms = group_spots(rays, flag, self.n_groups)
-        return sch, ds, cnt, cx, cy, rms
+        sp = lib.group_spots(rays, flag, self.n_groups)
+        return sch, ds, sp
 
     def spots(self, values):
-        _, ds, cnt, cx, cy, rms = self._forward(values)
+        _, ds, sp = self._forward(values)
         ds.close()
-        return cnt, cx, cy, rms
+        return sp[:, 0], sp[:, 1], sp[:, 2], sp[:, 5]
 
     def value_and_grad(self, values, grad=True):
         import torch
-        sch, ds, cnt, cx, cy, rms = self._forward(values)
-        has = cnt > 0
-        tgt = torch.zeros((self.n_groups, 2), dtype=torch.float64, device="cuda")
-        if self.target is not None:
-            tgt = torch.from_numpy(self.target).cuda()
-        dev = (cx - tgt[:, 0]) ** 2 + (cy - tgt[:, 1]) ** 2
-        loss = float(torch.sum(torch.where(has, rms + self.weight * dev, torch.zeros_like(rms))).item())
+        sch, ds, sp = self._forward(values)
+        tgt = None if self.target is None else torch.from_numpy(self.target).cuda().contiguous()
+        # loss and adjoint seeds on the device (rt_hip_spot_loss): per group with
+        # live rays rms + w |centroid - target|^2; dloss/dX_r = ax + b X_r with
+        # b = 1/(n rms), ax = -b cx + 2 w (cx - tx)/n (likewise for y)
+        loss_t, seeds = lib.spot_loss(sp, tgt, self.weight)
+        loss = float(loss_t.item())
         if not grad:
             ds.close()
             return loss, None
-        safe = torch.clamp(cnt, min=1.0)
-        b = torch.where(has & (rms > 0), 1.0 / (safe * torch.clamp(rms, min=1e-300)), torch.zeros_like(rms))
-        ax = torch.where(has, -b * cx + 2.0 * self.weight * (cx - tgt[:, 0]) / safe, torch.zeros_like(rms))
-        ay = torch.where(has, -b * cy + 2.0 * self.weight * (cy - tgt[:, 1]) / safe, torch.zeros_like(rms))
-        seeds = torch.stack([ax, ay, b], dim=1).contiguous()
         g = ds.adjoint(seeds, self.ps, self.pf, seed=self.seed)
         torch.cuda.synchronize()
         ds.close()

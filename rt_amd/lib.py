@@ -57,6 +57,8 @@ def lib():
                                                 [_vp, _vp, _vp, C.POINTER(_vp)])
         L.rt_hip_adjoint.argtypes = [_vp, _dp, C.c_uint64, C.c_size_t, _vp, C.c_size_t,
                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, _vp, _vp, _vp]
+        L.rt_hip_group_spots.argtypes = [_vp, _vp, _vp, C.c_size_t, C.c_size_t, _vp, _vp]
+        L.rt_hip_spot_loss.argtypes = [_vp, C.c_size_t, _vp, C.c_double, _vp, _vp, _vp]
         L.rt_hip_graph_launch.argtypes = [_vp, _vp]
         L.rt_hip_graph_destroy.argtypes = [_vp]
         L.rt_hip_jit_check.argtypes = [_dp, C.c_int, _dp, C.c_size_t, C.c_int, C.c_char_p, C.c_size_t,
@@ -234,6 +236,36 @@ def init(device: int = 0):
 
 
 SPOT_STRIDE = 6     # RT_SPOT_STRIDE: n, centroid x, y, variance x, y, RMS radius
+
+
+def group_spots(rays, flag, n_groups: int, stream=None):
+    """rt_hip_group_spots: (n_groups, SPOT_STRIDE) float64 CUDA tensor of the
+    spots of the rays with flag 1, ray r in group r mod n_groups (rays: the
+    7 x n state on the device, flag: n int8)."""
+    import torch
+    n = rays.shape[1]
+    if rays.dtype != torch.float64 or not rays.is_contiguous() or flag.dtype != torch.int8:
+        raise EngineError("rays must be contiguous float64 and flag int8")
+    out = torch.empty((n_groups, SPOT_STRIDE), dtype=torch.float64, device=rays.device)
+    st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    check(lib().rt_hip_group_spots(C.c_void_p(rays[0].data_ptr()), C.c_void_p(rays[1].data_ptr()),
+                                   C.c_void_p(flag.data_ptr()), n, n_groups, C.c_void_p(out.data_ptr()),
+                                   C.c_void_p(st)), "rt_hip_group_spots")
+    return out
+
+
+def spot_loss(spots, target=None, weight: float = 0.0, stream=None):
+    """rt_hip_spot_loss: (loss, seeds) as CUDA tensors (a 1-element tensor and
+    n_groups x 3) from group_spots' output; target: n_groups x 2 or None."""
+    import torch
+    G = spots.shape[0]
+    seeds = torch.empty((G, 3), dtype=torch.float64, device=spots.device)
+    loss = torch.empty(1, dtype=torch.float64, device=spots.device)
+    st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    tp = None if target is None else C.c_void_p(target.data_ptr())
+    check(lib().rt_hip_spot_loss(C.c_void_p(spots.data_ptr()), G, tp, float(weight), C.c_void_p(seeds.data_ptr()),
+                                 C.c_void_p(loss.data_ptr()), C.c_void_p(st)), "rt_hip_spot_loss")
+    return loss, seeds
 
 
 def pack_batch(instances):

@@ -11,7 +11,7 @@ extern "C" int adj_host_surface(const double* rec, const double* entry, int n_fw
                                 double* a6, double* gp) {
     rt::adj::St e = {entry[0], entry[1], entry[2], entry[3], entry[4], entry[5]};
     rt::adj::St a = {a6[0], a6[1], a6[2], a6[3], a6[4], a6[5]};
-    bool ok = rt::adj::surface_adjoint(rec, e, n_fwd, mu, L, a, gp);
+    bool ok = rt::adj::surface_adjoint(rt::adj::DynRec{rec}, e, n_fwd, mu, L, a, gp);
     a6[0] = a.X; a6[1] = a.Y; a6[2] = a.Z; a6[3] = a.cX; a6[4] = a.cY; a6[5] = a.cZ;
     return ok ? 0 : 1;
 }

@@ -94,6 +94,19 @@ def test_generated_kernel_compiles(make, mode):
     assert text.count("rt::surface_step(") == make().n_surf
 
 
+def test_generated_adjoint_kernel_compiles():
+    """The adjoint kernel generated for the echelle with every slot of every
+    surface asked for (0x200 = RT_JIT_ADJOINT) compiles: one forward and one
+    backward step per surface, straight-line."""
+    sch = schemes.echelle(1, 4)
+    rc, size, text = lib.jit_check(sch, S.RT_BEAM_GAUSS | 0x200)
+    assert rc == 0, text[-2000:]
+    assert size > 10000
+    assert "rt_adjoint_jit" in text
+    assert text.count("rt::adj::fwd_step(") == sch.n_surf
+    assert text.count("rt::adj::bwd_step(") == sch.n_surf
+
+
 ENGINE_HEADER = os.path.join(ROOT, "include", "rt", "rt_engine_func.h")
 ENGINE_LIB = os.path.join(ROOT, "rt_amd", "librt_engine_hip.so")
 COMMON_LIB = os.path.join(ROOT, "rt_amd", "librt_common_hip.so")

@@ -136,3 +136,67 @@ def test_autograd_function(hip):
     loss = SchemeLoss.apply(x, L)
     loss.backward()
     assert np.array_equal(x.grad.numpy(), grad)
+
+
+@pytest.mark.parametrize("n,n_groups", [(10007, 64), (4096, 1), (3001, 7), (20000, 300), (0, 5)])
+def test_group_spots_match_two_pass_moments(hip, n, n_groups):
+    """rt_hip_group_spots (fixed-order Welford per chunk, Chan across chunks)
+    against two-pass moments in numpy: counts exact, centroids, variances and
+    RMS radii within 1e-12 relative; groups without a live ray give zeros."""
+    print("[group spots]", n, n_groups, flush=True)
+    rng = np.random.default_rng(n + n_groups)
+    rays = np.zeros((7, max(n, 1)))
+    rays[0] = rng.normal(30.0, 2.0, rays.shape[1])
+    rays[1] = rng.normal(-4.0, 0.5, rays.shape[1])
+    flag = (rng.random(rays.shape[1]) < 0.8).astype(np.int8)
+    if n_groups > 2 and n > 0:
+        flag[2::n_groups] = 0                           # group 2 has no live ray
+    R = torch.tensor(rays[:, :n] if n else rays[:, :0], dtype=torch.float64, device="cuda").contiguous()
+    F = torch.tensor(flag[:n], dtype=torch.int8, device="cuda")
+    sp = hip.group_spots(R, F, n_groups).cpu().numpy()
+    torch.cuda.synchronize()
+    for g in range(n_groups):
+        idx = np.arange(g, n, n_groups)
+        live = idx[flag[idx] == 1] if n else idx
+        assert sp[g, 0] == len(live)
+        if len(live) == 0:
+            assert np.all(sp[g] == 0.0)
+            continue
+        x, y = rays[0, live], rays[1, live]
+        cx, cy = x.mean(), y.mean()
+        vx, vy = np.mean((x - cx) ** 2), np.mean((y - cy) ** 2)
+        want = np.array([cx, cy, vx, vy, math.sqrt(vx + vy)])
+        assert np.all(np.abs(sp[g, 1:] - want) <= 1e-12 * np.maximum(np.abs(want), 1.0)), (g, sp[g], want)
+
+
+def test_spot_loss_and_seeds(hip):
+    """rt_hip_spot_loss against the loss and seeds written out in numpy, with
+    and without a target, including a group without live rays."""
+    print("[spot loss]", flush=True)
+    rng = np.random.default_rng(11)
+    G = 37
+    sp = np.zeros((G, 6))
+    sp[:, 0] = rng.integers(1, 500, G)
+    sp[5, 0] = 0
+    sp[:, 1] = rng.normal(10, 3, G)
+    sp[:, 2] = rng.normal(-2, 1, G)
+    sp[:, 3] = rng.random(G) * 1e-3
+    sp[:, 4] = rng.random(G) * 1e-3
+    sp[:, 5] = np.sqrt(sp[:, 3] + sp[:, 4])
+    sp[5, 1:] = 0.0
+    tgt = rng.normal(0, 1, (G, 2))
+    for target, w in ((None, 0.0), (tgt, 0.7)):
+        t = np.zeros((G, 2)) if target is None else target
+        live = sp[:, 0] > 0
+        n, cx, cy, rms = sp[:, 0], sp[:, 1], sp[:, 2], sp[:, 5]
+        loss = np.sum((rms + w * ((cx - t[:, 0]) ** 2 + (cy - t[:, 1]) ** 2))[live])
+        b = np.where(live, 1.0 / np.maximum(n * rms, 1e-300), 0.0)
+        ax = np.where(live, -b * cx + 2 * w * (cx - t[:, 0]) / np.maximum(n, 1), 0.0)
+        ay = np.where(live, -b * cy + 2 * w * (cy - t[:, 1]) / np.maximum(n, 1), 0.0)
+        S_ = torch.tensor(sp, dtype=torch.float64, device="cuda")
+        T_ = None if target is None else torch.tensor(target, dtype=torch.float64, device="cuda")
+        gl, gs = hip.spot_loss(S_, T_, w)
+        gs = gs.cpu().numpy()
+        assert abs(float(gl.item()) - loss) <= 1e-12 * abs(loss)
+        want = np.stack([ax, ay, b], axis=1)
+        assert np.all(np.abs(gs - want) <= 1e-12 * np.maximum(np.abs(want), 1e-30))
