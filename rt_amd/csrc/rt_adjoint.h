@@ -766,12 +766,16 @@ __device__ __forceinline__ void fwd_step(const SD& d, int s, Ray& r, bool& alive
     if (alive) t.st(k0 + 6, (double)c.last_it);
 }
 
-// backward through one surface; acc(gp) adds the asked-for slots to the
-// lane's accumulators.  false: no adjoint (curved grating, replay lost)
-template <class SD, class Acc>
+// backward through one surface: gp receives the surface's parameter
+// derivatives (zero where the replay fails).  false: no adjoint (curved
+// grating, replay lost).  The lanes of the wave are converged again when it
+// returns, so the caller can reduce gp across the wave.
+template <class SD>
 __device__ __forceinline__ bool bwd_step(const SD& d, const Ray& r, St& ad, const TraceCtx& c, const LaneRows& t,
-                                         int k0, Acc acc) {
+                                         int k0, double* gp) {
     const auto p = d.p;
+RT_UNROLL
+    for (int k = 0; k < PG_N; ++k) gp[k] = 0.0;
     const St entry = {t.ld(k0 + 0), t.ld(k0 + 1), t.ld(k0 + 2), t.ld(k0 + 3), t.ld(k0 + 4), t.ld(k0 + 5)};
     const int nit = (int)t.ld(k0 + 6);
     // mu and L exactly as the forward pass formed them
@@ -784,17 +788,36 @@ __device__ __forceinline__ bool bwd_step(const SD& d, const Ray& r, St& ad, cons
         mu = n1 * in2;
         if (d.type() == RT_GRATING) L = grating_order(d, r.lam) * r.lam * in2 * p[D_INV_D];
     }
-    double gp[PG_N];
-RT_UNROLL
-    for (int k = 0; k < PG_N; ++k) gp[k] = 0.0;
-    if (!surface_adjoint(d, entry, nit, mu, L, ad, gp)) return false;
-    acc(gp);
-    return true;
+    return surface_adjoint(d, entry, nit, mu, L, ad, gp);
 }
 
-// forward(r, alive, c, tape); backward(r, ad, c, tape, acc) -> ok, with
-// tape and acc the lane's rows of a.tape (n_surf * kTape) and a.gacc (n_par);
-// both row sizes must fit a buffer resource (checked by the host)
+// Sum over the 64 lanes of the wave, butterfly order (lane 0's value is the
+// same on every run).  Every lane must be active.
+__device__ __forceinline__ double wave_sum(double v) {
+RT_UNROLL
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Per-wave gradient accumulators: row j holds one double per wave (row size
+// W * 8 bytes); add(j, v) sums v over the wave and the wave's first lane adds
+// it to its own slot (one writer per slot: no race, fixed order).
+struct WaveAcc {
+    LaneRows rows;
+    bool lead;
+    __device__ void add(int j, double v) const {
+        const double sum = wave_sum(v);
+        if (lead) rows.st(j, rows.ld(j) + sum);
+    }
+};
+
+// forward(r, alive, c, tape); backward(r, ad, c, tape, acc, ok) -> ok, with
+// tape the lane's rows of a.tape (n_surf * kTape rows of G lanes) and acc the
+// wave's accumulators in a.gacc (n_par rows of G / 64 waves).  The ray loop is
+// wave-uniform (a wave's lanes take rays i0 .. i0 + 63 together and lanes past
+// the end run masked), so that the backward steps can reduce across the wave;
+// lanes whose ray is lost or whose replay fails compute on and are masked out
+// of the sums.  The tape size must fit a buffer resource (checked by the host).
 template <int BEAM, class Fwd, class Bwd>
 __device__ __forceinline__ void adjoint_body(const AdjArgs& a, Fwd forward, Bwd backward) {
     extern __shared__ double s_pool[];
@@ -812,32 +835,38 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs& a, Fwd forward, Bwd 
     c.n_oob = c.n_hit = c.n_off = c.n_edge = 0;
     c.n_prev = 0.0;
     c.last_it = 0;
-    const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t G = (uint64_t)gridDim.x * blockDim.x;      // lanes (blockDim a multiple of 64)
+    const uint64_t W = G / 64;                                 // waves
     const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (int j = 0; j < a.n_par; ++j) a.gacc[(uint64_t)j * G + lane] = 0.0;
+    const uint64_t wave = lane / 64, lw = lane % 64;
+    const uint64_t n_par = a.n_par > 0 ? (uint64_t)a.n_par : 1;
+    if (lw == 0)
+        for (uint64_t j = 0; j < (uint64_t)a.n_par; ++j) a.gacc[j * W + wave] = 0.0;
     unsigned skipped = 0;
 
-    for (uint64_t i = lane; i < a.n; i += G) {
+    for (uint64_t i0 = wave * 64; i0 < a.n; i0 += G) {
+        const uint64_t i = i0 + lw;
+        const bool in = i < a.n;
         cptr tbl = (cptr)a.table;
         __asm__ volatile("" : "+s"(tbl));
         c.table = tbl;
         Ray r;
         cptr bm = (cptr)__builtin_amdgcn_kernarg_segment_ptr();
         gen_ray<BEAM>(bm, a.seed, i, r);
-        bool alive = true;
+        bool alive = in;
         forward(r, alive, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane));
-        unsigned long long good = __ballot(alive), all = __ballot(true);
-        if (__lane_id() == 0) {
+        unsigned long long good = __ballot(alive), all = __ballot(in);
+        if (lw == 0) {
             atomicAdd(&s_stats[RT_ST_RAYS], (unsigned)__popcll(all));
             if (good) atomicAdd(&s_stats[RT_ST_GOOD], (unsigned)__popcll(good));
         }
-        if (!alive) continue;
+        if (!good) continue;                                   // wave-uniform
         const uint64_t g = i % a.n_groups;
         const double bx = a.seeds[3 * g], by = a.seeds[3 * g + 1], b = a.seeds[3 * g + 2];
         St ad = {bx + b * r.X, by + b * r.Y, 0.0, 0.0, 0.0, 0.0};
-        if (!backward(r, ad, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane),
-                      lane_rows(a.gacc, (uint64_t)(a.n_par > 0 ? a.n_par : 1), G, lane)))
-            ++skipped;
+        const WaveAcc acc{lane_rows(a.gacc, n_par, W, wave), lw == 0};
+        const bool ok = backward(r, ad, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane), acc, alive);
+        if (alive && !ok) ++skipped;
     }
     lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
     lds_add(&s_stats[RT_ST_ADJ_SKIP], skipped);

@@ -42,29 +42,29 @@ __global__ __launch_bounds__(kAdjBlock) void rt_adjoint_kernel(AdjArgs a) {
             for (int s = 0; s < a.n_surf; ++s)
                 rt::adj::fwd_step(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, s, r, alive, c, t, s * kTape);
         },
-        [&](const Ray& r, rt::adj::St& ad, rt::TraceCtx& c, const LaneRows& t, const LaneRows& g) {
-            bool ok = true;
-            // a wave-uniform loop (a lane whose replay failed skips the rest)
+        [&](const Ray& r, rt::adj::St& ad, rt::TraceCtx& c, const LaneRows& t, const rt::adj::WaveAcc& g,
+            bool ok) {
             for (int s = a.n_surf - 1; s >= 0; --s) {
-                if (!ok) continue;
                 // the row table is uniform: read through the constant address space
                 const __attribute__((address_space(4))) int* rows =
                     (const __attribute__((address_space(4))) int*)a.prow + s * rt::adj::PG_N;
-                ok = rt::adj::bwd_step(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, r, ad, c, t, s * kTape,
-                                       [&](const double* gp) {
+                double gp[rt::adj::PG_N];
+                const bool res =
+                    rt::adj::bwd_step(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, r, ad, c, t, s * kTape, gp);
+                const bool use = ok && res;
 #pragma unroll
-                                           for (int sl = 0; sl < rt::adj::PG_N; ++sl) {
-                                               const int j = rows[sl];
-                                               if (j >= 0) g.add(j, gp[sl]);
-                                           }
-                                       });
+                for (int sl = 0; sl < rt::adj::PG_N; ++sl) {
+                    const int j = rows[sl];
+                    if (j >= 0) g.add(j, use ? gp[sl] : 0.0);
+                }
+                ok = use;
             }
             return ok;
         });
 }
 
-// Fixed-order reduction of the per-lane accumulators: stage 1 sums chunk b of
-// parameter j's lanes (blockIdx = (b, j)) into part[j][b], stage 2 sums the
+// Fixed-order reduction of the per-wave accumulators: stage 1 sums chunk b of
+// parameter j's waves (blockIdx = (b, j)) into part[j][b], stage 2 sums the
 // chunks of each parameter; both with the same tree over the workgroup.
 __device__ __forceinline__ double block_sum_256(double v) {
     __shared__ double s[256];
@@ -223,7 +223,7 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
         B.device = dev;
     }
     if (!grow(B.tape, B.tape_len, (size_t)n_surf * kTape * G) ||
-        !grow(B.gacc, B.gacc_len, (size_t)std::max(n_params, 1) * G) ||
+        !grow(B.gacc, B.gacc_len, (size_t)std::max(n_params, 1) * (G / 64)) ||
         !grow(B.part, B.part_len, (size_t)std::max(n_params, 1) * kReduceChunks))
         return rt_set_error(RT_ERR_BAD_ALLOC, "adjoint buffers");
     if (B.params_len < host.size()) {
@@ -249,7 +249,7 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     if (n_params > 0) {
         hipLaunchKernelGGL(rt_adjoint_reduce_chunks, dim3(kReduceChunks, n_params), dim3(256), 0, st,
-                           (const double*)B.gacc, G, B.part);
+                           (const double*)B.gacc, G / 64, B.part);
         HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
         hipLaunchKernelGGL(rt_adjoint_reduce_parts, dim3(n_params), dim3(256), 0, st, (const double*)B.part, grad);
         HIP_TRY(hipGetLastError(), RT_ERR_FAILED);

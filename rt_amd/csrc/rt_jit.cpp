@@ -137,16 +137,15 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
            << ", r, alive, c, t, " << s * rt::adj::kTape << ");\n";
     os << "    },\n"
        << "    [&](const rt::Ray& r, rt::adj::St& ad, rt::TraceCtx& c, const rt::adj::LaneRows& t, "
-          "const rt::adj::LaneRows& g) -> bool {\n"
-       << "      bool ok = true;\n";
+          "const rt::adj::WaveAcc& g, bool ok) -> bool {\n";
     for (int s = n_surf - 1; s >= 0; --s) {
-        os << "      ok = ok && rt::adj::bwd_step(" << types[s] << "{c.table + " << s << " * rt::DEV_STRIDE}, r, ad, c, "
-           << "t, " << s * rt::adj::kTape << ", [&](const double* gp) {";
+        os << "      { double gp[rt::adj::PG_N]; const bool use = rt::adj::bwd_step(" << types[s] << "{c.table + " << s
+           << " * rt::DEV_STRIDE}, r, ad, c, t, " << s * rt::adj::kTape << ", gp) && ok;";
         for (int sl = 0; sl < rt::adj::PG_N; ++sl) {
             const int j = rows[(size_t)s * rt::adj::PG_N + sl];
-            if (j >= 0) os << " g.add(" << j << ", gp[" << sl << "]);";
+            if (j >= 0) os << " g.add(" << j << ", use ? gp[" << sl << "] : 0.0);";
         }
-        os << " });\n";
+        os << " ok = use; }\n";
     }
     os << "      return ok;\n    });\n}\n";
     return os.str();
