@@ -59,6 +59,10 @@ def main():
     ap.add_argument("--batch", type=int, default=0, metavar="N_INST",
                     help="measure the batched workload instead: N_INST Monte-Carlo tolerance instances of "
                          "the echelle, ray counts log-uniform in [1e4, 1e6], one persistent launch per step")
+    ap.add_argument("--batch-adjoint", type=int, default=0, metavar="N_INST",
+                    help="batched adjoint/forward ratio over N_INST tolerance instances of the echelle")
+    ap.add_argument("--max-rays", type=int, default=1_000_000,
+                    help="largest instance of the batched modes (log-uniform from 1e4)")
     ap.add_argument("--adjoint", action="store_true",
                     help="measure the adjoint instead: gradient of the spot loss over the echelle's 48 design "
                          "parameters at 1e6 rays over 64 wavelengths, as a time ratio to the plain forward trace")
@@ -88,6 +92,8 @@ def main():
         return bench_batch(args)
     if args.adjoint:
         return bench_adjoint(args)
+    if args.batch_adjoint:
+        return bench_batch_adjoint(args)
     sch = schemes.echelle(args.n_per_lambda, args.nlambda)
     n = sch.beam.n_rays
     ds = lib.DeviceScheme(sch)
@@ -277,6 +283,81 @@ def bench_batch(args):
                    "ray_surface_per_step": ray_surf // args.steps,
                    "median_rms_spot_um": float(np.median(sp[:, 5]) * 1e3),
                    "rays_per_s": int(sum(counts)) * args.steps / wall}}))
+
+
+def bench_batch_adjoint(args):
+    """Batched adjoint/forward ratio: N_INST tolerance instances of the echelle
+    (64 wavelengths, ray counts log-uniform in [1e4, --max-rays]), the config-4
+    loss per instance (sum over wavelengths of the RMS spot radius), gradient of
+    every instance over the 48 design parameters.  Times per step: the plain
+    batched forward (beams generated, nothing stored), the loss forward (rays
+    stored; spots per instance and wavelength, loss and adjoint seeds on the
+    device), the batched adjoint (kernel + per-instance reduction).  Ratio =
+    (loss forward + adjoint) / plain forward on the same rays and surfaces; the
+    per ray*surface ratio against the single-scheme forward rate of the
+    default bench is in detail."""
+    import torch
+    from rt_amd import diff, lib, schemes
+    from rt_amd import scheme as S
+    base = schemes.echelle(n_per_lambda=4883, nlambda=64)
+    spec = diff.echelle_design_params(base)
+    inst, counts = schemes.tolerance_instances(base, args.batch_adjoint, seed=1, min_rays=10_000,
+                                               max_rays=args.max_rays)
+    db = lib.DeviceBatch(inst, counts)
+    total = int(sum(counts))
+    rays = torch.zeros((7, total), dtype=torch.float64, device="cuda")
+    flag = torch.zeros(total, dtype=torch.int8, device="cuda")
+    ps = np.array([p.surface for p in spec.params], dtype=np.int32)
+    pf = np.array([p.field for p in spec.params], dtype=np.int32)
+    seeds = torch.zeros((len(inst), 64, 3), dtype=torch.float64, device="cuda")
+    grad = torch.zeros((len(inst), len(ps)), dtype=torch.float64, device="cuda")
+    stats = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
+
+    def plain():
+        db.trace(seed=5, flags=S.RT_TRACE_GENERATE, stats=stats)
+
+    def loss_forward():
+        db.trace(seed=5, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, rays=rays, flag=flag)
+        sp = db.group_spots(rays, flag, 64)
+        loss, sd = lib.spot_loss(sp.reshape(len(inst) * 64, lib.SPOT_STRIDE))
+        seeds.copy_(sd.reshape(len(inst), 64, 3))
+        return loss
+
+    def adjoint():
+        db.adjoint(seeds, ps, pf, seed=5, grad=grad)
+
+    def timed(fn, k):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(k):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    for _ in range(args.warmup):
+        plain(); loss_forward(); adjoint()
+    torch.cuda.synchronize()
+    stats.zero_()
+    t_plain = timed(plain, args.steps)
+    st = stats.cpu().numpy().view(np.uint64)
+    ray_surf = int(sum(int(st[S.ST_SURF + 4 * s]) for s in range(base.n_surf))) // args.steps
+    t_loss = timed(loss_forward, args.steps)
+    t_adj = timed(adjoint, args.steps)
+    g = grad.cpu().numpy()
+    print(json.dumps({
+        "metric": f"batched adjoint/forward time ratio ({len(inst)} echelle tolerance instances, 48 parameters)",
+        "value": (t_loss + t_adj) / t_plain, "unit": "x", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": False, "dtype": "f64",
+        "data": "synthetic (Philox beams, seed + instance)",
+        "config": {"workload": "echelle-tolerance-batch-adjoint", "instances": len(inst), "rays": total,
+                   "min_rays": int(min(counts)), "max_rays": int(max(counts)), "wavelengths": 64,
+                   "parameters": len(ps)},
+        "detail": {"plain_forward_ms": t_plain, "loss_forward_ms": t_loss, "adjoint_ms": t_adj,
+                   "ray_surface_per_step": ray_surf,
+                   "forward_ray_surface_per_s": ray_surf / (t_plain * 1e-3),
+                   "finite_gradients": bool(np.all(np.isfinite(g)))}}))
 
 
 if __name__ == "__main__":

@@ -359,6 +359,26 @@ int rt_hip_group_spots(const double *X, const double *Y, const signed char *flag
 int rt_hip_spot_loss(const double *spots, size_t n_groups, const double *target, double weight, double *seeds,
                      double *loss, void *stream);
 
+/* The adjoint over every instance of a batch (rt_hip_batch_create), each
+   with its own beam (seed + i, rays 0 .. ray_counts[i]-1) and group seeds
+   (device, n_inst x n_groups x 3: group g of instance i's ray r is r mod
+   n_groups); grad (device, n_inst x n_params) receives each instance's
+   gradient with respect to its own parameters, reduced in a fixed order.
+   The batch must use workgroups of 256 lanes.  No counterpart in the
+   reference. */
+int rt_hip_batch_adjoint(const rt_hip_batch *batch, const double *beam_desc, uint64_t seed,
+                         const double *group_seeds, size_t n_groups, const int *param_surface,
+                         const int *param_field, int n_params, double *grad, unsigned long long *stats,
+                         void *stream);
+
+/* Spots per (instance, group) of a batch's stored rays (rt_hip_batch_trace
+   with RT_TRACE_STORE, rays concatenated by instance; instance i's ray r is in
+   group r mod n_groups): spots (device) receives n_inst x n_groups x
+   RT_SPOT_STRIDE, with the summation order of rt_hip_group_spots on each
+   instance's rays alone. */
+int rt_hip_batch_group_spots(const rt_hip_batch *batch, const double *X, const double *Y,
+                             const signed char *flag, size_t n_groups, double *spots, void *stream);
+
 /* "generated" or "generic": the kernel the calling thread's last trace used. */
 int rt_hip_last_kernel(char *buf, size_t len);
 

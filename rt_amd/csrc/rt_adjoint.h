@@ -701,8 +701,16 @@ struct AdjArgs {
     const double* seeds;                 // n_groups x 3: ax, ay, b
     const int* prow;                     // n_surf x PG_N: the caller's index of (surface, slot), or -1
     double* tape;                        // (n_surf * kTape) x lanes
-    double* gacc;                        // n_par x lanes
+    double* gacc;                        // n_par x waves (single scheme); tile-wave x n_par (batch)
     unsigned long long* stats;
+    // batched adjoint (adjoint_batch_body): instance i owns records
+    // table[i * inst_stride ..], rays 0 .. ray_start[i+1] - ray_start[i] - 1 of
+    // its own beam (seed + i), tiles [tile_start[i], tile_start[i+1]) of one
+    // workgroup each, and group seeds seeds[(i * n_groups + g) * 3 ..]
+    const uint64_t* tile_start;
+    const uint64_t* ray_start;
+    const uint32_t* tile_inst;           // instance of each tile
+    uint64_t n_inst, n_tiles, inst_stride;
 };
 
 // Lane-interleaved rows of doubles (row k of lane l at (k * G + l) * 8 bytes)
@@ -811,6 +819,18 @@ struct WaveAcc {
     }
 };
 
+// Gradient partials of one tile-wave of a batch: row tw of n_par doubles
+// (tw = tile * waves per workgroup + wave); add(j, v) sums v over the wave
+// and the first lane writes it (each (tile-wave, parameter) once per call).
+struct TileOut {
+    LaneRows rows;                       // row = 8 bytes: slot j at voff + 8 j
+    bool lead;
+    __device__ void add(int j, double v) const {
+        const double sum = wave_sum(v);
+        if (lead) rows.st(j, sum);
+    }
+};
+
 // forward(r, alive, c, tape); backward(r, ad, c, tape, acc, ok) -> ok, with
 // tape the lane's rows of a.tape (n_surf * kTape rows of G lanes) and acc the
 // wave's accumulators in a.gacc (n_par rows of G / 64 waves).  The ray loop is
@@ -866,6 +886,88 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs& a, Fwd forward, Bwd 
         St ad = {bx + b * r.X, by + b * r.Y, 0.0, 0.0, 0.0, 0.0};
         const WaveAcc acc{lane_rows(a.gacc, n_par, W, wave), lw == 0};
         const bool ok = backward(r, ad, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane), acc, alive);
+        if (alive && !ok) ++skipped;
+    }
+    lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
+    lds_add(&s_stats[RT_ST_ADJ_SKIP], skipped);
+    __syncthreads();
+    if (a.stats) {
+        for (int k = threadIdx.x; k < RT_STATS_LEN; k += blockDim.x)
+            if (s_stats[k]) atomicAdd(&a.stats[k], (unsigned long long)s_stats[k]);
+    }
+}
+
+// The batched adjoint: tiles of one instance per workgroup (as the batched
+// trace), each lane one ray of the tile; the backward's gradient sums of each
+// tile-wave go to a.gacc[(tile * waves_per_group + wave) * n_par + j] and are
+// reduced per instance afterwards.  Every wave runs the backward (lanes
+// without a live ray masked), so every slot is written.
+template <int BEAM, class Fwd, class Bwd>
+__device__ __forceinline__ void adjoint_batch_body(const AdjArgs& a, Fwd forward, Bwd backward) {
+    extern __shared__ double s_pool[];
+    __shared__ unsigned s_stats[RT_STATS_LEN];
+    for (int i = threadIdx.x; i < RT_STATS_LEN; i += blockDim.x) s_stats[i] = 0u;
+    for (int i = threadIdx.x; i < a.pool_len; i += blockDim.x) s_pool[i] = a.pool[i];
+    __syncthreads();
+
+    TraceCtx c;
+    c.pool = s_pool;
+    c.st = s_stats;
+    c.hist = nullptr;
+    c.flux = nullptr;
+    c.do_hist = c.do_flux = false;
+    c.n_oob = c.n_hit = c.n_off = c.n_edge = 0;
+    c.n_prev = 0.0;
+    c.last_it = 0;
+    const uint64_t G = (uint64_t)gridDim.x * blockDim.x;      // tape lanes
+    const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t lw = threadIdx.x % 64, wpg = blockDim.x / 64, wib = threadIdx.x / 64;
+    const uint64_t n_par = a.n_par > 0 ? (uint64_t)a.n_par : 1;
+    unsigned skipped = 0;
+
+    for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+        // instance of tile t (uniform).  The batch fields are read through the
+        // kernel argument segment
+        // (AdjArgs is the kernel's only argument, at offset 0), re-derived
+        // here, so that they stay scalar.
+        typedef const __attribute__((address_space(4))) AdjArgs* cadj;
+        cadj ka = (cadj)__builtin_amdgcn_kernarg_segment_ptr();
+        __asm__ volatile("" : "+s"(ka));
+        const __attribute__((address_space(4))) uint64_t* ts =
+            (const __attribute__((address_space(4))) uint64_t*)ka->tile_start;
+        const uint64_t inst = (uint32_t)__builtin_amdgcn_readfirstlane(
+            (int)((const __attribute__((address_space(4))) uint32_t*)ka->tile_inst)[t]);
+        const __attribute__((address_space(4))) uint64_t* rs =
+            (const __attribute__((address_space(4))) uint64_t*)ka->ray_start;
+        const uint64_t n_i = rs[inst + 1] - rs[inst];
+        const uint64_t r = (t - ts[inst]) * blockDim.x + threadIdx.x;
+        const bool in = r < n_i;
+        // 32-bit offset (the host keeps n_inst x n_surf records below 2^32
+        // doubles); the pointer depends on the tile, so no record load is
+        // hoisted out of the tile loop
+        cptr tbl = (cptr)ka->table + (uint32_t)inst * (uint32_t)ka->inst_stride;
+        c.table = tbl;
+        Ray ray;
+        cptr bm = (cptr)ka;
+        gen_ray<BEAM>(bm, ka->seed + inst, r, ray);
+        bool alive = in;
+        forward(ray, alive, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane));
+        unsigned long long good = __ballot(alive), all = __ballot(in);
+        if (lw == 0) {
+            if (all) atomicAdd(&s_stats[RT_ST_RAYS], (unsigned)__popcll(all));
+            if (good) atomicAdd(&s_stats[RT_ST_GOOD], (unsigned)__popcll(good));
+        }
+        const uint64_t g = r % ka->n_groups;
+        const double* sd = ka->seeds + (inst * ka->n_groups + g) * 3;
+        const double bx = sd[0], by = sd[1], b = sd[2];
+        St ad = {bx + b * ray.X, by + b * ray.Y, 0.0, 0.0, 0.0, 0.0};
+        TileOut acc;
+        acc.rows.base = a.gacc;
+        acc.rows.bytes = (int)(a.n_tiles * wpg * n_par * sizeof(double));
+        acc.rows.voff = (uint32_t)((t * wpg + wib) * n_par * sizeof(double));
+        acc.rows.row = (uint32_t)sizeof(double);
+        acc.lead = lw == 0;
+        const bool ok = backward(ray, ad, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane), acc, alive);
         if (alive && !ok) ++skipped;
     }
     lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);

@@ -42,8 +42,37 @@ __global__ __launch_bounds__(kAdjBlock) void rt_adjoint_kernel(AdjArgs a) {
             for (int s = 0; s < a.n_surf; ++s)
                 rt::adj::fwd_step(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, s, r, alive, c, t, s * kTape);
         },
-        [&](const Ray& r, rt::adj::St& ad, rt::TraceCtx& c, const LaneRows& t, const rt::adj::WaveAcc& g,
-            bool ok) {
+        [&](const Ray& r, rt::adj::St& ad, rt::TraceCtx& c, const LaneRows& t, const auto& g, bool ok) {
+            for (int s = a.n_surf - 1; s >= 0; --s) {
+                // the row table is uniform: read through the constant address space
+                const __attribute__((address_space(4))) int* rows =
+                    (const __attribute__((address_space(4))) int*)a.prow + s * rt::adj::PG_N;
+                double gp[rt::adj::PG_N];
+                const bool res =
+                    rt::adj::bwd_step(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, r, ad, c, t, s * kTape, gp);
+                const bool use = ok && res;
+#pragma unroll
+                for (int sl = 0; sl < rt::adj::PG_N; ++sl) {
+                    const int j = rows[sl];
+                    if (j >= 0) g.add(j, use ? gp[sl] : 0.0);
+                }
+                ok = use;
+            }
+            return ok;
+        });
+}
+
+// the same surface steps over a batch's tiles (adjoint_batch_body)
+template <int BEAM>
+__global__ __launch_bounds__(kAdjBlock) void rt_adjoint_batch_kernel(AdjArgs a) {
+    using rt::adj::LaneRows;
+    rt::adj::adjoint_batch_body<BEAM>(
+        a,
+        [&](Ray& r, bool& alive, rt::TraceCtx& c, const LaneRows& t) {
+            for (int s = 0; s < a.n_surf; ++s)
+                rt::adj::fwd_step(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, s, r, alive, c, t, s * kTape);
+        },
+        [&](const Ray& r, rt::adj::St& ad, rt::TraceCtx& c, const LaneRows& t, const auto& g, bool ok) {
             for (int s = a.n_surf - 1; s >= 0; --s) {
                 // the row table is uniform: read through the constant address space
                 const __attribute__((address_space(4))) int* rows =
@@ -87,6 +116,18 @@ __global__ __launch_bounds__(256) void rt_adjoint_reduce_chunks(const double* ga
     if (threadIdx.x == 0) part[j * kReduceChunks + b] = v;
 }
 
+// grad[i][j] = sum over instance i's tile-waves of part[tw][j] (blockIdx.x =
+// i * n_par + j), fixed order
+__global__ __launch_bounds__(256) void rt_adjoint_batch_reduce(const double* part, const uint64_t* tile_start,
+                                                              uint64_t n_par, uint64_t wpg, double* grad) {
+    const uint64_t ij = blockIdx.x, i = ij / n_par, j = ij - i * n_par;
+    const uint64_t lo = tile_start[i] * wpg, hi = tile_start[i + 1] * wpg;
+    double v = 0.0;
+    for (uint64_t tw = lo + threadIdx.x; tw < hi; tw += blockDim.x) v += part[tw * n_par + j];
+    v = block_sum_256(v);
+    if (threadIdx.x == 0) grad[ij] = v;
+}
+
 __global__ __launch_bounds__(256) void rt_adjoint_reduce_parts(const double* part, double* grad) {
     const uint64_t j = blockIdx.x;
     double v = (int)threadIdx.x < kReduceChunks ? part[j * kReduceChunks + threadIdx.x] : 0.0;
@@ -123,6 +164,65 @@ bool grow(double*& p, size_t& have, size_t want) {
 extern "C" int rt_hip_scheme_device_tables(const rt_hip_scheme* s, const double** table_dev, const double** pool_dev,
                                            int* n_surf, size_t* pool_len, const double** host_table);
 
+namespace {
+
+// Validation shared by the single-scheme and batched adjoints: beam type,
+// plane gratings only, and the row table (the caller's index of each
+// (surface, slot), -1 where not asked for).  rec(s) is surface s's record
+// (public or device layout: the fields read here sit at the same offsets).
+template <class Rec>
+int adjoint_rows(Rec rec, int n_surf, const double* beam_desc, const double* group_seeds, size_t n_groups,
+                 const int* param_surface, const int* param_field, int n_params, double* grad,
+                 std::vector<int>& host) {
+    if (!beam_desc) return rt_set_error(RT_ERR_BAD_VALUE, "the adjoint traces a generated beam: beam_desc is NULL");
+    const int beam_type = (int)beam_desc[RT_B_TYPE];
+    if (beam_type != RT_BEAM_PARALLEL && beam_type != RT_BEAM_CONIC && beam_type != RT_BEAM_GAUSS)
+        return rt_set_error(RT_ERR_BAD_VALUE, "bad beam type");
+    if (!group_seeds || n_groups == 0) return rt_set_error(RT_ERR_BAD_VALUE, "group seeds missing");
+    for (int s = 0; s < n_surf; ++s) {
+        const double* p = rec(s);
+        if ((int)p[RT_S_TYPE] == RT_GRATING && (int)p[RT_S_CLASS] != RT_PLANE)
+            return rt_set_error(RT_ERR_BAD_VALUE, "surface %d: the adjoint covers plane gratings only", s);
+    }
+    if (n_params < 0 || n_params > 4096 || (n_params > 0 && (!param_surface || !param_field || !grad)))
+        return rt_set_error(RT_ERR_BAD_VALUE, "bad parameter list");
+    host.assign((size_t)n_surf * rt::adj::PG_N, -1);
+    for (int j = 0; j < n_params; ++j) {
+        if (param_surface[j] < 0 || param_surface[j] >= n_surf)
+            return rt_set_error(RT_ERR_BAD_VALUE, "parameter %d: surface %d out of range", j, param_surface[j]);
+        int slot = rt::adj::slot_of_field(param_field[j]);
+        if (slot < 0) return rt_set_error(RT_ERR_BAD_VALUE, "parameter %d: field %d is not differentiable", j,
+                                          param_field[j]);
+        if (slot >= rt::adj::PG_A && slot < rt::adj::PG_CR &&
+            slot - rt::adj::PG_A >= (int)rec(param_surface[j])[RT_S_NPOLY])
+            return rt_set_error(RT_ERR_BAD_VALUE, "parameter %d: aspheric term beyond the surface's N", j);
+        int& row = host[(size_t)param_surface[j] * rt::adj::PG_N + slot];
+        if (row >= 0) return rt_set_error(RT_ERR_BAD_VALUE, "parameter %d repeats parameter %d", j, row);
+        row = j;
+    }
+    return RT_OK;
+}
+
+// beam fields + the host-evaluated uniforms of the direction generators
+void fill_beam(AdjArgs& a, const double* beam_desc) {
+    memcpy(a.beam, beam_desc, RT_BEAM_STRIDE * sizeof(double));
+    double ang = a.beam[RT_B_ANGUNIT] != 0.0 ? a.beam[RT_B_ANGPAR] : a.beam[RT_B_ANGPAR] * rt::kPi / 180.0;
+    a.beam[rt::BU_CONE_COS] = cos(ang / 2.0);                           // beam.cu:593-594
+    a.beam[rt::BU_CONE_SIN] = sin(ang / 2.0);
+    a.beam[rt::BU_GAUSS_TG] = tan(a.beam[RT_B_ANGPAR] / 2.35482 * rt::kPi / 180 / 3600);   // beam.cu:1034,1099
+}
+
+// minimum waves per SIMD asked of the generated kernels' register allocation:
+// 3 (168 VGPRs) measured fastest on the echelle's 48 parameters against 2 and
+// 4 and the compiler's own choice; RT_HIP_ADJOINT_WAVES overrides (0: the
+// compiler's choice)
+int adjoint_waves() {
+    const char* w = getenv("RT_HIP_ADJOINT_WAVES");
+    return w ? atoi(w) : 3;
+}
+
+}  // namespace
+
 extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc, uint64_t seed, size_t n_rays,
                               const double* group_seeds, size_t n_groups, const int* param_surface,
                               const int* param_field, int n_params, double* grad, unsigned long long* stats,
@@ -132,33 +232,11 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     size_t pool_len = 0;
     int rc = rt_hip_scheme_device_tables(sch, &table, &pool, &n_surf, &pool_len, &host_table);
     if (rc != RT_OK) return rc;
-    if (!beam_desc) return rt_set_error(RT_ERR_BAD_VALUE, "the adjoint traces a generated beam: beam_desc is NULL");
+    std::vector<int> host;
+    rc = adjoint_rows([&](int s) { return host_table + (size_t)s * RT_SURF_STRIDE; }, n_surf, beam_desc, group_seeds,
+                      n_groups, param_surface, param_field, n_params, grad, host);
+    if (rc != RT_OK) return rc;
     const int beam_type = (int)beam_desc[RT_B_TYPE];
-    if (beam_type != RT_BEAM_PARALLEL && beam_type != RT_BEAM_CONIC && beam_type != RT_BEAM_GAUSS)
-        return rt_set_error(RT_ERR_BAD_VALUE, "bad beam type");
-    if (!group_seeds || n_groups == 0) return rt_set_error(RT_ERR_BAD_VALUE, "group seeds missing");
-    for (int s = 0; s < n_surf; ++s) {
-        const double* p = host_table + (size_t)s * RT_SURF_STRIDE;
-        if ((int)p[RT_S_TYPE] == RT_GRATING && (int)p[RT_S_CLASS] != RT_PLANE)
-            return rt_set_error(RT_ERR_BAD_VALUE, "surface %d: the adjoint covers plane gratings only", s);
-    }
-    if (n_params < 0 || n_params > 4096 || (n_params > 0 && (!param_surface || !param_field || !grad)))
-        return rt_set_error(RT_ERR_BAD_VALUE, "bad parameter list");
-    // row table: the caller's index of each (surface, slot), -1 where not asked for
-    std::vector<int> host((size_t)n_surf * rt::adj::PG_N, -1);
-    for (int j = 0; j < n_params; ++j) {
-        if (param_surface[j] < 0 || param_surface[j] >= n_surf)
-            return rt_set_error(RT_ERR_BAD_VALUE, "parameter %d: surface %d out of range", j, param_surface[j]);
-        int slot = rt::adj::slot_of_field(param_field[j]);
-        if (slot < 0) return rt_set_error(RT_ERR_BAD_VALUE, "parameter %d: field %d is not differentiable", j,
-                                          param_field[j]);
-        if (slot >= rt::adj::PG_A && slot < rt::adj::PG_CR &&
-            slot - rt::adj::PG_A >= (int)host_table[(size_t)param_surface[j] * RT_SURF_STRIDE + RT_S_NPOLY])
-            return rt_set_error(RT_ERR_BAD_VALUE, "parameter %d: aspheric term beyond the surface's N", j);
-        int& row = host[(size_t)param_surface[j] * rt::adj::PG_N + slot];
-        if (row >= 0) return rt_set_error(RT_ERR_BAD_VALUE, "parameter %d repeats parameter %d", j, row);
-        row = j;
-    }
     if (n_rays == 0) {
         if (n_params) HIP_TRY(hipMemsetAsync(grad, 0, n_params * sizeof(double), (hipStream_t)stream), RT_ERR_FAILED);
         return RT_OK;
@@ -168,11 +246,7 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
 
     AdjArgs a;
     memset(&a, 0, sizeof a);
-    memcpy(a.beam, beam_desc, RT_BEAM_STRIDE * sizeof(double));
-    double ang = a.beam[RT_B_ANGUNIT] != 0.0 ? a.beam[RT_B_ANGPAR] : a.beam[RT_B_ANGPAR] * rt::kPi / 180.0;
-    a.beam[rt::BU_CONE_COS] = cos(ang / 2.0);                           // beam.cu:593-594
-    a.beam[rt::BU_CONE_SIN] = sin(ang / 2.0);
-    a.beam[rt::BU_GAUSS_TG] = tan(a.beam[RT_B_ANGPAR] / 2.35482 * rt::kPi / 180 / 3600);   // beam.cu:1034,1099
+    fill_beam(a, beam_desc);
     a.table = table;
     a.pool = pool;
     a.n_surf = n_surf;
@@ -186,15 +260,9 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
 
     // the kernel generated for this structure and parameter list, else the generic one
     hipFunction_t fn = nullptr;
-    if (rt_jit_enabled()) {
-        // minimum waves per SIMD asked of the generated kernel's register
-        // allocation: 3 (168 VGPRs) measured fastest on the echelle's 48
-        // parameters against 2 and 4 and the compiler's own choice;
-        // RT_HIP_ADJOINT_WAVES overrides (0: the compiler's choice)
-        const char* w = getenv("RT_HIP_ADJOINT_WAVES");
-        fn = rt_jit_adjoint_kernel(rt_scheme_signature(sch), beam_type, host, n_surf, kAdjBlock, w ? atoi(w) : 3,
-                                   rt_scheme_device(sch));
-    }
+    if (rt_jit_enabled())
+        fn = rt_jit_adjoint_kernel(rt_scheme_signature(sch), beam_type, host, n_surf, kAdjBlock, adjoint_waves(),
+                                   rt_scheme_device(sch), false);
     auto kern = beam_type == RT_BEAM_PARALLEL ? rt_adjoint_kernel<RT_BEAM_PARALLEL>
               : beam_type == RT_BEAM_CONIC    ? rt_adjoint_kernel<RT_BEAM_CONIC>
                                               : rt_adjoint_kernel<RT_BEAM_GAUSS>;
@@ -234,8 +302,9 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
         B.params_len = host.size();
     }
     hipStream_t st = (hipStream_t)stream;
-    HIP_TRY(hipMemcpyAsync(B.params, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice, st),
-            RT_ERR_FAILED);
+    // the row table is read by the generic kernel only (the generated one has
+    // the rows in its code): a synchronous upload, so no host buffer outlives the call
+    if (!fn) HIP_TRY(hipMemcpy(B.params, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice), RT_ERR_FAILED);
     a.prow = B.params;
     a.tape = B.tape;
     a.gacc = B.gacc;
@@ -254,7 +323,108 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
         hipLaunchKernelGGL(rt_adjoint_reduce_parts, dim3(n_params), dim3(256), 0, st, (const double*)B.part, grad);
         HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     }
-    // the host copy of the parameter list must outlive the asynchronous upload
-    HIP_TRY(hipStreamSynchronize(st), RT_ERR_FAILED);
+    return RT_OK;
+}
+
+extern "C" int rt_hip_batch_adjoint(const rt_hip_batch* b, const double* beam_desc, uint64_t seed,
+                                    const double* group_seeds, size_t n_groups, const int* param_surface,
+                                    const int* param_field, int n_params, double* grad, unsigned long long* stats,
+                                    void* stream) {
+    BatchView v;
+    int rc = rt_batch_view(b, &v);
+    if (rc != RT_OK) return rc;
+    std::vector<int> host;
+    rc = adjoint_rows([&](int s) { return v.dtables + (size_t)s * rt::DEV_STRIDE; }, v.n_surf, beam_desc, group_seeds,
+                      n_groups, param_surface, param_field, n_params, grad, host);
+    if (rc != RT_OK) return rc;
+    if ((uint64_t)v.n_inst * v.n_surf * rt::DEV_STRIDE >= (1ull << 32))
+        return rt_set_error(RT_ERR_BAD_VALUE, "batched adjoint: too many instance records");
+    if (v.block != kAdjBlock)
+        return rt_set_error(RT_ERR_BAD_VALUE, "the batched adjoint takes workgroups of %d lanes (the batch has %d)",
+                            kAdjBlock, v.block);
+    const int beam_type = (int)beam_desc[RT_B_TYPE];
+    rc = rt_ensure_device();
+    if (rc != RT_OK) return rc;
+    if (n_params == 0 || v.n_tiles == 0) {
+        if (n_params) HIP_TRY(hipMemsetAsync(grad, 0, (size_t)v.n_inst * n_params * sizeof(double),
+                                             (hipStream_t)stream), RT_ERR_FAILED);
+        return RT_OK;
+    }
+
+    AdjArgs a;
+    memset(&a, 0, sizeof a);
+    fill_beam(a, beam_desc);
+    a.table = v.tables_dev;
+    a.pool = v.pool_dev;
+    a.n_surf = v.n_surf;
+    a.pool_len = (int)v.pool_len;
+    a.n_par = n_params;
+    a.seed = seed;
+    a.n = v.ray_start[v.n_inst];
+    a.n_groups = n_groups;
+    a.seeds = group_seeds;
+    a.stats = stats;
+    a.tile_start = v.tile_start_dev;
+    a.tile_inst = v.tile_inst_dev;
+    a.ray_start = v.ray_start_dev;
+    a.n_inst = (uint64_t)v.n_inst;
+    a.n_tiles = v.n_tiles;
+    a.inst_stride = (uint64_t)v.n_surf * rt::DEV_STRIDE;
+
+    hipFunction_t fn = nullptr;
+    if (rt_jit_enabled())
+        fn = rt_jit_adjoint_kernel(*v.signature, beam_type, host, v.n_surf, kAdjBlock, adjoint_waves(), v.device,
+                                   true);
+    auto kern = beam_type == RT_BEAM_PARALLEL ? rt_adjoint_batch_kernel<RT_BEAM_PARALLEL>
+              : beam_type == RT_BEAM_CONIC    ? rt_adjoint_batch_kernel<RT_BEAM_CONIC>
+                                              : rt_adjoint_batch_kernel<RT_BEAM_GAUSS>;
+    const size_t lds = v.pool_len * sizeof(double);
+    int resident = 0;
+    hipError_t oe = fn ? hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, kAdjBlock, lds)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, kAdjBlock, lds);
+    if (oe != hipSuccess || resident < 1) resident = 1;
+    uint64_t cap = (uint64_t)rt_device_cus() * resident;
+    const uint64_t lanes_max = 0x7fffffffull / ((uint64_t)v.n_surf * kTape * sizeof(double));
+    if (cap > lanes_max / kAdjBlock) cap = lanes_max / kAdjBlock;
+    const unsigned grid = (unsigned)(v.n_tiles < cap ? v.n_tiles : cap);
+    const uint64_t G = (uint64_t)grid * kAdjBlock, wpg = kAdjBlock / 64;
+    const uint64_t part_len = v.n_tiles * wpg * (uint64_t)n_params;
+    if (part_len * sizeof(double) > 0x7fffffffull)
+        return rt_set_error(RT_ERR_BAD_VALUE, "batched adjoint: %llu tiles x %d parameters exceed one buffer",
+                            (unsigned long long)v.n_tiles, n_params);
+
+    AdjBuffers& B = g_adj;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), RT_ERR_FAILED);
+    if (B.device != dev) {
+        (void)hipFree(B.tape); (void)hipFree(B.gacc); (void)hipFree(B.part); (void)hipFree(B.params);
+        B = AdjBuffers();
+        B.device = dev;
+    }
+    if (!grow(B.tape, B.tape_len, (size_t)v.n_surf * kTape * G) || !grow(B.gacc, B.gacc_len, part_len))
+        return rt_set_error(RT_ERR_BAD_ALLOC, "adjoint buffers");
+    if (B.params_len < host.size()) {
+        (void)hipFree(B.params);
+        B.params = nullptr;
+        B.params_len = 0;
+        HIP_TRY(hipMalloc(&B.params, host.size() * sizeof(int)), RT_ERR_BAD_ALLOC);
+        B.params_len = host.size();
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (!fn) HIP_TRY(hipMemcpy(B.params, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice), RT_ERR_FAILED);
+    a.prow = B.params;
+    a.tape = B.tape;
+    a.gacc = B.gacc;
+    if (fn) {
+        void* params[] = {&a};
+        HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, kAdjBlock, 1, 1, (unsigned)lds, st, params, nullptr),
+                RT_ERR_FAILED);
+    } else {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kAdjBlock), lds, st, a);
+    }
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    hipLaunchKernelGGL(rt_adjoint_batch_reduce, dim3((unsigned)((uint64_t)v.n_inst * n_params)), dim3(256), 0, st,
+                       (const double*)B.gacc, v.tile_start_dev, (uint64_t)n_params, wpg, grad);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     return RT_OK;
 }

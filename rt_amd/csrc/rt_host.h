@@ -30,3 +30,20 @@ void rt_device_record(const double* pub, double* dev);
 struct rt_hip_scheme;
 const std::string& rt_scheme_signature(const rt_hip_scheme* s);
 int rt_scheme_device(const rt_hip_scheme* s);
+
+// What the batched adjoint needs of a batch (device pointers and structure).
+struct rt_hip_batch;
+struct BatchView {
+    int device, n_inst, n_surf, block;
+    const std::string* signature;
+    const double* tables_dev;        // n_inst x n_surf device records
+    const double* pool_dev;
+    size_t pool_len;
+    const uint64_t* ray_start_dev;   // n_inst + 1
+    const uint64_t* tile_start_dev;  // n_inst + 1 (tiles of `block` rays)
+    const uint32_t* tile_inst_dev;   // n_tiles: the instance of each tile
+    uint64_t n_tiles;
+    const double* dtables;           // host copy of the device records
+    const uint64_t* ray_start;       // host copy
+};
+int rt_batch_view(const rt_hip_batch* b, BatchView* v);

@@ -117,7 +117,7 @@ static std::string generate_source(const std::string& sig, int beam_mode, bool d
 // backward step adding exactly the asked-for slots of that surface to the
 // lane's accumulators.
 static std::string generate_adjoint_source(const std::string& sig, int beam_type, const std::vector<int>& rows,
-                                           int n_surf, int block, int waves = 0) {
+                                           int n_surf, int block, int waves = 0, bool batch = false) {
     std::vector<std::string> types;
     size_t pos = 0;
     while (pos < sig.size()) {
@@ -130,14 +130,14 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
        << "extern \"C\" __global__ __launch_bounds__(" << block;
     if (waves > 0) os << ", " << waves;
     os << ") void rt_adjoint_jit(rt::adj::AdjArgs a) {\n"
-       << "  rt::adj::adjoint_body<" << beam_type << ">(a,\n"
+       << "  rt::adj::" << (batch ? "adjoint_batch_body<" : "adjoint_body<") << beam_type << ">(a,\n"
        << "    [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, const rt::adj::LaneRows& t) {\n";
     for (int s = 0; s < n_surf; ++s)
         os << "      rt::adj::fwd_step(" << types[s] << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s
            << ", r, alive, c, t, " << s * rt::adj::kTape << ");\n";
     os << "    },\n"
        << "    [&](const rt::Ray& r, rt::adj::St& ad, rt::TraceCtx& c, const rt::adj::LaneRows& t, "
-          "const rt::adj::WaveAcc& g, bool ok) -> bool {\n";
+          "const auto& g, bool ok) -> bool {\n";
     for (int s = n_surf - 1; s >= 0; --s) {
         os << "      { double gp[rt::adj::PG_N]; const bool use = rt::adj::bwd_step(" << types[s] << "{c.table + " << s
            << " * rt::DEV_STRIDE}, r, ad, c, t, " << s * rt::adj::kTape << ", gp) && ok;";
@@ -215,9 +215,9 @@ hipFunction_t rt_jit_kernel(const std::string& sig, int beam_mode, bool dump, in
 }
 
 hipFunction_t rt_jit_adjoint_kernel(const std::string& sig, int beam_type, const std::vector<int>& rows, int n_surf,
-                                    int block, int waves, int device) {
+                                    int block, int waves, int device, bool batch) {
     std::ostringstream key;
-    key << device << "|adjoint|" << beam_type << "|" << block << "|" << waves << "|";
+    key << device << "|adjoint|" << (batch ? "batch|" : "") << beam_type << "|" << block << "|" << waves << "|";
     for (size_t k = 0; k < rows.size(); ++k)
         if (rows[k] >= 0) key << k << ":" << rows[k] << ",";
     key << "|" << sig;
@@ -225,7 +225,7 @@ hipFunction_t rt_jit_adjoint_kernel(const std::string& sig, int beam_type, const
     Entry& e = g_cache[key.str()];
     if (e.fn) return e.fn;
     if (e.failed) return nullptr;
-    if (!compile(generate_adjoint_source(sig, beam_type, rows, n_surf, block, waves), e, "rt_adjoint_jit")) {
+    if (!compile(generate_adjoint_source(sig, beam_type, rows, n_surf, block, waves, batch), e, "rt_adjoint_jit")) {
         e.failed = true;
         fprintf(stderr, "rt_hip: generated adjoint kernel failed to build, using the generic kernel:\n%s\n",
                 e.log.c_str());
@@ -253,7 +253,7 @@ extern "C" int rt_jit_check_dtable(const double* dtable, int n_surf, int beam_mo
                 if (sl < rt::adj::PG_A || sl >= rt::adj::PG_CR || sl - rt::adj::PG_A < npoly)
                     rows[(size_t)s * rt::adj::PG_N + sl] = j++;
         }
-        src = generate_adjoint_source(rt_jit_signature(dtable, n_surf), beam_mode, rows, n_surf, rt::kBlock);
+        src = generate_adjoint_source(rt_jit_signature(dtable, n_surf), beam_mode, rows, n_surf, rt::kBlock, 0, batch);
     } else {
         src = generate_source(rt_jit_signature(dtable, n_surf), beam_mode, false, rt::kBlock, 0, batch);
     }

@@ -181,6 +181,84 @@ __global__ __launch_bounds__(256) void rt_group_spots_kernel(const double* part,
     }
 }
 
+// Batched spots: instance i's rays are [ray_start[i], ray_start[i+1]) of the
+// concatenated beam and ray r of the instance is in group r mod G.  Chunks of
+// kSpotRays group strides per instance, chunk_start the prefix over
+// instances (one thread, in order).
+__global__ void rt_batch_chunks_kernel(const uint64_t* ray_start, uint64_t n_inst, uint64_t G,
+                                       uint64_t* chunk_start) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint64_t acc = 0;
+    chunk_start[0] = 0;
+    for (uint64_t i = 0; i < n_inst; ++i) {
+        const uint64_t n = ray_start[i + 1] - ray_start[i];
+        const uint64_t strides = (n + G - 1) / G;
+        acc += strides > 0 ? (strides + kSpotRays - 1) / kSpotRays : 1;
+        chunk_start[i + 1] = acc;
+    }
+}
+
+__global__ void rt_batch_moments_kernel(const double* X, const double* Y, const signed char* F,
+                                        const uint64_t* ray_start, const uint64_t* chunk_start, uint64_t n_inst,
+                                        uint64_t G, double* part) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t total = chunk_start[n_inst];
+    if (idx >= total * G) return;
+    const uint64_t c = idx / G, g = idx - c * G;
+    uint64_t lo = 0, hi = n_inst;
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (chunk_start[mid] <= c) lo = mid; else hi = mid;
+    }
+    const uint64_t r0 = ray_start[lo], n = ray_start[lo + 1] - r0, lc = c - chunk_start[lo];
+    Mom m{0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kSpotRays; ++k) {
+        const uint64_t r = (lc * kSpotRays + k) * G + g;
+        if (r < n && F[r0 + r] == 1) {
+            const double x = X[r0 + r], y = Y[r0 + r];
+            m.n += 1.0;
+            const double dx = x - m.mx, dy = y - m.my;
+            m.mx += dx / m.n;
+            m.my += dy / m.n;
+            m.qx += dx * (x - m.mx);
+            m.qy += dy * (y - m.my);
+        }
+    }
+    double* o = part + idx * 5;
+    o[0] = m.n; o[1] = m.mx; o[2] = m.my; o[3] = m.qx; o[4] = m.qy;
+}
+
+// one workgroup per (instance, group) = blockIdx.x; the same merge order as
+// rt_group_spots_kernel over the instance's chunks
+__global__ __launch_bounds__(256) void rt_batch_spots_groups_kernel(const double* part, const uint64_t* chunk_start,
+                                                                    uint64_t G, double* spots) {
+    __shared__ Mom s[256];
+    const uint64_t ig = blockIdx.x, i = ig / G, g = ig - i * G;
+    const uint64_t c0 = chunk_start[i], nc = chunk_start[i + 1] - c0;
+    Mom m{0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint64_t b = threadIdx.x; b < nc; b += 256) {
+        const double* o = part + ((c0 + b) * G + g) * 5;
+        m = mom_merge(m, Mom{o[0], o[1], o[2], o[3], o[4]});
+    }
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) s[threadIdx.x] = mom_merge(s[threadIdx.x], s[threadIdx.x + k]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const Mom r = s[0];
+        double* out = spots + ig * RT_SPOT_STRIDE;
+        out[0] = r.n;
+        out[1] = r.mx;
+        out[2] = r.my;
+        out[3] = r.n > 0.0 ? r.qx / r.n : 0.0;
+        out[4] = r.n > 0.0 ? r.qy / r.n : 0.0;
+        out[5] = r.n > 0.0 ? sqrt((r.qx + r.qy) / r.n) : 0.0;
+    }
+}
+
 // The spot loss of the adjoint and its seeds, one workgroup: per group with
 // live rays, rms + w ((cx - tx)^2 + (cy - ty)^2); seeds (ax, ay, b) with
 // dloss/dX_r = ax + b X_r for the group's rays: b = 1 / (n rms),
@@ -735,6 +813,7 @@ struct rt_hip_batch {
     int tiles_block = 0;             // workgroup size the tile arrays below are built for
     uint64_t n_tiles = 0;
     uint64_t* tile_start_dev = nullptr;
+    uint32_t* tile_inst_dev = nullptr;   // instance of each tile
     double* moments_dev = nullptr;
 };
 
@@ -750,6 +829,7 @@ void free_batch(rt_hip_batch* b) {
     (void)hipFree(b->tables_dev);
     (void)hipFree(b->pool_dev);
     (void)hipFree(b->ray_start_dev);
+    (void)hipFree(b->tile_inst_dev);
     (void)hipFree(b->tile_start_dev);
     (void)hipFree(b->moments_dev);
     delete b;
@@ -808,11 +888,19 @@ int prepare_tiles(rt_hip_batch* b) {
         ts[i + 1] = ts[i] + (n + b->block - 1) / b->block;
     }
     (void)hipFree(b->tile_start_dev);
+    (void)hipFree(b->tile_inst_dev);
     (void)hipFree(b->moments_dev);
     b->tile_start_dev = nullptr;
+    b->tile_inst_dev = nullptr;
     b->moments_dev = nullptr;
     b->tiles_block = 0;
     b->n_tiles = ts[b->n_inst];
+    std::vector<uint32_t> ti(b->n_tiles ? b->n_tiles : 1, 0);
+    for (int i = 0; i < b->n_inst; ++i)
+        for (uint64_t t = ts[i]; t < ts[i + 1]; ++t) ti[t] = (uint32_t)i;
+    HIP_TRY(hipMalloc(&b->tile_inst_dev, ti.size() * sizeof(uint32_t)), RT_ERR_BAD_ALLOC);
+    HIP_TRY(hipMemcpy(b->tile_inst_dev, ti.data(), ti.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+            RT_ERR_FAILED);
     HIP_TRY(hipMalloc(&b->tile_start_dev, ts.size() * sizeof(uint64_t)), RT_ERR_BAD_ALLOC);
     HIP_TRY(hipMalloc(&b->moments_dev, (b->n_tiles ? b->n_tiles : 1) * 5 * sizeof(double)), RT_ERR_BAD_ALLOC);
     HIP_TRY(hipMemcpy(b->tile_start_dev, ts.data(), ts.size() * sizeof(uint64_t), hipMemcpyHostToDevice),
@@ -1052,6 +1140,26 @@ extern "C" int rt_hip_graph_destroy(rt_hip_graph* g) {
 }
 
 // device tables of a scheme, for the other translation units of the library
+int rt_batch_view(const rt_hip_batch* b, BatchView* v) {
+    if (!b) return set_error(RT_ERR_BAD_VALUE, "batch is NULL");
+    if (b->tiles_block != b->block) return set_error(RT_ERR_FAILED, "batch tiles not prepared");
+    v->device = b->device;
+    v->n_inst = b->n_inst;
+    v->n_surf = b->n_surf;
+    v->block = b->block;
+    v->signature = &b->signature;
+    v->tables_dev = b->tables_dev;
+    v->pool_dev = b->pool_dev;
+    v->pool_len = b->pool_len;
+    v->ray_start_dev = b->ray_start_dev;
+    v->tile_start_dev = b->tile_start_dev;
+    v->tile_inst_dev = b->tile_inst_dev;
+    v->n_tiles = b->n_tiles;
+    v->dtables = b->dtables.data();
+    v->ray_start = b->ray_start.data();
+    return RT_OK;
+}
+
 const std::string& rt_scheme_signature(const rt_hip_scheme* s) { return s->signature; }
 int rt_scheme_device(const rt_hip_scheme* s) { return s->device; }
 
@@ -1110,6 +1218,56 @@ extern "C" int rt_hip_spot_loss(const double* spots, size_t n_groups, const doub
     if (rc != RT_OK) return rc;
     hipLaunchKernelGGL(rt_spot_loss_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, spots, (uint64_t)n_groups,
                        target, weight, seeds, loss);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    return RT_OK;
+}
+
+// Spots per (instance, group) of a batch's stored rays (rt_hip_batch_trace
+// with RT_TRACE_STORE): spots receives n_inst x n_groups x RT_SPOT_STRIDE.
+extern "C" int rt_hip_batch_group_spots(const rt_hip_batch* b, const double* X, const double* Y,
+                                        const signed char* flag, size_t n_groups, double* spots, void* stream) {
+    if (!b) return set_error(RT_ERR_BAD_VALUE, "batch is NULL");
+    if (!spots || n_groups == 0) return set_error(RT_ERR_BAD_VALUE, "spots NULL or no groups");
+    const uint64_t total_rays = b->ray_start[b->n_inst];
+    if (total_rays > 0 && (!X || !Y || !flag)) return set_error(RT_ERR_BAD_VALUE, "NULL ray array");
+    int rc = ensure_device();
+    if (rc != RT_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const uint64_t G = n_groups;
+    uint64_t chunks = 0;                 // the same count the device prefix forms
+    for (int i = 0; i < b->n_inst; ++i) {
+        const uint64_t n = b->ray_start[i + 1] - b->ray_start[i], strides = (n + G - 1) / G;
+        chunks += strides > 0 ? (strides + kSpotRays - 1) / kSpotRays : 1;
+    }
+    struct Buf { double* part = nullptr; size_t cap = 0; uint64_t* cs = nullptr; size_t cs_cap = 0; int device = -1; };
+    static thread_local Buf buf;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), RT_ERR_FAILED);
+    if (buf.device != dev) buf = Buf();
+    if (buf.cap < chunks * G * 5) {
+        if (buf.part) (void)hipFree(buf.part);
+        buf.part = nullptr;
+        buf.cap = 0;
+        HIP_TRY(hipMalloc(&buf.part, chunks * G * 5 * sizeof(double)), RT_ERR_BAD_ALLOC);
+        buf.cap = chunks * G * 5;
+    }
+    if (buf.cs_cap < (size_t)b->n_inst + 1) {
+        if (buf.cs) (void)hipFree(buf.cs);
+        buf.cs = nullptr;
+        buf.cs_cap = 0;
+        HIP_TRY(hipMalloc(&buf.cs, ((size_t)b->n_inst + 1) * sizeof(uint64_t)), RT_ERR_BAD_ALLOC);
+        buf.cs_cap = (size_t)b->n_inst + 1;
+    }
+    buf.device = dev;
+    hipLaunchKernelGGL(rt_batch_chunks_kernel, dim3(1), dim3(64), 0, st, (const uint64_t*)b->ray_start_dev,
+                       (uint64_t)b->n_inst, G, buf.cs);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    const uint64_t threads = chunks * G;
+    hipLaunchKernelGGL(rt_batch_moments_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, X, Y, flag,
+                       (const uint64_t*)b->ray_start_dev, (const uint64_t*)buf.cs, (uint64_t)b->n_inst, G, buf.part);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    hipLaunchKernelGGL(rt_batch_spots_groups_kernel, dim3((unsigned)(b->n_inst * G)), dim3(256), 0, st,
+                       (const double*)buf.part, (const uint64_t*)buf.cs, G, spots);
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     return RT_OK;
 }

@@ -59,6 +59,9 @@ def lib():
                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, _vp, _vp, _vp]
         L.rt_hip_group_spots.argtypes = [_vp, _vp, _vp, C.c_size_t, C.c_size_t, _vp, _vp]
         L.rt_hip_spot_loss.argtypes = [_vp, C.c_size_t, _vp, C.c_double, _vp, _vp, _vp]
+        L.rt_hip_batch_group_spots.argtypes = [_vp, _vp, _vp, _vp, C.c_size_t, _vp, _vp]
+        L.rt_hip_batch_adjoint.argtypes = [_vp, _dp, C.c_uint64, _vp, C.c_size_t, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int), C.c_int, _vp, _vp, _vp]
         L.rt_hip_graph_launch.argtypes = [_vp, _vp]
         L.rt_hip_graph_destroy.argtypes = [_vp]
         L.rt_hip_jit_check.argtypes = [_dp, C.c_int, _dp, C.c_size_t, C.c_int, C.c_char_p, C.c_size_t,
@@ -336,6 +339,41 @@ class DeviceBatch:
         self._beam = b
         check(lib().rt_hip_batch_trace(self.handle, _dptr(b), seed, flags, *ptrs, _tp(spots), _tp(stats), stream),
               "rt_hip_batch_trace")
+
+    def group_spots(self, rays, flag, n_groups: int, stream=None):
+        """rt_hip_batch_group_spots: (n_inst, n_groups, SPOT_STRIDE) float64 CUDA
+        tensor from the stored rays (7, total) of a batched trace."""
+        import torch
+        assert rays.shape == (7, self.total) and rays.is_contiguous() and flag.dtype == torch.int8
+        out = torch.empty((self.n_inst, n_groups, SPOT_STRIDE), dtype=torch.float64, device=rays.device)
+        st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        check(lib().rt_hip_batch_group_spots(self.handle, C.c_void_p(rays[0].data_ptr()),
+                                             C.c_void_p(rays[1].data_ptr()), C.c_void_p(flag.data_ptr()), n_groups,
+                                             C.c_void_p(out.data_ptr()), C.c_void_p(st)), "rt_hip_batch_group_spots")
+        return out
+
+    def adjoint(self, group_seeds, param_surface, param_field, *, seed=0, grad=None, stats=None, beam=None,
+                stream=None):
+        """rt_hip_batch_adjoint: per-instance gradients (n_inst, n_params) for
+        the loss seeded per (instance, group) by group_seeds (CUDA float64,
+        n_inst x n_groups x 3)."""
+        import torch
+        b = np.ascontiguousarray((beam if beam is not None else self.instances[0].beam).pack())
+        ps = np.ascontiguousarray(param_surface, dtype=np.int32)
+        pf = np.ascontiguousarray(param_field, dtype=np.int32)
+        assert len(ps) == len(pf)
+        assert group_seeds.dtype == torch.float64 and group_seeds.is_cuda and group_seeds.is_contiguous()
+        assert group_seeds.shape[0] == self.n_inst and group_seeds.shape[2] == 3
+        if grad is None:
+            grad = torch.zeros((self.n_inst, len(ps)), dtype=torch.float64, device=group_seeds.device)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        ip = C.POINTER(C.c_int)
+        self._adj_keep = (b, ps, pf)
+        check(lib().rt_hip_batch_adjoint(self.handle, _dptr(b), seed, group_seeds.data_ptr(), group_seeds.shape[1],
+                                         ps.ctypes.data_as(ip), pf.ctypes.data_as(ip), len(ps), _tp(grad),
+                                         _tp(stats), stream), "rt_hip_batch_adjoint")
+        return grad
 
     def graph(self, *, stream, seed=0, flags=0, rays=None, flag=None, spots=None, stats=None, beam=None):
         """Record the batched trace as a hipGraph on `stream` (a torch.cuda.Stream)."""
