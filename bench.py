@@ -63,6 +63,9 @@ def main():
                     help="batched adjoint/forward ratio over N_INST tolerance instances of the echelle")
     ap.add_argument("--max-rays", type=int, default=1_000_000,
                     help="largest instance of the batched modes (log-uniform from 1e4)")
+    ap.add_argument("--design", type=int, default=0, metavar="N_INST",
+                    help="batched design loop: gradient descent on the compensators of N_INST instances")
+    ap.add_argument("--iters", type=int, default=50, help="iterations of the design loop")
     ap.add_argument("--adjoint", action="store_true",
                     help="measure the adjoint instead: gradient of the spot loss over the echelle's 48 design "
                          "parameters at 1e6 rays over 64 wavelengths, as a time ratio to the plain forward trace")
@@ -94,6 +97,8 @@ def main():
         return bench_adjoint(args)
     if args.batch_adjoint:
         return bench_batch_adjoint(args)
+    if args.design:
+        return bench_design(args)
     sch = schemes.echelle(args.n_per_lambda, args.nlambda)
     n = sch.beam.n_rays
     ds = lib.DeviceScheme(sch)
@@ -358,6 +363,100 @@ def bench_batch_adjoint(args):
                    "ray_surface_per_step": ray_surf,
                    "forward_ray_surface_per_s": ray_surf / (t_plain * 1e-3),
                    "finite_gradients": bool(np.all(np.isfinite(g)))}}))
+
+
+def bench_design(args):
+    """Batched and differentiable design loop: N_INST tolerance instances of the
+    echelle (64 wavelengths, ray counts log-uniform in [1e4, --max-rays]); one
+    iteration = batched trace with stored rays, spots per instance and
+    wavelength, the config-4 loss and its seeds, the batched adjoint over the
+    48 design parameters plus the compensators, and a gradient-descent step on
+    the compensators (CCD focus and tilts about Y and X) on the device tables;
+    the iteration is captured once in a hipGraph and replayed --iters times.
+    Reported: wall time per iteration against 256 (N_INST) sequential plain
+    single-scheme forwards of the same instances, and the loss before and
+    after."""
+    import torch
+    from rt_amd import diff, lib, schemes
+    from rt_amd import scheme as S
+    G = 64
+    base = schemes.echelle(n_per_lambda=4883, nlambda=G)
+    inst, counts = schemes.tolerance_instances(base, args.design, seed=2, min_rays=10_000, max_rays=args.max_rays)
+    spec = diff.echelle_design_params(base)
+    ps = [p.surface for p in spec.params]
+    pf = [p.field for p in spec.params]
+    ccd = base.n_surf - 1
+    for f in (S.S_DIST + 2, S.S_ANG + 0, S.S_ANG + 1):
+        if (ccd, f) not in set(zip(ps, pf)):
+            ps.append(ccd)
+            pf.append(f)
+    ps, pf = np.array(ps, dtype=np.int32), np.array(pf, dtype=np.int32)
+    comp = [j for j, (s, f) in enumerate(zip(ps, pf)) if s == ccd and f in (S.S_DIST + 2, S.S_ANG, S.S_ANG + 1)]
+    n, total = len(inst), int(sum(counts))
+    db = lib.DeviceBatch(inst, counts)
+    b = dict(rays=torch.zeros((7, total), dtype=torch.float64, device="cuda"),
+             flag=torch.zeros(total, dtype=torch.int8, device="cuda"),
+             spots=torch.zeros((n, G, lib.SPOT_STRIDE), dtype=torch.float64, device="cuda"),
+             seeds=torch.zeros((n * G, 3), dtype=torch.float64, device="cuda"),
+             loss=torch.zeros(1, dtype=torch.float64, device="cuda"),
+             grad=torch.zeros((n, len(ps)), dtype=torch.float64, device="cuda"))
+    lr = torch.zeros(len(ps), dtype=torch.float64, device="cuda")
+
+    def step():
+        db.trace(seed=9, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, rays=b["rays"], flag=b["flag"])
+        db.group_spots(b["rays"], b["flag"], G, out=b["spots"])
+        lib.spot_loss(b["spots"].view(n * G, lib.SPOT_STRIDE), seeds=b["seeds"], loss=b["loss"])
+        db.adjoint(b["seeds"].view(n, G, 3), ps, pf, seed=9, grad=b["grad"])
+        db.descend(ps, pf, b["grad"], lr)
+
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step()                                  # lr = 0: compiles, allocates, measures the gradient scale
+    torch.cuda.synchronize()
+    loss0 = float(b["loss"].item())
+    # step sizes: 2 um of focus and 2 arcsec of tilt for the largest gradient of the batch
+    g = b["grad"].abs().max(dim=0).values.cpu().numpy()
+    lr_h = np.zeros(len(ps))
+    for j in comp:
+        step_size = 2e-3 if pf[j] == S.S_DIST + 2 else 2.0 / 3600.0 * S.DEG2RAD
+        lr_h[j] = step_size / max(g[j], 1e-300)
+    lr.copy_(torch.tensor(lr_h))
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s, capture_error_mode="thread_local"):
+        step()
+    torch.cuda.synchronize()
+    losses = []
+    t0 = time.perf_counter()
+    for _ in range(args.iters):
+        graph.replay()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # loss after the loop (one more replay computes it on the final tables)
+    graph.replay()
+    torch.cuda.synchronize()
+    loss_end = float(b["loss"].item())
+    # 256 (n) sequential plain single-scheme forwards of the same instances
+    dss = [lib.DeviceScheme(sch) for sch in inst]
+    stream = torch.cuda.current_stream()
+    for ds, c in zip(dss, counts):
+        ds.trace(None, None, seed=9, n=int(c), flags=S.RT_TRACE_GENERATE)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for ds, c in zip(dss, counts):
+        ds.trace(None, None, seed=9, n=int(c), flags=S.RT_TRACE_GENERATE, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    seq = time.perf_counter() - t1
+    per_iter = wall / args.iters
+    print(json.dumps({
+        "metric": f"design-loop iteration time / {n} sequential plain forwards (batched forward + adjoint + step)",
+        "value": per_iter / seq, "unit": "x", "n_gpus": 1, "steps": args.iters, "warmup": 1,
+        "higher_is_better": False, "dtype": "f64", "data": "synthetic (Philox beams, seed + instance)",
+        "config": {"workload": "echelle-tolerance-design-loop", "instances": n, "rays": total,
+                   "min_rays": int(min(counts)), "max_rays": int(max(counts)), "wavelengths": G,
+                   "parameters": len(ps), "compensators": len(comp), "iterations": args.iters},
+        "detail": {"iteration_ms": per_iter * 1e3, "sequential_forwards_ms": seq * 1e3,
+                   "loss_first": loss0, "loss_last": loss_end}}))
 
 
 if __name__ == "__main__":

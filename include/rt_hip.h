@@ -379,6 +379,19 @@ int rt_hip_batch_adjoint(const rt_hip_batch *batch, const double *beam_desc, uin
 int rt_hip_batch_group_spots(const rt_hip_batch *batch, const double *X, const double *Y,
                              const signed char *flag, size_t n_groups, double *spots, void *stream);
 
+/* One gradient-descent step on the device tables of every instance of a
+   batch: field -= lr[j] * grad[i][j] for the parameters (param_surface,
+   param_field) (grad: device, n_inst x n_params, as rt_hip_batch_adjoint
+   returns it; lr: device, n_params), with the derived record constants and
+   the sin/cos of stepped tilts re-formed on the device.  Steppable fields:
+   those of the adjoint; a tilt or decentre must be non-zero on that surface
+   in some instance of the batch.  The host copy of the tables is left as it
+   was (rt_hip_batch_update replaces both).  After its first call with a
+   parameter list it only launches a kernel (capturable in a hipGraph).  No
+   counterpart in the reference. */
+int rt_hip_batch_descend(rt_hip_batch *batch, const int *param_surface, const int *param_field, int n_params,
+                         const double *grad, const double *lr, void *stream);
+
 /* "generated" or "generic": the kernel the calling thread's last trace used. */
 int rt_hip_last_kernel(char *buf, size_t len);
 

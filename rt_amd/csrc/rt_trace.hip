@@ -185,16 +185,32 @@ __global__ __launch_bounds__(256) void rt_group_spots_kernel(const double* part,
 // concatenated beam and ray r of the instance is in group r mod G.  Chunks of
 // kSpotRays group strides per instance, chunk_start the prefix over
 // instances (one thread, in order).
-__global__ void rt_batch_chunks_kernel(const uint64_t* ray_start, uint64_t n_inst, uint64_t G,
-                                       uint64_t* chunk_start) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    uint64_t acc = 0;
-    chunk_start[0] = 0;
-    for (uint64_t i = 0; i < n_inst; ++i) {
-        const uint64_t n = ray_start[i + 1] - ray_start[i];
-        const uint64_t strides = (n + G - 1) / G;
-        acc += strides > 0 ? (strides + kSpotRays - 1) / kSpotRays : 1;
-        chunk_start[i + 1] = acc;
+__global__ __launch_bounds__(1024) void rt_batch_chunks_kernel(const uint64_t* ray_start, uint64_t n_inst,
+                                                               uint64_t G, uint64_t* chunk_start) {
+    // one workgroup: chunk counts of 1024 instances at a time, an inclusive
+    // scan in LDS (Hillis-Steele), the running total carried between passes
+    __shared__ uint64_t sc[1024];
+    uint64_t carry = 0;
+    if (threadIdx.x == 0) chunk_start[0] = 0;
+    for (uint64_t base = 0; base < n_inst; base += 1024) {
+        const uint64_t i = base + threadIdx.x;
+        uint64_t v = 0;
+        if (i < n_inst) {
+            const uint64_t n = ray_start[i + 1] - ray_start[i];
+            const uint64_t strides = (n + G - 1) / G;
+            v = strides > 0 ? (strides + kSpotRays - 1) / kSpotRays : 1;
+        }
+        sc[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const uint64_t add = (int)threadIdx.x >= off ? sc[threadIdx.x - off] : 0;
+            __syncthreads();
+            sc[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < n_inst) chunk_start[i + 1] = carry + sc[threadIdx.x];
+        carry += sc[1023];
+        __syncthreads();
     }
 }
 
@@ -257,6 +273,34 @@ __global__ __launch_bounds__(256) void rt_batch_spots_groups_kernel(const double
         out[4] = r.n > 0.0 ? r.qy / r.n : 0.0;
         out[5] = r.n > 0.0 ? sqrt((r.qx + r.qy) / r.n) : 0.0;
     }
+}
+
+// One gradient-descent step on the device records of a batch: thread (i, s)
+// applies p[field_k] -= lr[j_k] grad[i][j_k] for the parameters of surface s
+// (map: per surface a start in a list of (j, field) pairs), then re-derives
+// the record's host-derived constants (the formulas of rt_device_record) and
+// the sin/cos of every updated angle.
+__global__ void rt_batch_descend_kernel(double* tables, uint64_t n_inst, int n_surf, const int* map_start,
+                                        const int* map, const double* grad, const double* lr, int n_par) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_inst * (uint64_t)n_surf) return;
+    const uint64_t i = idx / n_surf;
+    const int s = (int)(idx - i * n_surf);
+    const int k0 = map_start[s], k1 = map_start[s + 1];
+    if (k0 == k1) return;
+    double* p = tables + idx * rt::DEV_STRIDE;
+    for (int k = k0; k < k1; ++k) {
+        const int j = map[2 * k], f = map[2 * k + 1];
+        p[f] -= lr[j] * grad[i * n_par + j];
+        if (f >= RT_S_ANG && f < RT_S_ANG + 3) {
+            p[RT_S_SIN + f - RT_S_ANG] = sin(p[f]);
+            p[RT_S_COS + f - RT_S_ANG] = cos(p[f]);
+        }
+    }
+    p[rt::D_INV_C] = p[RT_S_C] != 0.0 ? 1.0 / p[RT_S_C] : 0.0;
+    p[rt::D_INV_D] = p[RT_S_GRCONST] != 0.0 ? 1.0 / p[RT_S_GRCONST] : 0.0;
+    p[rt::D_K1C2] = (p[RT_S_K] + 1) * p[RT_S_C] * p[RT_S_C];
+    for (int a = 1; a <= RT_MAX_POLY; ++a) p[rt::D_IA + a - 1] = (double)a * p[RT_S_A + a - 1];
 }
 
 // The spot loss of the adjoint and its seeds, one workgroup: per group with
@@ -815,6 +859,9 @@ struct rt_hip_batch {
     uint64_t* tile_start_dev = nullptr;
     uint32_t* tile_inst_dev = nullptr;   // instance of each tile
     double* moments_dev = nullptr;
+    // descent map (rt_hip_batch_descend), kept until the parameter list changes
+    std::vector<int> descend_key;
+    int* descend_dev = nullptr;      // map_start[n_surf + 1] then the (j, field) pairs
 };
 
 struct rt_hip_graph {
@@ -830,6 +877,7 @@ void free_batch(rt_hip_batch* b) {
     (void)hipFree(b->pool_dev);
     (void)hipFree(b->ray_start_dev);
     (void)hipFree(b->tile_inst_dev);
+    (void)hipFree(b->descend_dev);
     (void)hipFree(b->tile_start_dev);
     (void)hipFree(b->moments_dev);
     delete b;
@@ -1259,7 +1307,7 @@ extern "C" int rt_hip_batch_group_spots(const rt_hip_batch* b, const double* X, 
         buf.cs_cap = (size_t)b->n_inst + 1;
     }
     buf.device = dev;
-    hipLaunchKernelGGL(rt_batch_chunks_kernel, dim3(1), dim3(64), 0, st, (const uint64_t*)b->ray_start_dev,
+    hipLaunchKernelGGL(rt_batch_chunks_kernel, dim3(1), dim3(1024), 0, st, (const uint64_t*)b->ray_start_dev,
                        (uint64_t)b->n_inst, G, buf.cs);
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     const uint64_t threads = chunks * G;
@@ -1268,6 +1316,78 @@ extern "C" int rt_hip_batch_group_spots(const rt_hip_batch* b, const double* X, 
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     hipLaunchKernelGGL(rt_batch_spots_groups_kernel, dim3((unsigned)(b->n_inst * G)), dim3(256), 0, st,
                        (const double*)buf.part, (const uint64_t*)buf.cs, G, spots);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    return RT_OK;
+}
+
+// One gradient-descent step on the device tables of every instance of a batch
+// for the parameters (param_surface, param_field): field -= lr[j] * grad[i][j]
+// (grad: device, n_inst x n_params; lr: device, n_params).  Differentiable
+// fields of the adjoint (curvature, conic constant, aspheric terms, toric cR,
+// decentres, tilts, grating constant); a tilt or decentre must be non-zero on
+// that surface in some instance (the generated kernels apply only the
+// rotations and translations the batch had when it was created).  The host
+// copy of the tables is not updated.  No allocation after the first call with
+// a given parameter list, and kernels only: capturable in a hipGraph.
+extern "C" int rt_hip_batch_descend(rt_hip_batch* b, const int* param_surface, const int* param_field, int n_params,
+                                    const double* grad, const double* lr, void* stream) {
+    if (!b) return set_error(RT_ERR_BAD_VALUE, "batch is NULL");
+    if (n_params <= 0) return RT_OK;
+    if (!param_surface || !param_field || !grad || !lr) return set_error(RT_ERR_BAD_VALUE, "NULL argument");
+    std::vector<int> key(param_surface, param_surface + n_params);
+    key.insert(key.end(), param_field, param_field + n_params);
+    if (key != b->descend_key) {
+        // structure flags of each surface from the signature (rotation mask, translation)
+        std::vector<std::string> sig;
+        size_t pos = 0;
+        while (pos < b->signature.size()) {
+            size_t end = b->signature.find(';', pos);
+            sig.push_back(b->signature.substr(pos, end - pos));
+            pos = end + 1;
+        }
+        std::vector<std::vector<std::pair<int, int>>> per(b->n_surf);
+        for (int j = 0; j < n_params; ++j) {
+            const int sf = param_surface[j], f = param_field[j];
+            if (sf < 0 || sf >= b->n_surf) return set_error(RT_ERR_BAD_VALUE, "parameter %d: bad surface", j);
+            const bool ok_field = f == RT_S_C || f == RT_S_K || (f >= RT_S_A && f < RT_S_A + RT_MAX_POLY) ||
+                                  f == RT_S_CR || (f >= RT_S_DIST && f < RT_S_DIST + 3) ||
+                                  (f >= RT_S_ANG && f < RT_S_ANG + 3) || f == RT_S_GRCONST;
+            if (!ok_field) return set_error(RT_ERR_BAD_VALUE, "parameter %d: field %d cannot be stepped", j, f);
+            // StatSurf<cls,type,shape,npoly,grt,ordm,ik,fnmask,rotmask,trans,ccd>
+            std::vector<std::string> tp;
+            std::string body = sig[sf].substr(sig[sf].find('<') + 1);
+            size_t q = 0;
+            while (q < body.size()) {
+                size_t e = body.find_first_of(",>", q);
+                tp.push_back(body.substr(q, e - q));
+                q = e + 1;
+                if (body[e] == '>') break;
+            }
+            const int rotmask = atoi(tp[8].c_str());
+            const bool trans = tp[9] == "true";
+            if (f >= RT_S_ANG && f < RT_S_ANG + 3 && !((rotmask >> (f - RT_S_ANG)) & 1))
+                return set_error(RT_ERR_BAD_VALUE, "parameter %d: surface %d has no rotation %d in the batch", j, sf,
+                                 f - RT_S_ANG);
+            if (f >= RT_S_DIST && f < RT_S_DIST + 3 && !trans)
+                return set_error(RT_ERR_BAD_VALUE, "parameter %d: surface %d has no translation in the batch", j, sf);
+            per[sf].push_back({j, f});
+        }
+        std::vector<int> host(b->n_surf + 1, 0);
+        for (int s = 0; s < b->n_surf; ++s) host[s + 1] = host[s] + (int)per[s].size();
+        for (int s = 0; s < b->n_surf; ++s)
+            for (auto& jf : per[s]) { host.push_back(jf.first); host.push_back(jf.second); }
+        (void)hipFree(b->descend_dev);
+        b->descend_dev = nullptr;
+        b->descend_key.clear();
+        HIP_TRY(hipMalloc(&b->descend_dev, host.size() * sizeof(int)), RT_ERR_BAD_ALLOC);
+        HIP_TRY(hipMemcpy(b->descend_dev, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice),
+                RT_ERR_FAILED);
+        b->descend_key = key;
+    }
+    const uint64_t threads = (uint64_t)b->n_inst * b->n_surf;
+    hipLaunchKernelGGL(rt_batch_descend_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, b->tables_dev, (uint64_t)b->n_inst, b->n_surf,
+                       (const int*)b->descend_dev, (const int*)b->descend_dev + b->n_surf + 1, grad, lr, n_params);
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     return RT_OK;
 }

@@ -60,6 +60,7 @@ def lib():
         L.rt_hip_group_spots.argtypes = [_vp, _vp, _vp, C.c_size_t, C.c_size_t, _vp, _vp]
         L.rt_hip_spot_loss.argtypes = [_vp, C.c_size_t, _vp, C.c_double, _vp, _vp, _vp]
         L.rt_hip_batch_group_spots.argtypes = [_vp, _vp, _vp, _vp, C.c_size_t, _vp, _vp]
+        L.rt_hip_batch_descend.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, _vp, _vp, _vp]
         L.rt_hip_batch_adjoint.argtypes = [_vp, _dp, C.c_uint64, _vp, C.c_size_t, C.POINTER(C.c_int),
                                            C.POINTER(C.c_int), C.c_int, _vp, _vp, _vp]
         L.rt_hip_graph_launch.argtypes = [_vp, _vp]
@@ -241,7 +242,7 @@ def init(device: int = 0):
 SPOT_STRIDE = 6     # RT_SPOT_STRIDE: n, centroid x, y, variance x, y, RMS radius
 
 
-def group_spots(rays, flag, n_groups: int, stream=None):
+def group_spots(rays, flag, n_groups: int, stream=None, out=None):
     """rt_hip_group_spots: (n_groups, SPOT_STRIDE) float64 CUDA tensor of the
     spots of the rays with flag 1, ray r in group r mod n_groups (rays: the
     7 x n state on the device, flag: n int8)."""
@@ -249,7 +250,8 @@ def group_spots(rays, flag, n_groups: int, stream=None):
     n = rays.shape[1]
     if rays.dtype != torch.float64 or not rays.is_contiguous() or flag.dtype != torch.int8:
         raise EngineError("rays must be contiguous float64 and flag int8")
-    out = torch.empty((n_groups, SPOT_STRIDE), dtype=torch.float64, device=rays.device)
+    if out is None:
+        out = torch.empty((n_groups, SPOT_STRIDE), dtype=torch.float64, device=rays.device)
     st = torch.cuda.current_stream().cuda_stream if stream is None else stream
     check(lib().rt_hip_group_spots(C.c_void_p(rays[0].data_ptr()), C.c_void_p(rays[1].data_ptr()),
                                    C.c_void_p(flag.data_ptr()), n, n_groups, C.c_void_p(out.data_ptr()),
@@ -257,13 +259,15 @@ def group_spots(rays, flag, n_groups: int, stream=None):
     return out
 
 
-def spot_loss(spots, target=None, weight: float = 0.0, stream=None):
+def spot_loss(spots, target=None, weight: float = 0.0, stream=None, seeds=None, loss=None):
     """rt_hip_spot_loss: (loss, seeds) as CUDA tensors (a 1-element tensor and
     n_groups x 3) from group_spots' output; target: n_groups x 2 or None."""
     import torch
     G = spots.shape[0]
-    seeds = torch.empty((G, 3), dtype=torch.float64, device=spots.device)
-    loss = torch.empty(1, dtype=torch.float64, device=spots.device)
+    if seeds is None:
+        seeds = torch.empty((G, 3), dtype=torch.float64, device=spots.device)
+    if loss is None:
+        loss = torch.empty(1, dtype=torch.float64, device=spots.device)
     st = torch.cuda.current_stream().cuda_stream if stream is None else stream
     tp = None if target is None else C.c_void_p(target.data_ptr())
     check(lib().rt_hip_spot_loss(C.c_void_p(spots.data_ptr()), G, tp, float(weight), C.c_void_p(seeds.data_ptr()),
@@ -340,12 +344,13 @@ class DeviceBatch:
         check(lib().rt_hip_batch_trace(self.handle, _dptr(b), seed, flags, *ptrs, _tp(spots), _tp(stats), stream),
               "rt_hip_batch_trace")
 
-    def group_spots(self, rays, flag, n_groups: int, stream=None):
+    def group_spots(self, rays, flag, n_groups: int, stream=None, out=None):
         """rt_hip_batch_group_spots: (n_inst, n_groups, SPOT_STRIDE) float64 CUDA
         tensor from the stored rays (7, total) of a batched trace."""
         import torch
         assert rays.shape == (7, self.total) and rays.is_contiguous() and flag.dtype == torch.int8
-        out = torch.empty((self.n_inst, n_groups, SPOT_STRIDE), dtype=torch.float64, device=rays.device)
+        if out is None:
+            out = torch.empty((self.n_inst, n_groups, SPOT_STRIDE), dtype=torch.float64, device=rays.device)
         st = torch.cuda.current_stream().cuda_stream if stream is None else stream
         check(lib().rt_hip_batch_group_spots(self.handle, C.c_void_p(rays[0].data_ptr()),
                                              C.c_void_p(rays[1].data_ptr()), C.c_void_p(flag.data_ptr()), n_groups,
@@ -374,6 +379,19 @@ class DeviceBatch:
                                          ps.ctypes.data_as(ip), pf.ctypes.data_as(ip), len(ps), _tp(grad),
                                          _tp(stats), stream), "rt_hip_batch_adjoint")
         return grad
+
+    def descend(self, param_surface, param_field, grad, lr, stream=None):
+        """rt_hip_batch_descend: field -= lr[j] * grad[i][j] on the device tables
+        (grad: CUDA float64 n_inst x n_params, lr: CUDA float64 n_params)."""
+        import torch
+        ps = np.ascontiguousarray(param_surface, dtype=np.int32)
+        pf = np.ascontiguousarray(param_field, dtype=np.int32)
+        st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        ip = C.POINTER(C.c_int)
+        self._desc_keep = (ps, pf)
+        check(lib().rt_hip_batch_descend(self.handle, ps.ctypes.data_as(ip), pf.ctypes.data_as(ip), len(ps),
+                                         C.c_void_p(grad.data_ptr()), C.c_void_p(lr.data_ptr()), C.c_void_p(st)),
+              "rt_hip_batch_descend")
 
     def graph(self, *, stream, seed=0, flags=0, rays=None, flag=None, spots=None, stats=None, beam=None):
         """Record the batched trace as a hipGraph on `stream` (a torch.cuda.Stream)."""

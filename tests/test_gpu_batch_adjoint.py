@@ -78,3 +78,124 @@ def test_batch_adjoint_matches_single_scheme_adjoints(hip):
         scale = float(np.max(np.abs(gB[i])))
         assert np.all(np.abs(gG[i] - gB[i]) <= 1e-11 * scale), i
     db.close()
+
+
+def _design_params(base):
+    """the 48 design parameters plus the compensators (CCD focus and tilts)"""
+    spec = diff.echelle_design_params(base)
+    ps = [p.surface for p in spec.params]
+    pf = [p.field for p in spec.params]
+    ccd = base.n_surf - 1
+    for f in (S.S_DIST + 2, S.S_ANG + 0, S.S_ANG + 1):
+        if (ccd, f) not in set(zip(ps, pf)):
+            ps.append(ccd)
+            pf.append(f)
+    return np.array(ps, dtype=np.int32), np.array(pf, dtype=np.int32)
+
+
+def test_batch_descend_matches_host_update(hip):
+    """rt_hip_batch_descend against the same step applied to the packed
+    tables on the host (sin/cos of stepped tilts from the host libm): the
+    traces of the two batches agree to the parity bar."""
+    print("[descend]", flush=True)
+    base, inst, counts, _, _, db = setup_batch(hip, n_inst=3, seed=9)
+    ps, pf = _design_params(base)
+    rng = np.random.default_rng(4)
+    grad = rng.normal(0.0, 1.0, (len(inst), len(ps)))
+    lr = np.zeros(len(ps))
+    ccd = base.n_surf - 1
+    for j, (s, f) in enumerate(zip(ps, pf)):
+        if s == ccd:
+            lr[j] = 1e-3 if f == S.S_DIST + 2 else 2e-5
+    lr[0] = 1e-9                                   # one curvature too
+    db.descend(ps, pf, torch.tensor(grad, device="cuda"), torch.tensor(lr, device="cuda"))
+    host = []
+    for i, sch in enumerate(inst):
+        t = np.array(sch.table, dtype=np.float64).copy()
+        for j, (s, f) in enumerate(zip(ps, pf)):
+            t[s * S.SURF_STRIDE + f] -= lr[j] * grad[i, j]
+            if S.S_ANG <= f < S.S_ANG + 3:
+                a = t[s * S.SURF_STRIDE + f]
+                t[s * S.SURF_STRIDE + S.S_SIN + f - S.S_ANG] = np.sin(a)
+                t[s * S.SURF_STRIDE + S.S_COS + f - S.S_ANG] = np.cos(a)
+        c = S.Scheme(list(sch.surfaces), sch.beam, name=sch.name)
+        c.table = t
+        host.append(c)
+    dh = hip.DeviceBatch(host, counts)
+    total = int(sum(counts))
+    out = []
+    for b in (db, dh):
+        rays = torch.zeros((7, total), dtype=torch.float64, device="cuda")
+        flag = torch.zeros(total, dtype=torch.int8, device="cuda")
+        b.trace(seed=3, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, rays=rays, flag=flag)
+        out.append((rays.cpu().numpy(), flag.cpu().numpy()))
+    (ra, fa), (rb, fb) = out
+    assert np.array_equal(fa, fb)
+    ok = fa == 1
+    L = 1000.0
+    assert np.max(np.abs(ra[:3, ok] - rb[:3, ok])) <= 1e-12 * L
+    assert np.max(np.abs(ra[3:6, ok] - rb[3:6, ok])) <= 1e-12
+    db.close()
+    dh.close()
+
+
+def test_design_step_graph_replays_match_eager(hip):
+    """One design-loop step (batched trace with stored rays, spots, loss and
+    seeds, batched adjoint, descent on the compensators) captured in a graph:
+    three replays give bit for bit the losses and tables of three eager steps
+    on an identical batch."""
+    print("[design graph]", flush=True)
+
+    def make():
+        base, inst, counts, _, _, db = setup_batch(hip, n_inst=3, seed=13)
+        return base, inst, counts, db
+
+    base, inst, counts, db_e = make()
+    _, _, _, db_g = make()
+    ps, pf = _design_params(base)
+    ccd = base.n_surf - 1
+    lr = torch.tensor([1e-4 if (s == ccd and f == S.S_DIST + 2) else (1e-6 if s == ccd and f in (S.S_ANG, S.S_ANG + 1)
+                                                                      else 0.0) for s, f in zip(ps, pf)],
+                      dtype=torch.float64, device="cuda")
+    total = int(sum(counts))
+    n = len(inst)
+
+    def buffers():
+        return dict(rays=torch.zeros((7, total), dtype=torch.float64, device="cuda"),
+                    flag=torch.zeros(total, dtype=torch.int8, device="cuda"),
+                    spots=torch.zeros((n, G, hip.SPOT_STRIDE), dtype=torch.float64, device="cuda"),
+                    seeds=torch.zeros((n * G, 3), dtype=torch.float64, device="cuda"),
+                    loss=torch.zeros(1, dtype=torch.float64, device="cuda"),
+                    grad=torch.zeros((n, len(ps)), dtype=torch.float64, device="cuda"))
+
+    def step(db, b):
+        db.trace(seed=11, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, rays=b["rays"], flag=b["flag"])
+        db.group_spots(b["rays"], b["flag"], G, out=b["spots"])
+        hip.spot_loss(b["spots"].view(n * G, hip.SPOT_STRIDE), seeds=b["seeds"], loss=b["loss"])
+        db.adjoint(b["seeds"].view(n, G, 3), ps, pf, seed=11, grad=b["grad"])
+        db.descend(ps, pf, b["grad"], lr)
+
+    be, bg = buffers(), buffers()
+    losses_e = []
+    for _ in range(3):
+        step(db_e, be)
+        losses_e.append(float(be["loss"].item()))
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        # the first step runs eagerly (kernels compiled, buffers allocated), then one is captured
+        step(db_g, bg)
+    torch.cuda.synchronize()
+    losses_g = [float(bg["loss"].item())]
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
+        step(db_g, bg)
+    for _ in range(2):
+        g.replay()
+        torch.cuda.synchronize()
+        losses_g.append(float(bg["loss"].item()))
+    assert losses_g == losses_e, (losses_g, losses_e)
+    assert torch.equal(bg["grad"], be["grad"])
+    assert torch.equal(bg["rays"], be["rays"])
+    db_e.close()
+    db_g.close()
