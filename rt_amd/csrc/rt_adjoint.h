@@ -799,35 +799,154 @@ RT_UNROLL
     return surface_adjoint(d, entry, nit, mu, L, ad, gp);
 }
 
-// Sum over the 64 lanes of the wave, butterfly order (lane 0's value is the
-// same on every run).  Every lane must be active.
-__device__ __forceinline__ double wave_sum(double v) {
-RT_UNROLL
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+// A double moved between lanes by a DPP pattern inside rows of 16 lanes
+// (every source lane active).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// x and y exchanged by v_permlane16_swap (H = 16: odd rows of 16 lanes of x
+// with even rows of y) or v_permlane32_swap (H = 32: upper half of x with
+// lower half of y), the two results added: lanes of row pair (2k, 2k+1), or of
+// the two halves, get x's rows summed in the even row (half) and y's rows
+// summed in the odd one, the lower row's value first.
+template <int H>
+__device__ __forceinline__ double swap_sum(double x, double y) {
+    const uint64_t ux = __builtin_bit_cast(uint64_t, x), uy = __builtin_bit_cast(uint64_t, y);
+    uint32_t l0, l1, h0, h1;
+    if (H == 16) {
+        const auto a = __builtin_amdgcn_permlane16_swap((uint32_t)ux, (uint32_t)uy, false, false);
+        const auto b = __builtin_amdgcn_permlane16_swap((uint32_t)(ux >> 32), (uint32_t)(uy >> 32), false, false);
+        l0 = a[0]; l1 = a[1]; h0 = b[0]; h1 = b[1];
+    } else {
+        const auto a = __builtin_amdgcn_permlane32_swap((uint32_t)ux, (uint32_t)uy, false, false);
+        const auto b = __builtin_amdgcn_permlane32_swap((uint32_t)(ux >> 32), (uint32_t)(uy >> 32), false, false);
+        l0 = a[0]; l1 = a[1]; h0 = b[0]; h1 = b[1];
+    }
+    return __builtin_bit_cast(double, ((uint64_t)h0 << 32) | l0) +
+           __builtin_bit_cast(double, ((uint64_t)h1 << 32) | l1);
+}
+
+// sum over the 16 lanes of each row (pairs, quads, 8, 16), the same value in
+// every lane of the row
+__device__ __forceinline__ double row_sum(double v) {
+    v += dpp_f64<0xB1>(v);                   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);                   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);                  // row_half_mirror
+    v += dpp_f64<0x140>(v);                  // row_mirror
     return v;
 }
 
-// Per-wave gradient accumulators: row j holds one double per wave (row size
-// W * 8 bytes); add(j, v) sums v over the wave and the wave's first lane adds
-// it to its own slot (one writer per slot: no race, fixed order).
+// Sum over the 64 lanes of the wave in a fixed tree (pairs, quads, 8, 16
+// lanes by DPP, then rows by lane swaps), the same value in every lane and on
+// every run; VALU only (no LDS permutes).  Every lane must be active.
+__device__ __forceinline__ double wave_sum(double v) {
+    v = row_sum(v);
+    v = swap_sum<16>(v, v);
+    return swap_sum<32>(v, v);
+}
+
+// Gradient accumulators of the backward lambdas: add(j, v) adds the wave's
+// sum of v to parameter slot j.  The ray loops call begin/end once per lane
+// (adjoint_body) or tile_begin/tile_end per tile (adjoint_batch_body).
+//
+// WaveAcc: per-wave sums in memory, a.gacc[j * W + wave] (W waves), the
+// wave's first lane adding to its own slot (one writer per slot: no race,
+// fixed order); slot numbers may be run-time values.
 struct WaveAcc {
     LaneRows rows;
     bool lead;
+    __device__ void begin(const AdjArgs& a, uint64_t wave, uint64_t W, uint64_t lw) {
+        const uint64_t n_par = a.n_par > 0 ? (uint64_t)a.n_par : 1;
+        rows = lane_rows(a.gacc, n_par, W, wave);
+        lead = lw == 0;
+        if (lead)
+            for (uint64_t j = 0; j < (uint64_t)a.n_par; ++j) a.gacc[j * W + wave] = 0.0;
+    }
+    __device__ void end(const AdjArgs&, uint64_t, uint64_t, uint64_t) const {}
     __device__ void add(int j, double v) const {
         const double sum = wave_sum(v);
         if (lead) rows.st(j, rows.ld(j) + sum);
     }
 };
 
-// Gradient partials of one tile-wave of a batch: row tw of n_par doubles
-// (tw = tile * waves per workgroup + wave); add(j, v) sums v over the wave
-// and the first lane writes it (each (tile-wave, parameter) once per call).
+// TileOut: the sums of one tile-wave of a batch written to row tw of n_par
+// doubles, a.gacc[tw * n_par + j] (tw = tile * waves per workgroup + wave),
+// each slot once per tile; slot numbers may be run-time values.
 struct TileOut {
     LaneRows rows;                       // row = 8 bytes: slot j at voff + 8 j
     bool lead;
+    __device__ void tile_begin(const AdjArgs& a, uint64_t tw, uint64_t lw, uint64_t n_rows) {
+        const uint64_t n_par = a.n_par > 0 ? (uint64_t)a.n_par : 1;
+        rows.base = a.gacc;
+        rows.bytes = (int)(n_rows * n_par * sizeof(double));
+        rows.voff = (uint32_t)(tw * n_par * sizeof(double));
+        rows.row = (uint32_t)sizeof(double);
+        lead = lw == 0;
+    }
+    __device__ void tile_end(const AdjArgs&, uint64_t, uint64_t) const {}
     __device__ void add(int j, double v) const {
         const double sum = wave_sum(v);
         if (lead) rows.st(j, sum);
+    }
+};
+
+// GroupAcc: the wave's sums held in registers, for compile-time slot groups
+// (the generated kernels).  add4<GRP>(x0, x1, x2, x3) reduces four slots'
+// values over the wave together: two 32-lane swaps and one 16-lane swap
+// transpose them into the four rows of one register (row 0: x0, row 1: x2,
+// row 2: x1, row 3: x3, each summed over 4 lanes), the row sums finish them,
+// and lane 16 r + GRP % 16 of v[GRP / 16] accumulates row r.  Map::slot(k)
+// names the parameter held by lane k % 64 of v[k / 64] (-1: none).  Written
+// once (per lane loop or per tile) to the layouts of WaveAcc and TileOut; no
+// memory traffic and no waits inside the ray loop.
+template <int NR, class Map>
+struct GroupAcc {
+    double v[NR];
+    uint32_t lw;
+    __device__ void zero() {
+RT_UNROLL
+        for (int q = 0; q < NR; ++q) v[q] = 0.0;
+    }
+    __device__ void begin(const AdjArgs&, uint64_t, uint64_t, uint64_t lane_in_wave) {
+        lw = (uint32_t)lane_in_wave;
+        zero();
+    }
+    __device__ void end(const AdjArgs& a, uint64_t wave, uint64_t W, uint64_t) const {
+RT_UNROLL
+        for (int q = 0; q < NR; ++q) {
+            const int j = Map::slot(q * 64 + (int)lw);
+            if (j >= 0) a.gacc[(uint64_t)j * W + wave] = v[q];
+        }
+    }
+    __device__ void tile_begin(const AdjArgs&, uint64_t, uint64_t lane_in_wave, uint64_t) {
+        lw = (uint32_t)lane_in_wave;
+        zero();
+    }
+    __device__ void tile_end(const AdjArgs& a, uint64_t tw, uint64_t) const {
+RT_UNROLL
+        for (int q = 0; q < NR; ++q) {
+            const int j = Map::slot(q * 64 + (int)lw);
+            if (j >= 0) a.gacc[tw * (uint64_t)a.n_par + (uint64_t)j] = v[q];
+        }
+    }
+    template <int GRP>
+    __device__ void add4(double x0, double x1, double x2, double x3) {
+        const double t = row_sum(swap_sum<16>(swap_sum<32>(x0, x1), swap_sum<32>(x2, x3)));
+        // the lane index made opaque at each use (else the compares with the
+        // constant group offsets are hoisted out of the ray loop and held as
+        // 64-bit lane masks across it), and the update kept in program order
+        // (else the sums are sunk towards the loop end and every surface's
+        // partials stay live until there)
+        uint32_t l = lw;
+        __asm__ volatile("" : "+v"(l));
+        double& acc = v[GRP / 16];
+        acc += (l & 15u) == (uint32_t)(GRP % 16) ? t : 0.0;
+        __asm__ volatile("" : "+v"(acc));
     }
 };
 
@@ -838,7 +957,7 @@ struct TileOut {
 // the end run masked), so that the backward steps can reduce across the wave;
 // lanes whose ray is lost or whose replay fails compute on and are masked out
 // of the sums.  The tape size must fit a buffer resource (checked by the host).
-template <int BEAM, class Fwd, class Bwd>
+template <int BEAM, class Acc, class Fwd, class Bwd>
 __device__ __forceinline__ void adjoint_body(const AdjArgs& a, Fwd forward, Bwd backward) {
     extern __shared__ double s_pool[];
     __shared__ unsigned s_stats[RT_STATS_LEN];
@@ -859,9 +978,8 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs& a, Fwd forward, Bwd 
     const uint64_t W = G / 64;                                 // waves
     const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t wave = lane / 64, lw = lane % 64;
-    const uint64_t n_par = a.n_par > 0 ? (uint64_t)a.n_par : 1;
-    if (lw == 0)
-        for (uint64_t j = 0; j < (uint64_t)a.n_par; ++j) a.gacc[j * W + wave] = 0.0;
+    Acc acc;
+    acc.begin(a, wave, W, lw);
     unsigned skipped = 0;
 
     for (uint64_t i0 = wave * 64; i0 < a.n; i0 += G) {
@@ -884,10 +1002,10 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs& a, Fwd forward, Bwd 
         const uint64_t g = i % a.n_groups;
         const double bx = a.seeds[3 * g], by = a.seeds[3 * g + 1], b = a.seeds[3 * g + 2];
         St ad = {bx + b * r.X, by + b * r.Y, 0.0, 0.0, 0.0, 0.0};
-        const WaveAcc acc{lane_rows(a.gacc, n_par, W, wave), lw == 0};
         const bool ok = backward(r, ad, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane), acc, alive);
         if (alive && !ok) ++skipped;
     }
+    acc.end(a, wave, W, lw);
     lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
     lds_add(&s_stats[RT_ST_ADJ_SKIP], skipped);
     __syncthreads();
@@ -902,7 +1020,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs& a, Fwd forward, Bwd 
 // tile-wave go to a.gacc[(tile * waves_per_group + wave) * n_par + j] and are
 // reduced per instance afterwards.  Every wave runs the backward (lanes
 // without a live ray masked), so every slot is written.
-template <int BEAM, class Fwd, class Bwd>
+template <int BEAM, class Acc, class Fwd, class Bwd>
 __device__ __forceinline__ void adjoint_batch_body(const AdjArgs& a, Fwd forward, Bwd backward) {
     extern __shared__ double s_pool[];
     __shared__ unsigned s_stats[RT_STATS_LEN];
@@ -922,7 +1040,7 @@ __device__ __forceinline__ void adjoint_batch_body(const AdjArgs& a, Fwd forward
     const uint64_t G = (uint64_t)gridDim.x * blockDim.x;      // tape lanes
     const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t lw = threadIdx.x % 64, wpg = blockDim.x / 64, wib = threadIdx.x / 64;
-    const uint64_t n_par = a.n_par > 0 ? (uint64_t)a.n_par : 1;
+    Acc acc;
     unsigned skipped = 0;
 
     for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
@@ -961,14 +1079,10 @@ __device__ __forceinline__ void adjoint_batch_body(const AdjArgs& a, Fwd forward
         const double* sd = ka->seeds + (inst * ka->n_groups + g) * 3;
         const double bx = sd[0], by = sd[1], b = sd[2];
         St ad = {bx + b * ray.X, by + b * ray.Y, 0.0, 0.0, 0.0, 0.0};
-        TileOut acc;
-        acc.rows.base = a.gacc;
-        acc.rows.bytes = (int)(a.n_tiles * wpg * n_par * sizeof(double));
-        acc.rows.voff = (uint32_t)((t * wpg + wib) * n_par * sizeof(double));
-        acc.rows.row = (uint32_t)sizeof(double);
-        acc.lead = lw == 0;
+        acc.tile_begin(a, t * wpg + wib, lw, a.n_tiles * wpg);
         const bool ok = backward(ray, ad, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane), acc, alive);
         if (alive && !ok) ++skipped;
+        acc.tile_end(a, t * wpg + wib, lw);
     }
     lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
     lds_add(&s_stats[RT_ST_ADJ_SKIP], skipped);

@@ -36,7 +36,7 @@ constexpr int kReduceChunks = 64;        // first-stage workgroups per parameter
 template <int BEAM>
 __global__ __launch_bounds__(kAdjBlock) void rt_adjoint_kernel(AdjArgs a) {
     using rt::adj::LaneRows;
-    rt::adj::adjoint_body<BEAM>(
+    rt::adj::adjoint_body<BEAM, rt::adj::WaveAcc>(
         a,
         [&](Ray& r, bool& alive, rt::TraceCtx& c, const LaneRows& t) {
             for (int s = 0; s < a.n_surf; ++s)
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kAdjBlock) void rt_adjoint_kernel(AdjArgs a) {
 template <int BEAM>
 __global__ __launch_bounds__(kAdjBlock) void rt_adjoint_batch_kernel(AdjArgs a) {
     using rt::adj::LaneRows;
-    rt::adj::adjoint_batch_body<BEAM>(
+    rt::adj::adjoint_batch_body<BEAM, rt::adj::TileOut>(
         a,
         [&](Ray& r, bool& alive, rt::TraceCtx& c, const LaneRows& t) {
             for (int s = 0; s < a.n_surf; ++s)

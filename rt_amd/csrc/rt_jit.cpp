@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -115,7 +116,7 @@ static std::string generate_source(const std::string& sig, int beam_mode, bool d
 // The adjoint kernel of a scheme structure and parameter list: the forward
 // surface steps and the backward steps as straight-line sequences, each
 // backward step adding exactly the asked-for slots of that surface to the
-// lane's accumulators.
+// wave's register accumulators, in groups of up to four slots.
 static std::string generate_adjoint_source(const std::string& sig, int beam_type, const std::vector<int>& rows,
                                            int n_surf, int block, int waves = 0, bool batch = false) {
     std::vector<std::string> types;
@@ -125,25 +126,63 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
         types.push_back(sig.substr(pos, end - pos));
         pos = end + 1;
     }
+    // gradient sums in registers (rt::adj::GroupAcc): each surface's slots
+    // reduced in groups of up to four; group GRP's inputs x0..x3 end in rows
+    // 0, 2, 1, 3 of register GRP / 16 at lane offset GRP % 16
+    struct Group { int surf; int sl[4]; int par[4]; };
+    std::vector<Group> groups;
+    for (int s = 0; s < n_surf; ++s) {
+        Group g{s, {-1, -1, -1, -1}, {-1, -1, -1, -1}};
+        int k = 0;
+        for (int sl = 0; sl < rt::adj::PG_N; ++sl) {
+            const int j = rows[(size_t)s * rt::adj::PG_N + sl];
+            if (j < 0) continue;
+            g.sl[k] = sl;
+            g.par[k] = j;
+            if (++k == 4) {
+                groups.push_back(g);
+                g = Group{s, {-1, -1, -1, -1}, {-1, -1, -1, -1}};
+                k = 0;
+            }
+        }
+        if (k > 0) groups.push_back(g);
+    }
+    const int n_reg = std::max(1, ((int)groups.size() + 15) / 16);
+    std::vector<int> map((size_t)n_reg * 64, -1);
+    static const int kRowOf[4] = {0, 2, 1, 3};
+    for (size_t gi = 0; gi < groups.size(); ++gi)
+        for (int k = 0; k < 4; ++k)
+            if (groups[gi].par[k] >= 0) map[(gi / 16) * 64 + 16 * kRowOf[k] + gi % 16] = groups[gi].par[k];
     std::ostringstream os;
     os << "#include \"rt_adjoint.h\"\n"
+       << "__constant__ int rt_adj_slot_map[" << map.size() << "] = {";
+    for (size_t k = 0; k < map.size(); ++k) os << (k ? "," : "") << map[k];
+    os << "};\n"
+       << "struct RtAdjMap { __device__ static int slot(int k) { return rt_adj_slot_map[k]; } };\n"
        << "extern \"C\" __global__ __launch_bounds__(" << block;
     if (waves > 0) os << ", " << waves;
     os << ") void rt_adjoint_jit(rt::adj::AdjArgs a) {\n"
-       << "  rt::adj::" << (batch ? "adjoint_batch_body<" : "adjoint_body<") << beam_type << ">(a,\n"
+       << "  rt::adj::" << (batch ? "adjoint_batch_body<" : "adjoint_body<") << beam_type
+       << ", rt::adj::GroupAcc<" << n_reg << ", RtAdjMap>>(a,\n"
        << "    [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, const rt::adj::LaneRows& t) {\n";
     for (int s = 0; s < n_surf; ++s)
         os << "      rt::adj::fwd_step(" << types[s] << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s
            << ", r, alive, c, t, " << s * rt::adj::kTape << ");\n";
     os << "    },\n"
        << "    [&](const rt::Ray& r, rt::adj::St& ad, rt::TraceCtx& c, const rt::adj::LaneRows& t, "
-          "const auto& g, bool ok) -> bool {\n";
+          "auto& g, bool ok) -> bool {\n";
     for (int s = n_surf - 1; s >= 0; --s) {
         os << "      { double gp[rt::adj::PG_N]; const bool use = rt::adj::bwd_step(" << types[s] << "{c.table + " << s
            << " * rt::DEV_STRIDE}, r, ad, c, t, " << s * rt::adj::kTape << ", gp) && ok;";
-        for (int sl = 0; sl < rt::adj::PG_N; ++sl) {
-            const int j = rows[(size_t)s * rt::adj::PG_N + sl];
-            if (j >= 0) os << " g.add(" << j << ", use ? gp[" << sl << "] : 0.0);";
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            if (groups[gi].surf != s) continue;
+            os << " g.template add4<" << gi << ">(";
+            for (int k = 0; k < 4; ++k) {
+                if (k) os << ", ";
+                if (groups[gi].sl[k] >= 0) os << "use ? gp[" << groups[gi].sl[k] << "] : 0.0";
+                else os << "0.0";
+            }
+            os << ");";
         }
         os << " ok = use; }\n";
     }
