@@ -61,8 +61,9 @@ def main():
                          "the echelle, ray counts log-uniform in [1e4, 1e6], one persistent launch per step")
     ap.add_argument("--batch-adjoint", type=int, default=0, metavar="N_INST",
                     help="batched adjoint/forward ratio over N_INST tolerance instances of the echelle")
-    ap.add_argument("--max-rays", type=int, default=1_000_000,
-                    help="largest instance of the batched modes (log-uniform from 1e4)")
+    ap.add_argument("--max-rays", type=int, default=0,
+                    help="largest instance of the batched modes (log-uniform from 1e4); default 1e6, "
+                         "5e5 for --design")
     ap.add_argument("--design", type=int, default=0, metavar="N_INST",
                     help="batched design loop: gradient descent on the compensators of N_INST instances")
     ap.add_argument("--iters", type=int, default=50, help="iterations of the design loop")
@@ -243,7 +244,8 @@ def bench_adjoint(args):
         "warmup": args.warmup, "higher_is_better": False, "dtype": "f64",
         "data": "synthetic (Philox beam)",
         "config": {"workload": "echelle-adjoint", "rays": n, "wavelengths": 64, "parameters": len(spec)},
-        "detail": {"forward_ms": t_plain, "loss_forward_ms": t_loss, "adjoint_ms": t_adj}}))
+        "detail": {"forward_ms": t_plain, "loss_forward_ms": t_loss, "adjoint_ms": t_adj,
+                   "adjoint_kernel_ratio": t_adj / t_plain}}))
 
 
 def bench_batch(args):
@@ -292,7 +294,7 @@ def bench_batch(args):
 
 def bench_batch_adjoint(args):
     """Batched adjoint/forward ratio: N_INST tolerance instances of the echelle
-    (64 wavelengths, ray counts log-uniform in [1e4, --max-rays]), the config-4
+    (64 wavelengths, ray counts log-uniform in [1e4, --max-rays]), the spot
     loss per instance (sum over wavelengths of the RMS spot radius), gradient of
     every instance over the 48 design parameters.  Times per step: the plain
     batched forward (beams generated, nothing stored), the loss forward (rays
@@ -307,7 +309,7 @@ def bench_batch_adjoint(args):
     base = schemes.echelle(n_per_lambda=4883, nlambda=64)
     spec = diff.echelle_design_params(base)
     inst, counts = schemes.tolerance_instances(base, args.batch_adjoint, seed=1, min_rays=10_000,
-                                               max_rays=args.max_rays)
+                                               max_rays=args.max_rays or 1_000_000)
     db = lib.DeviceBatch(inst, counts)
     total = int(sum(counts))
     rays = torch.zeros((7, total), dtype=torch.float64, device="cuda")
@@ -360,7 +362,7 @@ def bench_batch_adjoint(args):
                    "min_rays": int(min(counts)), "max_rays": int(max(counts)), "wavelengths": 64,
                    "parameters": len(ps)},
         "detail": {"plain_forward_ms": t_plain, "loss_forward_ms": t_loss, "adjoint_ms": t_adj,
-                   "ray_surface_per_step": ray_surf,
+                   "adjoint_kernel_ratio": t_adj / t_plain, "ray_surface_per_step": ray_surf,
                    "forward_ray_surface_per_s": ray_surf / (t_plain * 1e-3),
                    "finite_gradients": bool(np.all(np.isfinite(g)))}}))
 
@@ -369,7 +371,7 @@ def bench_design(args):
     """Batched and differentiable design loop: N_INST tolerance instances of the
     echelle (64 wavelengths, ray counts log-uniform in [1e4, --max-rays]); one
     iteration = batched trace with stored rays, spots per instance and
-    wavelength, the config-4 loss and its seeds, the batched adjoint over the
+    wavelength, the spot loss and its seeds, the batched adjoint over the
     48 design parameters plus the compensators, and a gradient-descent step on
     the compensators (CCD focus and tilts about Y and X) on the device tables;
     the iteration is captured once in a hipGraph and replayed --iters times.
@@ -381,7 +383,8 @@ def bench_design(args):
     from rt_amd import scheme as S
     G = 64
     base = schemes.echelle(n_per_lambda=4883, nlambda=G)
-    inst, counts = schemes.tolerance_instances(base, args.design, seed=2, min_rays=10_000, max_rays=args.max_rays)
+    inst, counts = schemes.tolerance_instances(base, args.design, seed=2, min_rays=10_000,
+                                               max_rays=args.max_rays or 500_000)
     spec = diff.echelle_design_params(base)
     ps = [p.surface for p in spec.params]
     pf = [p.field for p in spec.params]
