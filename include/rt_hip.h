@@ -262,6 +262,18 @@ int rt_hip_spline_eval(const double *spline, size_t n, const double *x, double *
    environment variable RT_HIP_JIT=0 sets mode 0 at start-up. */
 int rt_hip_set_jit(int mode);
 
+/* For beams of one wavelength or a composite uniform grid (at most 16384
+   wavelengths), the generated traces read the scheme's spline-valued
+   functions (refractive indices, detector QE and spectrum) from a table of
+   their values at the grid's wavelengths, filled by a small kernel on the
+   trace's stream before each trace, with the same spline code and wavelength
+   formula as the per-ray evaluation (the results are the same bit for bit).
+   Mode 0 evaluates the splines per ray.  Returns the previous mode.  The
+   environment variable RT_HIP_LTAB=0 sets mode 0 at start-up.  The table is
+   per scheme (or batch): traces of one scheme with different wavelength grids
+   must be ordered on one stream.  No counterpart in the reference. */
+int rt_hip_set_ltab(int mode);
+
 /* Launch of a scheme's generated kernel: lanes per workgroup (64..1024,
    powers of two), the minimum number of waves per SIMD the compiler must fit
    its registers to (0: its own choice), and persistent workgroups per CU as a
@@ -270,13 +282,18 @@ int rt_hip_set_jit(int mode);
 int rt_hip_scheme_set_launch(rt_hip_scheme *scheme, int block, int waves, int grid_mult);
 int rt_hip_scheme_get_launch(const rt_hip_scheme *scheme, int *block, int *waves, int *grid_mult);
 
-/* Autotuner: traces n_sample rays of the beam (generated from seed, with
-   `flags`) under each candidate launch (block 128/256, waves 0/5/6/7/8,
-   grid multiple 1/2/4/8/16; register allocations that spill to scratch are
-   skipped when a spill-free one exists), re-times the four fastest in
-   interleaved rounds, keeps the one with the lowest median in the scheme and
-   returns that median in *best_ms.  Allocates its sample buffers; call it once after
-   rt_hip_scheme_create, outside any timed or captured region. */
+/* Autotuner: traces n_sample rays of the beam (generated from seed, seed + 1,
+   ... for successive launches, with `flags`) under each candidate launch
+   (block 128/256, waves 0/5/6/7/8, grid multiple 1/2/4/8/16; those spilling
+   more than 64 bytes per lane to scratch are set aside when others exist) in
+   3 interleaved rounds of 3 back-to-back launches, re-times the four fastest
+   in 5 rounds of 20 (a sustained load runs at a lower clock than isolated
+   launches, by an amount that depends on the launch), keeps the one with the
+   lowest median (within 0.5%, the one with less scratch memory per lane) in
+   the scheme and returns that median (per launch) in *best_ms.
+   RT_HIP_AUTOTUNE_LOG=1 prints every candidate's times on stderr.  Allocates
+   its sample buffers; call it once after rt_hip_scheme_create, outside any
+   timed or captured region. */
 int rt_hip_autotune(rt_hip_scheme *scheme, const double *beam_desc, uint64_t seed, size_t n_sample,
                     int flags, double *best_ms, void *stream);
 
@@ -398,10 +415,12 @@ int rt_hip_last_kernel(char *buf, size_t len);
 /* Generate and compile (without loading, no device needed) the kernel of a
    packed scheme for beam mode (-1: rays given, else RT_BEAM_*, or'ed with
    RT_JIT_BATCH for the batched kernel, or with RT_JIT_ADJOINT for the adjoint
-   kernel with every slot of every surface asked for); `text` receives the generated source
+   kernel with every slot of every surface asked for, or with RT_JIT_LTAB for
+   the trace that reads its spline-valued functions from a per-wavelength
+   table); `text` receives the generated source
    and the compiler log, *code_size the code object size.  RT_OK when it
    compiles. */
-enum { RT_JIT_BATCH = 0x100, RT_JIT_ADJOINT = 0x200 };
+enum { RT_JIT_BATCH = 0x100, RT_JIT_ADJOINT = 0x200, RT_JIT_LTAB = 0x400 };
 int rt_hip_jit_check(const double *surf_table, int n_surf, const double *spline_pool,
                      size_t pool_len, int beam_mode, char *text, size_t len, size_t *code_size);
 

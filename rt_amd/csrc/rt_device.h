@@ -69,6 +69,8 @@ struct DynSurf {
     __device__ bool n1spline() const { return p[RT_S_N1KIND] == (double)RT_FN_SPLINE; }
     __device__ bool n2spline() const { return p[RT_S_N2KIND] == (double)RT_FN_SPLINE; }
     __device__ static constexpr bool n1reuse() { return false; }
+    __device__ bool qespline() const { return p[RT_S_QEKIND] == (double)RT_FN_SPLINE; }
+    __device__ bool spspline() const { return p[RT_S_SPKIND] == (double)RT_FN_SPLINE; }
     __device__ bool trans() const {
         return (p[RT_S_DIST + 2] != 0.0) || (p[RT_S_DIST + 0] != 0.0) || (p[RT_S_DIST + 1] != 0.0);
     }
@@ -97,6 +99,9 @@ struct StatSurf {
     __device__ static constexpr bool n2spline() { return (FNMASK & 2) != 0; }
     // n1 is the n2 table of the previous surface's action: its value is reused
     __device__ static constexpr bool n1reuse() { return (FNMASK & 4) != 0; }
+    // detector QE and spectrum given as tables (set on CCD surfaces only)
+    __device__ static constexpr bool qespline() { return (FNMASK & 8) != 0; }
+    __device__ static constexpr bool spspline() { return (FNMASK & 16) != 0; }
     __device__ static constexpr bool trans() { return TRANS; }
     __device__ static constexpr bool rot(int j) { return ((ROTMASK >> j) & 1) != 0; }
     __device__ static constexpr bool ccd() { return CCD; }
@@ -296,20 +301,42 @@ __device__ __forceinline__ void gen_direction(cptr B, uint64_t seed, uint64_t r,
 // global index), random_range_beam_kernel (beam.cu:801-812).  Products and
 // sums rounded separately (no FMA contraction), so every wavelength is the
 // correctly rounded value of the written expression, bit for bit.
-__device__ __forceinline__ double gen_lambda(cptr B, uint64_t seed, uint64_t r) {
+//
+// grid_lambda: the wavelength of grid index j of a uniform range (j < nl, or
+// any j when nl <= 1); shared by the per-ray generator and the per-wavelength
+// tables, so both see the same value.
+__device__ __forceinline__ double grid_lambda(cptr B, uint64_t nl, uint64_t j) {
+#pragma clang fp contract(off)
+    const double r0 = B[RT_B_RANGE + 0], r1 = B[RT_B_RANGE + 1];
+    if (nl <= 1) return r0;
+    const double d_l = (r1 - r0) / (double)(nl - 1);
+    return r0 + (double)j * d_l;
+}
+
+__device__ __forceinline__ double mono_lambda(cptr B) {
+#pragma clang fp contract(off)
+    return (B[RT_B_RANGE + 1] + B[RT_B_RANGE + 0]) / 2.0;
+}
+
+// the wavelength of ray r and its grid index j (0 for one wavelength; the
+// ray's own draw index for random wavelengths)
+__device__ __forceinline__ double gen_lambda(cptr B, uint64_t seed, uint64_t r, uint64_t& j) {
 #pragma clang fp contract(off)
     int distr = (int)B[RT_B_RANGE_DISTR];
     double r0 = B[RT_B_RANGE + 0], r1 = B[RT_B_RANGE + 1];
     uint64_t nper = (uint64_t)B[RT_B_NPER];
     bool composite = B[RT_B_COMPOSITE] != 0.0;
     uint64_t nl = composite ? (uint64_t)B[RT_B_NLAMBDA] : nper;
-    uint64_t j = composite ? (r % nl) : r;
-    if (distr == RT_RANGE_MONO) return (r1 + r0) / 2.0;
-    if (distr == RT_RANGE_UNIFORM) {
-        if (nl <= 1) return r0;
-        double d_l = (r1 - r0) / (double)(nl - 1);
-        return r0 + (double)j * d_l;
+    // r mod nl in 32 bits where both fit (the 64-bit remainder is a long
+    // instruction sequence)
+    j = r;
+    if (composite) {
+        if (nl <= 1) j = 0;
+        else if (((r | nl) >> 32) == 0) j = (uint32_t)r % (uint32_t)nl;
+        else j = r % nl;
     }
+    if (distr == RT_RANGE_MONO) { j = 0; return mono_lambda(B); }
+    if (distr == RT_RANGE_UNIFORM) return grid_lambda(B, nl, j);
     uint32_t o[4];
     philox_stream(seed, j, 2u, o);
     return (r1 - r0) * u01(o[0], o[1]) + r0;
@@ -317,10 +344,16 @@ __device__ __forceinline__ double gen_lambda(cptr B, uint64_t seed, uint64_t r) 
 
 // B: RT_BEAM_STRIDE beam fields followed by the BeamUniform values
 template <int TYPE>
-__device__ __forceinline__ void gen_ray(cptr B, uint64_t seed, uint64_t r, Ray& ray) {
+__device__ __forceinline__ void gen_ray(cptr B, uint64_t seed, uint64_t r, Ray& ray, uint64_t& j) {
     gen_position(B, seed, r, ray);
     gen_direction<TYPE>(B, seed, r, ray);
-    ray.lam = gen_lambda(B, seed, r);
+    ray.lam = gen_lambda(B, seed, r, j);
+}
+
+template <int TYPE>
+__device__ __forceinline__ void gen_ray(cptr B, uint64_t seed, uint64_t r, Ray& ray) {
+    uint64_t j;
+    gen_ray<TYPE>(B, seed, r, ray, j);
 }
 
 // ------------------------------------------------------------------ splines
@@ -779,6 +812,11 @@ struct TraceArgs {
     const uint64_t* ray_start;
     double* tile_moments;         // per tile: n, mean x, mean y, M2 x, M2 y of the rays alive at the end
     uint64_t n_inst, n_tiles, inst_stride;
+    // grid-wavelength beams: per-wavelength values of the scheme's spline
+    // tables, ltab[q * lt_nk + j] for column q at grid index j (generated
+    // kernels built for it only)
+    const double* ltab;
+    uint64_t lt_nk;
 };
 
 struct TraceCtx {
@@ -791,6 +829,7 @@ struct TraceCtx {
     unsigned n_oob, n_hit, n_off, n_edge;
     double n_prev;                // n2 of the last action (reused where the next n1 is the same table)
     unsigned last_it;             // Newton iterations of the last intersection (adjoint tape)
+    uint32_t lk;                  // wavelength grid index of the current ray
 };
 
 __device__ __forceinline__ void lds_add(unsigned* counter, unsigned v) {
@@ -824,8 +863,15 @@ __device__ __forceinline__ void hist_add_wave(unsigned* hist, uint32_t pix, bool
 // for one ray: transform, normalize, intersection, aperture mask, action
 // (reflection, refraction or diffraction), normalize, then the CCD frame on
 // a detector.  Counters are wave-aggregated into LDS.
+//
+// lv, col: the ray's per-wavelength table values (generated kernels of
+// grid-wavelength beams): this surface's spline-valued functions read
+// lv[col], lv[col + 1], ... in the order n1 (unless reused), n2, QE,
+// spectrum, a NaN meaning a wavelength outside the table; col < 0 evaluates
+// the splines here.
 template <class SD>
-__device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& alive, TraceCtx& c) {
+__device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& alive, TraceCtx& c,
+                                             const double* lv = nullptr, int col = -1) {
     // the record's loads are issued at this surface, not hoisted ahead of
     // the previous surfaces' work (that would keep every surface's
     // parameters live in SGPRs at once)
@@ -853,8 +899,18 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
                 reflect_ray(d, r);
             } else if (d.type() == RT_LENS || d.type() == RT_GRATING) {
                 int e1 = 0, e2 = 0;
-                double n1 = d.n1reuse() ? c.n_prev : fn_eval(c.pool, d.n1spline(), p[RT_S_N1VAL], r.lam, e1);
-                double n2 = fn_eval(c.pool, d.n2spline(), p[RT_S_N2VAL], r.lam, e2);
+                double n1, n2;
+                if (col >= 0) {
+                    int q = col;
+                    const bool t1 = !d.n1reuse() && d.n1spline();
+                    n1 = d.n1reuse() ? c.n_prev : (t1 ? lv[q++] : p[RT_S_N1VAL]);
+                    n2 = d.n2spline() ? lv[q] : p[RT_S_N2VAL];
+                    e1 = t1 && n1 != n1;
+                    e2 = d.n2spline() && n2 != n2;
+                } else {
+                    n1 = d.n1reuse() ? c.n_prev : fn_eval(c.pool, d.n1spline(), p[RT_S_N1VAL], r.lam, e1);
+                    n2 = fn_eval(c.pool, d.n2spline(), p[RT_S_N2VAL], r.lam, e2);
+                }
                 c.n_prev = n2;
                 if (e1 | e2) {
                     alive = false;
@@ -900,8 +956,17 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
                 if (c.do_flux) {
                     // compute_ccd_image_kernel (ccd.cu:219-232), weights of ccd.cu:286-287,360
                     int eq = 0, es = 0;
-                    double qe = fn_eval(c.pool, p[RT_S_QEKIND] == (double)RT_FN_SPLINE, p[RT_S_QEVAL], r.lam, eq);
-                    double sp = fn_eval(c.pool, p[RT_S_SPKIND] == (double)RT_FN_SPLINE, p[RT_S_SPVAL], r.lam, es);
+                    double qe, sp;
+                    if (col >= 0) {
+                        int q = col;
+                        qe = d.qespline() ? lv[q++] : p[RT_S_QEVAL];
+                        sp = d.spspline() ? lv[q] : p[RT_S_SPVAL];
+                        eq = d.qespline() && qe != qe;
+                        es = d.spspline() && sp != sp;
+                    } else {
+                        qe = fn_eval(c.pool, d.qespline(), p[RT_S_QEVAL], r.lam, eq);
+                        sp = fn_eval(c.pool, d.spspline(), p[RT_S_SPVAL], r.lam, es);
+                    }
                     if (eq | es) {
                         ++c.n_oob;
                     } else {
@@ -936,6 +1001,46 @@ __device__ __forceinline__ void dump_state(const TraceArgs& a, int s, const Ray&
     d[3 * n + i] = r.cX; d[4 * n + i] = r.cY; d[5 * n + i] = r.cZ;
     d[6 * n + i] = r.lam;
     a.dump_flag[(size_t)s * n + i] = (signed char)alive;
+}
+
+// Column q of the current ray's per-wavelength table values, loaded at the
+// start of the surface that uses it (its latency hidden behind that
+// surface's transform and intersection).  The grid index is clamped into the
+// table.
+template <int NC>
+__device__ __forceinline__ void load_lval(double (&lv)[NC], const TraceCtx& c, int q) {
+    const cargs ka = kernel_args();
+    const uint64_t nk = ka->lt_nk;
+    const uint64_t k = (uint64_t)c.lk < nk ? (uint64_t)c.lk : nk - 1;
+    lv[q] = ka->ltab[(uint64_t)q * nk + k];
+}
+
+// Arguments of the table kernel; the beam first (offset 0 of the argument
+// segment, read through the constant address space as in the traces).
+struct LtabArgs {
+    BeamArg beam;
+    const double* pool;           // device spline pool
+    const double* col_off;        // pool offset of each column's table
+    double* out;                  // n_cols x nk
+    uint64_t nk;
+    int n_cols, pad;
+};
+
+// out[q * nk + j]: column q's table at the wavelength of grid index j of the
+// beam (one wavelength, or a composite uniform grid), evaluated by the same
+// spline code and wavelength formula as the per-ray path; NaN outside the
+// table.
+__device__ __forceinline__ void ltab_body(const LtabArgs& a) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint64_t)a.n_cols * a.nk) return;
+    const uint64_t q = idx / a.nk, j = idx % a.nk;
+    cptr B = (cptr)__builtin_amdgcn_kernarg_segment_ptr();
+    const double lam = (int)B[RT_B_RANGE_DISTR] == RT_RANGE_MONO
+                           ? mono_lambda(B)
+                           : grid_lambda(B, (uint64_t)B[RT_B_NLAMBDA], j);
+    int err = 0;
+    const double v = spline_eval(a.pool + (size_t)a.col_off[q], lam, err);
+    a.out[idx] = err ? __builtin_nan("") : v;
 }
 
 // Kernel body: the spline pool goes to LDS, every lane of a persistent grid
@@ -981,9 +1086,12 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
             __asm__ volatile("" : "+s"(karg));
             cptr bm = (cptr)(karg + __builtin_offsetof(TraceArgs, beam));
             cptr64 sd = (cptr64)(karg + __builtin_offsetof(TraceArgs, seed));
-            gen_ray<BEAM>(bm, *sd, a.ray_begin + i, r);
+            uint64_t j;
+            gen_ray<BEAM>(bm, *sd, a.ray_begin + i, r, j);
+            c.lk = (uint32_t)j;
             alive = true;
         } else {
+            c.lk = 0;
             const cargs ka = kernel_args();
             r.X = ka->X[i]; r.Y = ka->Y[i]; r.Z = ka->Z[i];
             r.cX = ka->cX[i]; r.cY = ka->cY[i]; r.cZ = ka->cZ[i];
@@ -1082,9 +1190,12 @@ __device__ __forceinline__ void trace_batch_body(const TraceArgs& a, Surfaces su
         c.table = tbl;
         Ray ray;
         bool alive = valid;
+        c.lk = 0;
         if (valid) {
             cptr bm = (cptr)((const __attribute__((address_space(4))) char*)ka + __builtin_offsetof(TraceArgs, beam));
-            gen_ray<BEAM>(bm, ka->seed + inst, r, ray);
+            uint64_t j;
+            gen_ray<BEAM>(bm, ka->seed + inst, r, ray, j);
+            c.lk = (uint32_t)j;
         } else {
             ray.X = ray.Y = ray.Z = ray.cX = ray.cY = ray.lam = 0.0;
             ray.cZ = 1.0;

@@ -30,6 +30,7 @@ namespace {
 
 std::mutex g_mu;
 int g_mode = -1;   // -1: not yet read from the environment
+int g_ltab = -1;   // per-wavelength tables; -1: not yet read from the environment
 struct Entry {
     hipModule_t module = nullptr;
     hipFunction_t fn = nullptr;
@@ -58,19 +59,61 @@ std::string rt_jit_signature(const double* dtable, int n_surf) {
         int grt = type == RT_GRATING ? (int)p[RT_S_GRTYPE] : 0;
         int ordm = type == RT_GRATING ? (int)p[RT_S_ORDMODE] : 0;
         bool n1s = p[RT_S_N1KIND] == (double)RT_FN_SPLINE, n2s = p[RT_S_N2KIND] == (double)RT_FN_SPLINE;
+        bool ccd = type == RT_DETECTOR && p[RT_S_CCD_XDIM] > 0.0;
         // bit 4: n1 is the table the previous non-auxiliary surface's action evaluated as
         // its n2, for the same ray and wavelength, so its value is reused
         int fnmask = opt ? ((n1s ? 1 : 0) | (n2s ? 2 : 0) | (n1s && p[RT_S_N1VAL] == prev_n2 ? 4 : 0)) : 0;
+        // bits 8, 16: the detector's QE and spectrum are tables
+        if (ccd)
+            fnmask |= (p[RT_S_QEKIND] == (double)RT_FN_SPLINE ? 8 : 0) | (p[RT_S_SPKIND] == (double)RT_FN_SPLINE ? 16 : 0);
         if (type != RT_AUXTYPE) prev_n2 = (opt && n2s) ? p[RT_S_N2VAL] : -1.0;
         int rot = (p[RT_S_ANG + 0] != 0.0 ? 1 : 0) | (p[RT_S_ANG + 1] != 0.0 ? 2 : 0) |
                   (p[RT_S_ANG + 2] != 0.0 ? 4 : 0);
         bool trans = p[RT_S_DIST + 0] != 0.0 || p[RT_S_DIST + 1] != 0.0 || p[RT_S_DIST + 2] != 0.0;
-        bool ccd = type == RT_DETECTOR && p[RT_S_CCD_XDIM] > 0.0;
         os << "rt::StatSurf<" << (int)p[RT_S_CLASS] << "," << type << "," << (int)p[RT_S_SHAPE] << ","
            << (int)p[RT_S_NPOLY] << "," << grt << "," << ordm << "," << surface_ikind(p) << "," << fnmask
            << "," << rot << "," << (trans ? "true" : "false") << "," << (ccd ? "true" : "false") << ">;";
     }
     return os.str();
+}
+
+// The columns of the per-wavelength table of a scheme with this signature:
+// per surface, in order, n1 (a table not reused from the previous surface),
+// n2, the detector QE, the detector spectrum, for the functions given as
+// tables (StatSurf's FNMASK, the 8th template argument).
+void rt_jit_ltab_layout(const std::string& sig, std::vector<int>& col_base, std::vector<int>& col_surf,
+                        std::vector<int>& col_role) {
+    col_base.clear();
+    col_surf.clear();
+    col_role.clear();
+    size_t pos = 0;
+    int s = 0;
+    while (pos < sig.size()) {
+        const size_t end = sig.find(';', pos);
+        const std::string t = sig.substr(pos, end - pos);
+        const size_t lt = t.find('<');
+        int fnmask = 0;
+        if (lt != std::string::npos) {
+            // the 8th comma-separated template argument
+            size_t q = lt + 1;
+            for (int k = 0; k < 7 && q != std::string::npos; ++k) {
+                q = t.find(',', q);
+                if (q != std::string::npos) ++q;
+            }
+            if (q != std::string::npos) fnmask = atoi(t.c_str() + q);
+        }
+        const bool roles[4] = {(fnmask & 1) != 0 && (fnmask & 4) == 0, (fnmask & 2) != 0, (fnmask & 8) != 0,
+                               (fnmask & 16) != 0};
+        col_base.push_back(-1);
+        for (int r = 0; r < 4; ++r) {
+            if (!roles[r]) continue;
+            if (col_base.back() < 0) col_base.back() = (int)col_surf.size();
+            col_surf.push_back(s);
+            col_role.push_back(r);
+        }
+        pos = end + 1;
+        ++s;
+    }
 }
 
 bool rt_jit_enabled() {
@@ -82,6 +125,22 @@ bool rt_jit_enabled() {
     return g_mode == 1;
 }
 
+bool rt_jit_ltab_enabled() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_ltab < 0) {
+        const char* e = getenv("RT_HIP_LTAB");
+        g_ltab = (e && e[0] == '0') ? 0 : 1;
+    }
+    return g_ltab == 1;
+}
+
+extern "C" int rt_hip_set_ltab(int mode) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int prev = g_ltab < 0 ? 1 : g_ltab;
+    g_ltab = mode ? 1 : 0;
+    return prev;
+}
+
 extern "C" int rt_hip_set_jit(int mode) {
     std::lock_guard<std::mutex> lk(g_mu);
     int prev = g_mode < 0 ? 1 : g_mode;
@@ -91,6 +150,13 @@ extern "C" int rt_hip_set_jit(int mode) {
 
 static std::string generate_source(const std::string& sig, int beam_mode, bool dump, int block, int waves,
                                    bool batch = false) {
+    // RT_JIT_LTAB: the spline-valued functions come from the per-wavelength
+    // table (TraceArgs::ltab), loaded at the start of each ray
+    const bool ltab = beam_mode >= 0 && (beam_mode & RT_JIT_LTAB) != 0;
+    if (beam_mode >= 0) beam_mode &= ~RT_JIT_LTAB;
+    std::vector<int> col_base, col_surf, col_role;
+    rt_jit_ltab_layout(sig, col_base, col_surf, col_role);
+    const bool use_tab = ltab && !col_surf.empty();
     std::ostringstream os;
     os << "#include \"rt_device.h\"\n"
        << "extern \"C\" __global__ __launch_bounds__(" << block;
@@ -98,13 +164,19 @@ static std::string generate_source(const std::string& sig, int beam_mode, bool d
     os << ") void rt_trace_jit(rt::TraceArgs a) {\n"
        << "  rt::" << (batch ? "trace_batch_body<" : "trace_body<") << beam_mode
        << ">(a, [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, uint64_t i) {\n";
+    if (use_tab) os << "    double lv[" << col_surf.size() << "];\n";
     size_t pos = 0;
     int s = 0;
     while (pos < sig.size()) {
         size_t end = sig.find(';', pos);
         std::string type = sig.substr(pos, end - pos);
+        if (use_tab)
+            for (size_t q = 0; q < col_surf.size(); ++q)
+                if (col_surf[q] == s) os << "    rt::load_lval(lv, c, " << q << ");\n";
         os << "    rt::surface_step(" << type << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s
-           << ", r, alive, c);\n";
+           << ", r, alive, c";
+        if (use_tab && col_base[(size_t)s] >= 0) os << ", lv, " << col_base[(size_t)s];
+        os << ");\n";
         if (dump) os << "    rt::dump_state(a, " << s << ", r, alive, i);\n";
         pos = end + 1;
         ++s;
@@ -222,6 +294,17 @@ static bool compile_code(const std::string& src, std::vector<char>& code, std::s
 }
 
 static bool compile(const std::string& src, Entry& e, const char* name = "rt_trace_jit") {
+    // RT_HIP_JIT_SOURCE_DIR: each generated source is also written there
+    // (rt_jit_<n>.hip, in compilation order), for inspection
+    if (const char* dir = getenv("RT_HIP_JIT_SOURCE_DIR")) {
+        static int n_written = 0;
+        char path[4096];
+        snprintf(path, sizeof path, "%s/rt_jit_%03d.hip", dir, n_written++);
+        if (FILE* f = fopen(path, "w")) {
+            fputs(src.c_str(), f);
+            fclose(f);
+        }
+    }
     std::vector<char> code;
     if (!compile_code(src, code, e.log)) return false;
     if (hipModuleLoadData(&e.module, code.data()) != hipSuccess) {
@@ -281,6 +364,7 @@ extern "C" int rt_jit_check_dtable(const double* dtable, int n_surf, int beam_mo
     const bool batch = beam_mode >= 0 && (beam_mode & RT_JIT_BATCH) != 0;
     const bool adjoint = beam_mode >= 0 && (beam_mode & RT_JIT_ADJOINT) != 0;
     if (beam_mode >= 0) beam_mode &= ~(RT_JIT_BATCH | RT_JIT_ADJOINT);
+    if (adjoint) beam_mode &= ~RT_JIT_LTAB;
     std::string src;
     if (adjoint) {
         // every slot a surface has (aspheric terms up to its count) asked for

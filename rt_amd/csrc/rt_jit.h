@@ -9,9 +9,20 @@
 // Structural key of a scheme: one StatSurf type per device record.
 std::string rt_jit_signature(const double* dtable, int n_surf);
 
+// The columns of the per-wavelength table of a scheme with this signature:
+// col_base[s] is surface s's first column or -1; column q holds the function
+// of role col_role[q] (0 n1, 1 n2, 2 detector QE, 3 detector spectrum) of
+// surface col_surf[q].  Kernels generated with RT_JIT_LTAB read that layout.
+void rt_jit_ltab_layout(const std::string& signature, std::vector<int>& col_base, std::vector<int>& col_surf,
+                        std::vector<int>& col_role);
+
 // Whether generated kernels are used (environment RT_HIP_JIT=0 disables;
 // rt_hip_set_jit overrides).
 bool rt_jit_enabled();
+
+// Whether generated traces of grid-wavelength beams read the per-wavelength
+// table (environment RT_HIP_LTAB=0 disables; rt_hip_set_ltab overrides).
+bool rt_jit_ltab_enabled();
 
 // The kernel generated for (signature, beam mode, per-surface state dump or
 // not, lanes per workgroup, minimum waves per SIMD or 0) on `device`,

@@ -115,7 +115,8 @@ __global__ __launch_bounds__(kThreads) void rt_beam_kernel(BeamGenArgs a) {
             else rt::gen_direction<RT_BEAM_PARALLEL>(B, a.seed, g, r);
             a.cX[i] = r.cX; a.cY[i] = r.cY; a.cZ[i] = r.cZ;
         } else {                               // uniform_range_beam / random_range_beam
-            a.L[i] = rt::gen_lambda(B, a.seed, g);
+            uint64_t j;
+            a.L[i] = rt::gen_lambda(B, a.seed, g, j);
         }
     }
 }

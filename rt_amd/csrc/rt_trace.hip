@@ -338,6 +338,26 @@ __global__ __launch_bounds__(256) void rt_spot_loss_kernel(const double* spots, 
 
 // ------------------------------------------------------------------- C ABI
 
+// Per-wavelength table of a scheme's spline-valued functions (refractive
+// indices, detector QE and spectrum), used by the generated traces of
+// grid-wavelength beams: column q (layout of rt_jit_ltab_layout) at grid
+// index j is the spline at pool offset off[q] evaluated at the beam's j-th
+// wavelength.  Filled on the trace's stream before each such trace, so traces
+// of one scheme with different wavelength grids must be stream-ordered.
+struct Ltab {
+    std::vector<int> base, surf, role;
+    double* off_dev = nullptr;       // pool offset of each column
+    double* tab_dev = nullptr;       // n_cols x kLtabMax
+    bool ok = false;                 // layout set up (a column exists, offsets equal in every instance)
+    void release() {
+        (void)hipFree(off_dev);
+        (void)hipFree(tab_dev);
+        off_dev = tab_dev = nullptr;
+        ok = false;
+    }
+};
+constexpr uint64_t kLtabMax = 16384;   // wavelengths of the largest grid served by the table
+
 struct rt_hip_scheme {
     int device;
     std::string signature;           // structural key of the generated kernels
@@ -353,6 +373,7 @@ struct rt_hip_scheme {
     int block = rt::kBlock;          // lanes per workgroup
     int waves = 0;                   // minimum waves per SIMD asked of the compiler (0: its choice)
     int grid_mult = 2;               // persistent workgroups per CU = grid_mult x resident ones
+    Ltab lt;
 };
 
 namespace {
@@ -527,6 +548,64 @@ static void build_device_tables(rt_hip_scheme* s) {
     s->signature = rt_jit_signature(s->dtable.data(), n);
 }
 
+__global__ __launch_bounds__(256) void rt_ltab_kernel(rt::LtabArgs a) { rt::ltab_body(a); }
+
+namespace {
+
+// The table's layout from the signature and its column offsets from the
+// device records of n_inst instances (the table is used only when every
+// instance names the same spline in every column); buffers on first use.
+int ltab_setup(Ltab& t, const std::string& sig, const double* dtables, int n_inst, int n_surf) {
+    static const int kRoleField[4] = {RT_S_N1VAL, RT_S_N2VAL, RT_S_QEVAL, RT_S_SPVAL};
+    rt_jit_ltab_layout(sig, t.base, t.surf, t.role);
+    t.ok = false;
+    const size_t nc = t.surf.size();
+    if (nc == 0) return RT_OK;
+    const size_t inst = (size_t)n_surf * rt::DEV_STRIDE;
+    std::vector<double> off(nc);
+    for (size_t q = 0; q < nc; ++q) {
+        const size_t f = (size_t)t.surf[q] * rt::DEV_STRIDE + (size_t)kRoleField[t.role[q]];
+        off[q] = dtables[f];
+        for (int i = 1; i < n_inst; ++i)
+            if (dtables[(size_t)i * inst + f] != off[q]) return RT_OK;
+    }
+    if (!t.off_dev) HIP_TRY(hipMalloc(&t.off_dev, nc * sizeof(double)), RT_ERR_BAD_ALLOC);
+    if (!t.tab_dev) HIP_TRY(hipMalloc(&t.tab_dev, nc * kLtabMax * sizeof(double)), RT_ERR_BAD_ALLOC);
+    HIP_TRY(hipMemcpy(t.off_dev, off.data(), nc * sizeof(double), hipMemcpyHostToDevice), RT_ERR_FAILED);
+    t.ok = true;
+    return RT_OK;
+}
+
+// wavelengths of the beam's grid when the table serves it (one wavelength, or
+// a composite uniform grid of at most kLtabMax), else 0
+uint64_t ltab_nk(const Ltab& t, const double* beam) {
+    if (!t.ok || !rt_jit_ltab_enabled() || !rt_jit_enabled()) return 0;
+    const int distr = (int)beam[RT_B_RANGE_DISTR];
+    if (distr == RT_RANGE_MONO) return 1;
+    if (distr != RT_RANGE_UNIFORM || beam[RT_B_COMPOSITE] == 0.0) return 0;
+    const double nl = beam[RT_B_NLAMBDA];
+    if (!(nl >= 1.0) || nl > (double)kLtabMax) return 0;
+    return (uint64_t)nl;
+}
+
+// fill the table for nk grid wavelengths of a's beam on `st`
+int ltab_fill(const Ltab& t, const TraceArgs& a, const double* pool_dev, uint64_t nk, hipStream_t st) {
+    rt::LtabArgs la;
+    memset(&la, 0, sizeof la);
+    la.beam = a.beam;
+    la.pool = pool_dev;
+    la.col_off = t.off_dev;
+    la.out = t.tab_dev;
+    la.nk = nk;
+    la.n_cols = (int)t.surf.size();
+    const uint64_t total = (uint64_t)la.n_cols * nk;
+    hipLaunchKernelGGL(rt_ltab_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, la);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    return RT_OK;
+}
+
+}  // namespace
+
 extern "C" int rt_hip_scheme_create(const double* surf_table, int n_surf, const double* spline_pool,
                                     size_t pool_len, rt_hip_scheme** out) {
     if (!out) return set_error(RT_ERR_BAD_VALUE, "out is NULL");
@@ -560,6 +639,14 @@ extern "C" int rt_hip_scheme_create(const double* surf_table, int n_surf, const 
         delete s;
         return set_error(RT_ERR_FAILED, "upload of scheme tables failed");
     }
+    rc = ltab_setup(s->lt, s->signature, s->dtable.data(), 1, n_surf);
+    if (rc != RT_OK) {
+        s->lt.release();
+        (void)hipFree(s->table_dev);
+        (void)hipFree(s->pool_dev);
+        delete s;
+        return rc;
+    }
     *out = s;
     return RT_OK;
 }
@@ -582,6 +669,7 @@ extern "C" int rt_hip_jit_check(const double* surf_table, int n_surf, const doub
 
 extern "C" int rt_hip_scheme_destroy(rt_hip_scheme* s) {
     if (!s) return RT_OK;
+    s->lt.release();
     (void)hipFree(s->table_dev);
     (void)hipFree(s->pool_dev);
     delete s;
@@ -659,10 +747,18 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
     hipStream_t st = (hipStream_t)stream;
     const int beam_mode = gen ? (int)a.beam.v[RT_B_TYPE] : kLoad;
 
+    const uint64_t nk = gen ? ltab_nk(sch->lt, beam_desc) : 0;
     hipFunction_t fn = nullptr;
     if (rt_jit_enabled())
-        fn = rt_jit_kernel(sch->signature, beam_mode, dump != nullptr, sch->block, sch->waves, sch->device);
+        fn = rt_jit_kernel(sch->signature, beam_mode | (nk ? RT_JIT_LTAB : 0), dump != nullptr, sch->block,
+                           sch->waves, sch->device);
     if (fn) {
+        if (nk) {
+            rc = ltab_fill(sch->lt, a, sch->pool_dev, nk, st);
+            if (rc != RT_OK) return rc;
+            a.ltab = sch->lt.tab_dev;
+            a.lt_nk = nk;
+        }
         // persistent grid: the workgroups that are resident at once on every CU
         // (from the compiled kernel's registers and LDS), times grid_mult
         int resident = 0;
@@ -747,8 +843,11 @@ extern "C" int rt_hip_autotune(rt_hip_scheme* sch, const double* beam_desc, uint
         ((flags & RT_TRACE_FLUX) && npix && !flux.alloc(npix * sizeof(double))))
         return set_error(RT_ERR_BAD_ALLOC, "autotune buffers");
     double* R = (double*)rays.p;
+    // each launch traces a new beam (seed, seed + 1, ...), as a workload does:
+    // repeating one beam repeats its frame accesses and runs faster
+    uint64_t launch_no = 0;
     auto run = [&]() {
-        return rt_hip_trace(sch, beam_desc, seed, 0, n_sample, store ? R : nullptr,
+        return rt_hip_trace(sch, beam_desc, seed + launch_no++, 0, n_sample, store ? R : nullptr,
                             store ? R + n_sample : nullptr, store ? R + 2 * n_sample : nullptr,
                             store ? R + 3 * n_sample : nullptr, store ? R + 4 * n_sample : nullptr,
                             store ? R + 5 * n_sample : nullptr, store ? R + 6 * n_sample : nullptr,
@@ -765,72 +864,103 @@ extern "C" int rt_hip_autotune(rt_hip_scheme* sch, const double* beam_desc, uint
     const int blocks[] = {128, 256};
     const int wavess[] = {0, 5, 6, 7, 8};
     const int mults[] = {1, 2, 4, 8, 16};
-    struct Cand { int b, w, m; float ms; };
+    struct Cand { int b, w, m; float ms; int regs, local; };
     std::vector<Cand> cands;
     const int bb0 = sch->block, bw0 = sch->waves, bm0 = sch->grid_mult;
-    auto timed = [&](const Cand& c, float& ms) -> int {
+    // mean time of `reps` back-to-back launches of candidate c
+    auto timed = [&](const Cand& c, int reps, float& ms) -> int {
         sch->block = c.b; sch->waves = c.w; sch->grid_mult = c.m;
         (void)hipEventRecord(e0, st);
-        int r = run();
+        int r = RT_OK;
+        for (int k = 0; k < reps && r == RT_OK; ++k) r = run();
         (void)hipEventRecord(e1, st);
         if (r != RT_OK) return r;
         if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
             return set_error(RT_ERR_FAILED, "autotune timing");
+        ms /= (float)reps;
         return RT_OK;
     };
-    // screening: every candidate, best of 2 after a compiling/warming launch.
-    // Register allocations that spill to scratch memory are skipped when a
-    // spill-free one exists: their time is within run-to-run noise of the
-    // spill-free launches, and the spills add memory traffic.
+    // every candidate launched once (compiling its kernel); its registers and
+    // private (scratch) bytes per lane as the runtime reports them
     rc = RT_OK;
     const int beam_mode = (int)beam_desc[RT_B_TYPE];
-    bool any_clean = false;
-    std::vector<Cand> spilling;
     for (int b : blocks)
         for (int w : wavess)
             for (int m : mults) {
                 if (rc != RT_OK) break;
-                Cand c{b, w, m, 1e30f};
+                Cand c{b, w, m, 1e30f, 0, 0};
                 sch->block = b; sch->waves = w; sch->grid_mult = m;
-                if ((rc = run()) != RT_OK) break;                      // first launch compiles
+                if ((rc = run()) != RT_OK) break;
                 if (hipStreamSynchronize(st) != hipSuccess) { rc = set_error(RT_ERR_FAILED, "autotune"); break; }
                 int local = 0;
+                const int mode = beam_mode | (ltab_nk(sch->lt, beam_desc) ? RT_JIT_LTAB : 0);
                 hipFunction_t fn =
-                    rt_jit_enabled() ? rt_jit_kernel(sch->signature, beam_mode, false, b, w, sch->device) : nullptr;
+                    rt_jit_enabled() ? rt_jit_kernel(sch->signature, mode, false, b, w, sch->device) : nullptr;
                 if (fn && hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fn) != hipSuccess)
                     local = 0;
-                for (int k = 0; k < 2 && rc == RT_OK; ++k) {
-                    float t = 0.0f;
-                    rc = timed(c, t);
-                    if (rc == RT_OK && t < c.ms) c.ms = t;
-                }
-                if (local > 0) {
-                    spilling.push_back(c);
-                } else {
-                    cands.push_back(c);
-                    any_clean = true;
-                }
+                if (fn && hipFuncGetAttribute(&c.regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, fn) != hipSuccess) c.regs = 0;
+                c.local = local;
+                cands.push_back(c);
             }
-    if (!any_clean) cands = spilling;
-    // the clock of a sustained FP64 load drifts between launches: the four best
-    // are re-timed in interleaved rounds and ranked by their median
-    int bb = bb0, bw = bw0, bm = bm0;
-    double best = 1e300;
-    if (rc == RT_OK && !cands.empty()) {
-        std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
-        const size_t top = cands.size() < 4 ? cands.size() : 4;
-        const int rounds = 7;
-        std::vector<std::vector<float>> t(top);
+    // launches that spill more than kMaxScratch bytes per lane are set aside
+    // when one at or below it exists: on the fused echelle trace they timed
+    // about 10% faster here than in the caller's sustained loops (7 waves/SIMD,
+    // 96 B per lane: 1.20 against 1.32 ms), the lighter ones within 2%
+    constexpr int kMaxScratch = 64;
+    if (std::any_of(cands.begin(), cands.end(), [](const Cand& c) { return c.local <= kMaxScratch; }))
+        cands.erase(std::remove_if(cands.begin(), cands.end(), [](const Cand& c) { return c.local > kMaxScratch; }),
+                    cands.end());
+    // Timed as the workload runs, back to back: the clock of a sustained FP64
+    // load settles below that of isolated launches, and by how much depends
+    // on the launch (occupancy, scratch traffic), so isolated launches rank
+    // candidates differently.  The clock also drifts, so candidates are timed
+    // in interleaved rounds: screening by the best of 3 rounds of 3-launch
+    // bursts, then the four best ranked by the median of 5 rounds of
+    // 20-launch bursts (long enough for the clock to settle under launches
+    // that spill to scratch); within 0.5% the one with less scratch memory
+    // wins.
+    auto rounds_of = [&](std::vector<Cand>& cs, int rounds, int reps, std::vector<std::vector<float>>& t) {
+        t.assign(cs.size(), {});
         for (int r = 0; r < rounds && rc == RT_OK; ++r)
-            for (size_t i = 0; i < top && rc == RT_OK; ++i) {
+            for (size_t i = 0; i < cs.size() && rc == RT_OK; ++i) {
                 float ms = 0.0f;
-                rc = timed(cands[i], ms);
+                rc = timed(cs[i], reps, ms);
                 t[i].push_back(ms);
             }
-        for (size_t i = 0; i < top && rc == RT_OK; ++i) {
+    };
+    int bb = bb0, bw = bw0, bm = bm0;
+    double best = 1e300;
+    std::vector<std::vector<float>> t;
+    if (rc == RT_OK && !cands.empty()) {
+        rounds_of(cands, 3, 3, t);
+        for (size_t i = 0; i < cands.size() && rc == RT_OK; ++i)
+            cands[i].ms = *std::min_element(t[i].begin(), t[i].end());
+    }
+    // RT_HIP_AUTOTUNE_LOG=1: every candidate's screening time on stderr
+    const char* lg = getenv("RT_HIP_AUTOTUNE_LOG");
+    const bool log = lg && lg[0] == '1';
+    if (log)
+        for (const Cand& c : cands)
+            fprintf(stderr, "rt_hip autotune: %d,%d,%d registers %d, private %d B: screening %.4f ms\n", c.b, c.w,
+                    c.m, c.regs, c.local, c.ms);
+    if (rc == RT_OK && !cands.empty()) {
+        std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+        if (cands.size() > 4) cands.resize(4);
+        rounds_of(cands, 5, 20, t);
+        int best_local = 0;
+        for (size_t i = 0; i < cands.size() && rc == RT_OK; ++i) {
             std::sort(t[i].begin(), t[i].end());
             const double med = t[i][t[i].size() / 2];
-            if (med < best) { best = med; bb = cands[i].b; bw = cands[i].w; bm = cands[i].m; }
+            if (log) fprintf(stderr, "rt_hip autotune: %d,%d,%d median of 5 %.4f ms\n", cands[i].b, cands[i].w,
+                             cands[i].m, med);
+            const bool better = best == 1e300 || med < best * (1.0 - 0.005) ||
+                                (med < best * (1.0 + 0.005) && (cands[i].local < best_local ||
+                                                                 (cands[i].local == best_local && med < best)));
+            if (better) {
+                best = med;
+                best_local = cands[i].local;
+                bb = cands[i].b; bw = cands[i].w; bm = cands[i].m;
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -862,6 +992,7 @@ struct rt_hip_batch {
     // descent map (rt_hip_batch_descend), kept until the parameter list changes
     std::vector<int> descend_key;
     int* descend_dev = nullptr;      // map_start[n_surf + 1] then the (j, field) pairs
+    Ltab lt;
 };
 
 struct rt_hip_graph {
@@ -880,6 +1011,7 @@ void free_batch(rt_hip_batch* b) {
     (void)hipFree(b->descend_dev);
     (void)hipFree(b->tile_start_dev);
     (void)hipFree(b->moments_dev);
+    b->lt.release();
     delete b;
 }
 
@@ -997,6 +1129,7 @@ extern "C" int rt_hip_batch_create(const double* tables, int n_inst, int n_surf,
         return set_error(RT_ERR_FAILED, "upload of batch tables failed");
     }
     rc = prepare_tiles(b);
+    if (rc == RT_OK) rc = ltab_setup(b->lt, b->signature, b->dtables.data(), b->n_inst, b->n_surf);
     if (rc != RT_OK) { free_batch(b); return rc; }
     *out = b;
     return RT_OK;
@@ -1021,7 +1154,7 @@ extern "C" int rt_hip_batch_update(rt_hip_batch* b, const double* tables, const 
     if (!b->dpool.empty())
         HIP_TRY(hipMemcpy(b->pool_dev, b->dpool.data(), b->dpool.size() * sizeof(double), hipMemcpyHostToDevice),
                 RT_ERR_FAILED);
-    return RT_OK;
+    return ltab_setup(b->lt, b->signature, b->dtables.data(), b->n_inst, b->n_surf);
 }
 
 extern "C" int rt_hip_batch_destroy(rt_hip_batch* b) {
@@ -1092,10 +1225,18 @@ extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_des
     const size_t lds = b->pool_len * sizeof(double);
     hipStream_t st = (hipStream_t)stream;
     const int beam_mode = (int)a.beam.v[RT_B_TYPE];
+    const uint64_t nk = ltab_nk(b->lt, beam_desc);
     hipFunction_t fn = nullptr;
     if (rt_jit_enabled())
-        fn = rt_jit_kernel(b->signature, beam_mode, false, b->block, b->waves, b->device, true);
+        fn = rt_jit_kernel(b->signature, beam_mode | (nk ? RT_JIT_LTAB : 0), false, b->block, b->waves, b->device,
+                           true);
     if (fn) {
+        if (nk) {
+            rc = ltab_fill(b->lt, a, b->pool_dev, nk, st);
+            if (rc != RT_OK) return rc;
+            a.ltab = b->lt.tab_dev;
+            a.lt_nk = nk;
+        }
         int resident = 0;
         if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, b->block, lds) != hipSuccess ||
             resident < 1)
@@ -1150,7 +1291,8 @@ extern "C" int rt_hip_batch_graph_create(const rt_hip_batch* b, const double* be
     if (rt_jit_enabled()) {
         int rc = validate_beam(beam_desc);
         if (rc != RT_OK) return rc;
-        (void)rt_jit_kernel(b->signature, (int)beam_desc[RT_B_TYPE], false, b->block, b->waves, b->device, true);
+        (void)rt_jit_kernel(b->signature, (int)beam_desc[RT_B_TYPE] | (ltab_nk(b->lt, beam_desc) ? RT_JIT_LTAB : 0),
+                            false, b->block, b->waves, b->device, true);
     }
     rt_hip_graph* g = new (std::nothrow) rt_hip_graph();
     if (!g) return set_error(RT_ERR_BAD_ALLOC, "out of host memory");

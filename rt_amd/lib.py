@@ -42,6 +42,7 @@ def lib():
         L.rt_hip_trace.argtypes = ([_vp, _dp, C.c_uint64, C.c_size_t, C.c_size_t] + [_vp] * 8 +
                                    [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp])
         L.rt_hip_set_jit.argtypes = [C.c_int]
+        L.rt_hip_set_ltab.argtypes = [C.c_int]
         L.rt_hip_scheme_set_launch.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
         L.rt_hip_scheme_get_launch.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.rt_hip_autotune.argtypes = [_vp, _dp, C.c_uint64, C.c_size_t, C.c_int, C.POINTER(C.c_double), _vp]
@@ -215,6 +216,12 @@ class DeviceScheme:
 def set_jit(on: bool) -> bool:
     """Use kernels generated per scheme structure (True) or the generic kernel."""
     return bool(lib().rt_hip_set_jit(1 if on else 0))
+
+
+def set_ltab(on: bool) -> bool:
+    """Generated traces of grid-wavelength beams read the per-wavelength table
+    of spline values (True, the default) or evaluate the splines per ray."""
+    return bool(lib().rt_hip_set_ltab(1 if on else 0))
 
 
 def last_kernel() -> str:

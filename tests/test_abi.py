@@ -94,6 +94,23 @@ def test_generated_kernel_compiles(make, mode):
     assert text.count("rt::surface_step(") == make().n_surf
 
 
+def test_generated_table_kernel_compiles():
+    """The trace generated for a grid-wavelength beam reading the
+    per-wavelength table (0x400 = RT_JIT_LTAB) compiles, with one column per
+    spline-valued function that is not reused from the previous surface: the
+    echelle's three glasses (n2 of each first lens surface) and the detector
+    QE; each column is loaded just before the surface that uses it."""
+    sch = schemes.echelle(1, 4)
+    rc, size, text = lib.jit_check(sch, S.RT_BEAM_GAUSS | 0x400)
+    assert rc == 0, text[-2000:]
+    assert "double lv[4];" in text
+    assert text.count("rt::load_lval(lv, c, ") == 4
+    assert text.count(", lv, ") == 4
+    # without the flag: no table
+    rc, size, text = lib.jit_check(sch, S.RT_BEAM_GAUSS)
+    assert rc == 0 and "lv" not in text
+
+
 def test_generated_adjoint_kernel_compiles():
     """The adjoint kernel generated for the echelle with every slot of every
     surface asked for (0x200 = RT_JIT_ADJOINT) compiles: one forward and one

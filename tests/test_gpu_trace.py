@@ -238,3 +238,47 @@ def test_clustered_ccd_hits_are_counted_exactly(hip, kernel):
     assert np.array_equal(g["stats"], ref.stats)
     nz = np.count_nonzero(ref.hist)
     assert 0 < nz <= 40 and ref.hist.sum() > 0.9 * sch.beam.n_rays, nz
+
+
+def _echelle_mono():
+    sch = schemes.echelle(2000, 8)
+    sch.beam.range_distr = S.RT_RANGE_MONO
+    return sch
+
+
+def _echelle_oob():
+    # 0.30-0.90 um: the glass tables start at 0.35 um, so the bluest grid
+    # wavelengths are out of range at the first lens
+    sch = schemes.echelle(600, 64)
+    sch.beam.range = (0.30, 0.90)
+    return sch
+
+
+@pytest.mark.parametrize("make", [lambda: schemes.echelle(600, 64), _echelle_mono, _echelle_oob],
+                         ids=["grid", "mono", "out-of-range"])
+def test_wavelength_table_matches_per_ray_splines(hip, make):
+    """Generated traces of grid-wavelength beams read their refractive indices
+    and detector QE from the per-wavelength table (filled by the same spline
+    code at the same wavelengths): rays, flags, hit counts and counters equal
+    those of the per-ray spline evaluation bit for bit, the flux frame to
+    1e-12 (FP64 atomics add in any order)."""
+    sch = make()
+    prev_jit = hip.set_jit(True)
+    prev = hip.set_ltab(False)
+    try:
+        off = gpu_trace(hip, sch, seed=3, dump=False, flux=True, expect_kernel="generated")
+    finally:
+        hip.set_ltab(prev)
+    try:
+        on = gpu_trace(hip, sch, seed=3, dump=False, flux=True, expect_kernel="generated")
+    finally:
+        hip.set_jit(prev_jit)
+    assert np.array_equal(on["rays"].view(np.uint64), off["rays"].view(np.uint64))
+    assert np.array_equal(on["flag"], off["flag"])
+    assert np.array_equal(on["hist"], off["hist"])
+    assert np.array_equal(on["stats"], off["stats"])
+    scale = max(float(np.max(np.abs(off["flux"]))), 1e-300)
+    assert float(np.max(np.abs(on["flux"] - off["flux"]))) <= 1e-12 * scale
+    assert int(on["stats"][S.ST_RAYS]) == sch.beam.n_rays
+    if make is _echelle_oob:
+        assert int(on["stats"][S.ST_SPLINE_OOB]) > 0
