@@ -711,7 +711,25 @@ struct AdjArgs {
     const uint64_t* ray_start;
     const uint32_t* tile_inst;           // instance of each tile
     uint64_t n_inst, n_tiles, inst_stride;
+    // grid-wavelength beams: per-wavelength table of the spline-valued
+    // functions (column q at ltab[q * lt_nk ..]); generated kernels built for it
+    const double* ltab;
+    uint64_t lt_nk;
 };
+
+// Column q of the current ray's per-wavelength table values (AdjArgs is the
+// adjoint kernels' only argument, at offset 0 of the argument segment, so
+// the table pointer and size are read as scalars there).  The grid index is
+// clamped into the table.
+template <int NC>
+__device__ __forceinline__ void load_lval_adj(double (&lv)[NC], const TraceCtx& c, int q) {
+    typedef const __attribute__((address_space(4))) AdjArgs* cadj;
+    cadj ka = (cadj)__builtin_amdgcn_kernarg_segment_ptr();
+    __asm__ volatile("" : "+s"(ka));
+    const uint64_t nk = ka->lt_nk;
+    const uint64_t k = (uint64_t)c.lk < nk ? (uint64_t)c.lk : nk - 1;
+    lv[q] = ka->ltab[(uint64_t)q * nk + k];
+}
 
 // Lane-interleaved rows of doubles (row k of lane l at (k * G + l) * 8 bytes)
 // addressed as a buffer resource: the lane's byte offset stays in one VGPR
@@ -763,24 +781,28 @@ __device__ __forceinline__ LaneRows lane_rows(double* base, uint64_t rows, uint6
 
 // forward through one surface: tape of the entry state (rows k0 .. k0+6),
 // the trace's surface step, the Newton count
+// (lv, col: the ray's per-wavelength table values, as surface_step takes them)
 template <class SD>
 __device__ __forceinline__ void fwd_step(const SD& d, int s, Ray& r, bool& alive, TraceCtx& c, const LaneRows& t,
-                                         int k0) {
+                                         int k0, const double* lv = nullptr, int col = -1) {
     if (alive) {
         t.st(k0 + 0, r.X); t.st(k0 + 1, r.Y); t.st(k0 + 2, r.Z);
         t.st(k0 + 3, r.cX); t.st(k0 + 4, r.cY); t.st(k0 + 5, r.cZ);
     }
-    surface_step(d, s, r, alive, c);
+    surface_step(d, s, r, alive, c, lv, col);
     if (alive) t.st(k0 + 6, (double)c.last_it);
 }
 
 // backward through one surface: gp receives the surface's parameter
 // derivatives (zero where the replay fails).  false: no adjoint (curved
 // grating, replay lost).  The lanes of the wave are converged again when it
-// returns, so the caller can reduce gp across the wave.
+// returns, so the caller can reduce gp across the wave.  With lv (the ray's
+// per-wavelength table values), n1 and n2 are lv[n1col] and lv[n2col], or
+// the surface's constants where a column is -1.
 template <class SD>
 __device__ __forceinline__ bool bwd_step(const SD& d, const Ray& r, St& ad, const TraceCtx& c, const LaneRows& t,
-                                         int k0, double* gp) {
+                                         int k0, double* gp, const double* lv = nullptr, int n1col = -1,
+                                         int n2col = -1) {
     const auto p = d.p;
 RT_UNROLL
     for (int k = 0; k < PG_N; ++k) gp[k] = 0.0;
@@ -790,8 +812,14 @@ RT_UNROLL
     double mu = 0.0, L = 0.0;
     if (d.type() == RT_LENS || d.type() == RT_GRATING) {
         int e1 = 0, e2 = 0;
-        const double n1 = fn_eval(c.pool, d.n1spline(), p[RT_S_N1VAL], r.lam, e1);
-        const double n2 = fn_eval(c.pool, d.n2spline(), p[RT_S_N2VAL], r.lam, e2);
+        double n1, n2;
+        if (lv) {
+            n1 = n1col >= 0 ? lv[n1col] : p[RT_S_N1VAL];
+            n2 = n2col >= 0 ? lv[n2col] : p[RT_S_N2VAL];
+        } else {
+            n1 = fn_eval(c.pool, d.n1spline(), p[RT_S_N1VAL], r.lam, e1);
+            n2 = fn_eval(c.pool, d.n2spline(), p[RT_S_N2VAL], r.lam, e2);
+        }
         const double in2 = rcp(n2);
         mu = n1 * in2;
         if (d.type() == RT_GRATING) L = grating_order(d, r.lam) * r.lam * in2 * p[D_INV_D];
@@ -990,7 +1018,9 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs& a, Fwd forward, Bwd 
         c.table = tbl;
         Ray r;
         cptr bm = (cptr)__builtin_amdgcn_kernarg_segment_ptr();
-        gen_ray<BEAM>(bm, a.seed, i, r);
+        uint64_t jl;
+        gen_ray<BEAM>(bm, a.seed, i, r, jl);
+        c.lk = (uint32_t)jl;
         bool alive = in;
         forward(r, alive, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane));
         unsigned long long good = __ballot(alive), all = __ballot(in);
@@ -1067,7 +1097,9 @@ __device__ __forceinline__ void adjoint_batch_body(const AdjArgs& a, Fwd forward
         c.table = tbl;
         Ray ray;
         cptr bm = (cptr)ka;
-        gen_ray<BEAM>(bm, ka->seed + inst, r, ray);
+        uint64_t jl;
+        gen_ray<BEAM>(bm, ka->seed + inst, r, ray, jl);
+        c.lk = (uint32_t)jl;
         bool alive = in;
         forward(ray, alive, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane));
         unsigned long long good = __ballot(alive), all = __ballot(in);

@@ -31,6 +31,19 @@ using rt::adj::kTape;
 namespace {
 
 constexpr int kAdjBlock = 256;
+
+// p[0 .. n) = 0 by a kernel (a caller may capture the call in a hipGraph,
+// where memset nodes are avoided)
+__global__ __launch_bounds__(256) void rt_zero_kernel(double* p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+int zero_fill(double* p, uint64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(rt_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    return RT_OK;
+}
 constexpr int kReduceChunks = 64;        // first-stage workgroups per parameter
 
 template <int BEAM>
@@ -238,7 +251,7 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     if (rc != RT_OK) return rc;
     const int beam_type = (int)beam_desc[RT_B_TYPE];
     if (n_rays == 0) {
-        if (n_params) HIP_TRY(hipMemsetAsync(grad, 0, n_params * sizeof(double), (hipStream_t)stream), RT_ERR_FAILED);
+        if (n_params) return zero_fill(grad, (uint64_t)n_params, (hipStream_t)stream);
         return RT_OK;
     }
     rc = rt_ensure_device();
@@ -259,10 +272,12 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     a.stats = stats;
 
     // the kernel generated for this structure and parameter list, else the generic one
+    // (grid-wavelength beams: the generated kernel reads the per-wavelength table)
+    const uint64_t nk = rt_scheme_ltab_nk(sch, beam_desc);
     hipFunction_t fn = nullptr;
     if (rt_jit_enabled())
         fn = rt_jit_adjoint_kernel(rt_scheme_signature(sch), beam_type, host, n_surf, kAdjBlock, adjoint_waves(),
-                                   rt_scheme_device(sch), false);
+                                   rt_scheme_device(sch), false, nk != 0);
     auto kern = beam_type == RT_BEAM_PARALLEL ? rt_adjoint_kernel<RT_BEAM_PARALLEL>
               : beam_type == RT_BEAM_CONIC    ? rt_adjoint_kernel<RT_BEAM_CONIC>
                                               : rt_adjoint_kernel<RT_BEAM_GAUSS>;
@@ -308,6 +323,11 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     a.prow = B.params;
     a.tape = B.tape;
     a.gacc = B.gacc;
+    if (fn && nk) {
+        rc = rt_scheme_ltab_fill(sch, beam_desc, nk, st, &a.ltab);
+        if (rc != RT_OK) return rc;
+        a.lt_nk = nk;
+    }
     if (fn) {
         void* params[] = {&a};
         HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, kAdjBlock, 1, 1, (unsigned)lds, st, params, nullptr),
@@ -346,8 +366,7 @@ extern "C" int rt_hip_batch_adjoint(const rt_hip_batch* b, const double* beam_de
     rc = rt_ensure_device();
     if (rc != RT_OK) return rc;
     if (n_params == 0 || v.n_tiles == 0) {
-        if (n_params) HIP_TRY(hipMemsetAsync(grad, 0, (size_t)v.n_inst * n_params * sizeof(double),
-                                             (hipStream_t)stream), RT_ERR_FAILED);
+        if (n_params) return zero_fill(grad, (uint64_t)v.n_inst * (uint64_t)n_params, (hipStream_t)stream);
         return RT_OK;
     }
 
@@ -371,10 +390,11 @@ extern "C" int rt_hip_batch_adjoint(const rt_hip_batch* b, const double* beam_de
     a.n_tiles = v.n_tiles;
     a.inst_stride = (uint64_t)v.n_surf * rt::DEV_STRIDE;
 
+    const uint64_t nk = rt_batch_ltab_nk(b, beam_desc);
     hipFunction_t fn = nullptr;
     if (rt_jit_enabled())
         fn = rt_jit_adjoint_kernel(*v.signature, beam_type, host, v.n_surf, kAdjBlock, adjoint_waves(), v.device,
-                                   true);
+                                   true, nk != 0);
     auto kern = beam_type == RT_BEAM_PARALLEL ? rt_adjoint_batch_kernel<RT_BEAM_PARALLEL>
               : beam_type == RT_BEAM_CONIC    ? rt_adjoint_batch_kernel<RT_BEAM_CONIC>
                                               : rt_adjoint_batch_kernel<RT_BEAM_GAUSS>;
@@ -415,6 +435,11 @@ extern "C" int rt_hip_batch_adjoint(const rt_hip_batch* b, const double* beam_de
     a.prow = B.params;
     a.tape = B.tape;
     a.gacc = B.gacc;
+    if (fn && nk) {
+        rc = rt_batch_ltab_fill(b, beam_desc, nk, st, &a.ltab);
+        if (rc != RT_OK) return rc;
+        a.lt_nk = nk;
+    }
     if (fn) {
         void* params[] = {&a};
         HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, kAdjBlock, 1, 1, (unsigned)lds, st, params, nullptr),

@@ -47,3 +47,15 @@ struct BatchView {
     const uint64_t* ray_start;       // host copy
 };
 int rt_batch_view(const rt_hip_batch* b, BatchView* v);
+
+// The per-wavelength table of a scheme or batch (its spline-valued functions
+// at the grid wavelengths of a beam): the number of grid wavelengths it
+// serves for this beam description, 0 when it does not (random wavelengths,
+// no spline-valued function, tables disabled); and its filling on `stream`
+// (*tab: the device table, column q at q * nk).
+uint64_t rt_scheme_ltab_nk(const rt_hip_scheme* s, const double* beam_desc);
+int rt_scheme_ltab_fill(const rt_hip_scheme* s, const double* beam_desc, uint64_t nk, hipStream_t stream,
+                        const double** tab);
+uint64_t rt_batch_ltab_nk(const rt_hip_batch* b, const double* beam_desc);
+int rt_batch_ltab_fill(const rt_hip_batch* b, const double* beam_desc, uint64_t nk, hipStream_t stream,
+                       const double** tab);

@@ -189,14 +189,48 @@ static std::string generate_source(const std::string& sig, int beam_mode, bool d
 // surface steps and the backward steps as straight-line sequences, each
 // backward step adding exactly the asked-for slots of that surface to the
 // wave's register accumulators, in groups of up to four slots.
+// The n-th (0-based) integer template argument of one signature entry.
+static int sig_arg(const std::string& t, int n) {
+    size_t q = t.find('<');
+    if (q == std::string::npos) return 0;
+    ++q;
+    for (int k = 0; k < n && q != std::string::npos; ++k) {
+        q = t.find(',', q);
+        if (q != std::string::npos) ++q;
+    }
+    return q == std::string::npos ? 0 : atoi(t.c_str() + q);
+}
+
 static std::string generate_adjoint_source(const std::string& sig, int beam_type, const std::vector<int>& rows,
-                                           int n_surf, int block, int waves = 0, bool batch = false) {
+                                           int n_surf, int block, int waves = 0, bool batch = false,
+                                           bool ltab = false) {
     std::vector<std::string> types;
     size_t pos = 0;
     while (pos < sig.size()) {
         size_t end = sig.find(';', pos);
         types.push_back(sig.substr(pos, end - pos));
         pos = end + 1;
+    }
+    // per-wavelength table (RT_JIT_LTAB): the forward steps read their columns
+    // as the trace does; the backward steps read n1 and n2 from the columns
+    // the forward read them from (a reused n1 is the previous non-auxiliary
+    // surface's n2 column)
+    std::vector<int> col_base, col_surf, col_role;
+    rt_jit_ltab_layout(sig, col_base, col_surf, col_role);
+    const bool use_tab = ltab && !col_surf.empty();
+    std::vector<int> n1col(n_surf, -1), n2col(n_surf, -1);
+    {
+        int prev_n2 = -1;
+        for (int s = 0; s < n_surf; ++s) {
+            const int type = sig_arg(types[s], 1), fnmask = sig_arg(types[s], 7);
+            for (size_t q = 0; q < col_surf.size(); ++q)
+                if (col_surf[q] == s) {
+                    if (col_role[q] == 0) n1col[s] = (int)q;
+                    if (col_role[q] == 1) n2col[s] = (int)q;
+                }
+            if ((fnmask & 4) != 0) n1col[s] = prev_n2;
+            if (type != RT_AUXTYPE) prev_n2 = n2col[s];
+        }
     }
     // gradient sums in registers (rt::adj::GroupAcc): each surface's slots
     // reduced in groups of up to four; group GRP's inputs x0..x3 end in rows
@@ -226,6 +260,7 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
         for (int k = 0; k < 4; ++k)
             if (groups[gi].par[k] >= 0) map[(gi / 16) * 64 + 16 * kRowOf[k] + gi % 16] = groups[gi].par[k];
     std::ostringstream os;
+    const size_t nc = col_surf.size();
     os << "#include \"rt_adjoint.h\"\n"
        << "__constant__ int rt_adj_slot_map[" << map.size() << "] = {";
     for (size_t k = 0; k < map.size(); ++k) os << (k ? "," : "") << map[k];
@@ -237,15 +272,33 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
        << "  rt::adj::" << (batch ? "adjoint_batch_body<" : "adjoint_body<") << beam_type
        << ", rt::adj::GroupAcc<" << n_reg << ", RtAdjMap>>(a,\n"
        << "    [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, const rt::adj::LaneRows& t) {\n";
-    for (int s = 0; s < n_surf; ++s)
+    if (use_tab) os << "      double lv[" << nc << "];\n";
+    for (int s = 0; s < n_surf; ++s) {
+        if (use_tab)
+            for (size_t q = 0; q < nc; ++q)
+                if (col_surf[q] == s) os << "      rt::adj::load_lval_adj(lv, c, " << q << ");\n";
         os << "      rt::adj::fwd_step(" << types[s] << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s
-           << ", r, alive, c, t, " << s * rt::adj::kTape << ");\n";
+           << ", r, alive, c, t, " << s * rt::adj::kTape;
+        if (use_tab && col_base[(size_t)s] >= 0) os << ", lv, " << col_base[(size_t)s];
+        os << ");\n";
+    }
     os << "    },\n"
        << "    [&](const rt::Ray& r, rt::adj::St& ad, rt::TraceCtx& c, const rt::adj::LaneRows& t, "
           "auto& g, bool ok) -> bool {\n";
+    if (use_tab) os << "      double lb[" << nc << "];\n";
     for (int s = n_surf - 1; s >= 0; --s) {
-        os << "      { double gp[rt::adj::PG_N]; const bool use = rt::adj::bwd_step(" << types[s] << "{c.table + " << s
-           << " * rt::DEV_STRIDE}, r, ad, c, t, " << s * rt::adj::kTape << ", gp) && ok;";
+        const bool tab_s = use_tab && (n1col[(size_t)s] >= 0 || n2col[(size_t)s] >= 0);
+        os << "      {";
+        if (tab_s) {
+            if (n1col[(size_t)s] >= 0) os << " rt::adj::load_lval_adj(lb, c, " << n1col[(size_t)s] << ");";
+            if (n2col[(size_t)s] >= 0 && n2col[(size_t)s] != n1col[(size_t)s])
+                os << " rt::adj::load_lval_adj(lb, c, " << n2col[(size_t)s] << ");";
+        }
+        os << " double gp[rt::adj::PG_N]; const bool use = rt::adj::bwd_step(" << types[s] << "{c.table + " << s
+           << " * rt::DEV_STRIDE}, r, ad, c, t, " << s * rt::adj::kTape << ", gp";
+        if (tab_s) os << ", lb, " << n1col[(size_t)s] << ", " << n2col[(size_t)s];
+        else if (use_tab) os << ", lb, -1, -1";
+        os << ") && ok;";
         for (size_t gi = 0; gi < groups.size(); ++gi) {
             if (groups[gi].surf != s) continue;
             os << " g.template add4<" << gi << ">(";
@@ -337,9 +390,10 @@ hipFunction_t rt_jit_kernel(const std::string& sig, int beam_mode, bool dump, in
 }
 
 hipFunction_t rt_jit_adjoint_kernel(const std::string& sig, int beam_type, const std::vector<int>& rows, int n_surf,
-                                    int block, int waves, int device, bool batch) {
+                                    int block, int waves, int device, bool batch, bool ltab) {
     std::ostringstream key;
-    key << device << "|adjoint|" << (batch ? "batch|" : "") << beam_type << "|" << block << "|" << waves << "|";
+    key << device << "|adjoint|" << (batch ? "batch|" : "") << (ltab ? "ltab|" : "") << beam_type << "|" << block
+        << "|" << waves << "|";
     for (size_t k = 0; k < rows.size(); ++k)
         if (rows[k] >= 0) key << k << ":" << rows[k] << ",";
     key << "|" << sig;
@@ -347,7 +401,8 @@ hipFunction_t rt_jit_adjoint_kernel(const std::string& sig, int beam_type, const
     Entry& e = g_cache[key.str()];
     if (e.fn) return e.fn;
     if (e.failed) return nullptr;
-    if (!compile(generate_adjoint_source(sig, beam_type, rows, n_surf, block, waves, batch), e, "rt_adjoint_jit")) {
+    if (!compile(generate_adjoint_source(sig, beam_type, rows, n_surf, block, waves, batch, ltab), e,
+                 "rt_adjoint_jit")) {
         e.failed = true;
         fprintf(stderr, "rt_hip: generated adjoint kernel failed to build, using the generic kernel:\n%s\n",
                 e.log.c_str());
@@ -364,6 +419,7 @@ extern "C" int rt_jit_check_dtable(const double* dtable, int n_surf, int beam_mo
     const bool batch = beam_mode >= 0 && (beam_mode & RT_JIT_BATCH) != 0;
     const bool adjoint = beam_mode >= 0 && (beam_mode & RT_JIT_ADJOINT) != 0;
     if (beam_mode >= 0) beam_mode &= ~(RT_JIT_BATCH | RT_JIT_ADJOINT);
+    const bool ltab = beam_mode >= 0 && (beam_mode & RT_JIT_LTAB) != 0;
     if (adjoint) beam_mode &= ~RT_JIT_LTAB;
     std::string src;
     if (adjoint) {
@@ -376,7 +432,8 @@ extern "C" int rt_jit_check_dtable(const double* dtable, int n_surf, int beam_mo
                 if (sl < rt::adj::PG_A || sl >= rt::adj::PG_CR || sl - rt::adj::PG_A < npoly)
                     rows[(size_t)s * rt::adj::PG_N + sl] = j++;
         }
-        src = generate_adjoint_source(rt_jit_signature(dtable, n_surf), beam_mode, rows, n_surf, rt::kBlock, 0, batch);
+        src = generate_adjoint_source(rt_jit_signature(dtable, n_surf), beam_mode, rows, n_surf, rt::kBlock, 0, batch,
+                                      ltab);
     } else {
         src = generate_source(rt_jit_signature(dtable, n_surf), beam_mode, false, rt::kBlock, 0, batch);
     }

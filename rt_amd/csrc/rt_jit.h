@@ -34,9 +34,10 @@ hipFunction_t rt_jit_kernel(const std::string& signature, int beam_mode, bool du
 
 // The adjoint kernel generated for (signature, beam type, gradient rows,
 // lanes per workgroup, minimum waves per SIMD or 0, single scheme or batch of
-// instances) on `device`: rows holds
+// instances, reading the per-wavelength table or not) on `device`: rows holds
 // n_surf x PG_N entries, the caller's parameter index of each (surface, slot)
 // or -1.  Compiled on first use and cached; nullptr on failure (the caller
 // launches the generic adjoint kernel).
 hipFunction_t rt_jit_adjoint_kernel(const std::string& signature, int beam_type, const std::vector<int>& rows,
-                                    int n_surf, int block, int waves, int device, bool batch);
+                                    int n_surf, int block, int waves, int device, bool batch,
+                                    bool ltab = false);

@@ -1351,6 +1351,28 @@ int rt_batch_view(const rt_hip_batch* b, BatchView* v) {
 }
 
 const std::string& rt_scheme_signature(const rt_hip_scheme* s) { return s->signature; }
+
+uint64_t rt_scheme_ltab_nk(const rt_hip_scheme* s, const double* beam_desc) { return ltab_nk(s->lt, beam_desc); }
+
+int rt_scheme_ltab_fill(const rt_hip_scheme* s, const double* beam_desc, uint64_t nk, hipStream_t stream,
+                        const double** tab) {
+    TraceArgs a;
+    memset(&a, 0, sizeof a);
+    memcpy(a.beam.v, beam_desc, sizeof a.beam.v);
+    *tab = s->lt.tab_dev;
+    return ltab_fill(s->lt, a, s->pool_dev, nk, stream);
+}
+
+uint64_t rt_batch_ltab_nk(const rt_hip_batch* b, const double* beam_desc) { return ltab_nk(b->lt, beam_desc); }
+
+int rt_batch_ltab_fill(const rt_hip_batch* b, const double* beam_desc, uint64_t nk, hipStream_t stream,
+                       const double** tab) {
+    TraceArgs a;
+    memset(&a, 0, sizeof a);
+    memcpy(a.beam.v, beam_desc, sizeof a.beam.v);
+    *tab = b->lt.tab_dev;
+    return ltab_fill(b->lt, a, b->pool_dev, nk, stream);
+}
 int rt_scheme_device(const rt_hip_scheme* s) { return s->device; }
 
 extern "C" int rt_hip_scheme_device_tables(const rt_hip_scheme* s, const double** table_dev, const double** pool_dev,
