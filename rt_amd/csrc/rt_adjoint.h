@@ -827,57 +827,6 @@ RT_UNROLL
     return surface_adjoint(d, entry, nit, mu, L, ad, gp);
 }
 
-// A double moved between lanes by a DPP pattern inside rows of 16 lanes
-// (every source lane active).
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
-    const uint64_t u = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xF, 0xF, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
-    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
-// x and y exchanged by v_permlane16_swap (H = 16: odd rows of 16 lanes of x
-// with even rows of y) or v_permlane32_swap (H = 32: upper half of x with
-// lower half of y), the two results added: lanes of row pair (2k, 2k+1), or of
-// the two halves, get x's rows summed in the even row (half) and y's rows
-// summed in the odd one, the lower row's value first.
-template <int H>
-__device__ __forceinline__ double swap_sum(double x, double y) {
-    const uint64_t ux = __builtin_bit_cast(uint64_t, x), uy = __builtin_bit_cast(uint64_t, y);
-    uint32_t l0, l1, h0, h1;
-    if (H == 16) {
-        const auto a = __builtin_amdgcn_permlane16_swap((uint32_t)ux, (uint32_t)uy, false, false);
-        const auto b = __builtin_amdgcn_permlane16_swap((uint32_t)(ux >> 32), (uint32_t)(uy >> 32), false, false);
-        l0 = a[0]; l1 = a[1]; h0 = b[0]; h1 = b[1];
-    } else {
-        const auto a = __builtin_amdgcn_permlane32_swap((uint32_t)ux, (uint32_t)uy, false, false);
-        const auto b = __builtin_amdgcn_permlane32_swap((uint32_t)(ux >> 32), (uint32_t)(uy >> 32), false, false);
-        l0 = a[0]; l1 = a[1]; h0 = b[0]; h1 = b[1];
-    }
-    return __builtin_bit_cast(double, ((uint64_t)h0 << 32) | l0) +
-           __builtin_bit_cast(double, ((uint64_t)h1 << 32) | l1);
-}
-
-// sum over the 16 lanes of each row (pairs, quads, 8, 16), the same value in
-// every lane of the row
-__device__ __forceinline__ double row_sum(double v) {
-    v += dpp_f64<0xB1>(v);                   // quad_perm [1,0,3,2]
-    v += dpp_f64<0x4E>(v);                   // quad_perm [2,3,0,1]
-    v += dpp_f64<0x141>(v);                  // row_half_mirror
-    v += dpp_f64<0x140>(v);                  // row_mirror
-    return v;
-}
-
-// Sum over the 64 lanes of the wave in a fixed tree (pairs, quads, 8, 16
-// lanes by DPP, then rows by lane swaps), the same value in every lane and on
-// every run; VALU only (no LDS permutes).  Every lane must be active.
-__device__ __forceinline__ double wave_sum(double v) {
-    v = row_sum(v);
-    v = swap_sum<16>(v, v);
-    return swap_sum<32>(v, v);
-}
-
 // Gradient accumulators of the backward lambdas: add(j, v) adds the wave's
 // sum of v to parameter slot j.  The ray loops call begin/end once per lane
 // (adjoint_body) or tile_begin/tile_end per tile (adjoint_batch_body).

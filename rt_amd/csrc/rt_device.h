@@ -817,6 +817,7 @@ struct TraceArgs {
     // kernels built for it only)
     const double* ltab;
     uint64_t lt_nk;
+    const uint32_t* tile_inst;    // batched traces: the instance of each tile
 };
 
 struct TraceCtx {
@@ -1127,14 +1128,64 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
     }
 }
 
+// A double moved between lanes by a DPP pattern inside rows of 16 lanes
+// (every source lane active).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// x and y exchanged by v_permlane16_swap (H = 16: odd rows of 16 lanes of x
+// with even rows of y) or v_permlane32_swap (H = 32: upper half of x with
+// lower half of y), the two results added: lanes of row pair (2k, 2k+1), or of
+// the two halves, get x's rows summed in the even row (half) and y's rows
+// summed in the odd one, the lower row's value first.
+template <int H>
+__device__ __forceinline__ double swap_sum(double x, double y) {
+    const uint64_t ux = __builtin_bit_cast(uint64_t, x), uy = __builtin_bit_cast(uint64_t, y);
+    uint32_t l0, l1, h0, h1;
+    if (H == 16) {
+        const auto a = __builtin_amdgcn_permlane16_swap((uint32_t)ux, (uint32_t)uy, false, false);
+        const auto b = __builtin_amdgcn_permlane16_swap((uint32_t)(ux >> 32), (uint32_t)(uy >> 32), false, false);
+        l0 = a[0]; l1 = a[1]; h0 = b[0]; h1 = b[1];
+    } else {
+        const auto a = __builtin_amdgcn_permlane32_swap((uint32_t)ux, (uint32_t)uy, false, false);
+        const auto b = __builtin_amdgcn_permlane32_swap((uint32_t)(ux >> 32), (uint32_t)(uy >> 32), false, false);
+        l0 = a[0]; l1 = a[1]; h0 = b[0]; h1 = b[1];
+    }
+    return __builtin_bit_cast(double, ((uint64_t)h0 << 32) | l0) +
+           __builtin_bit_cast(double, ((uint64_t)h1 << 32) | l1);
+}
+
+// sum over the 16 lanes of each row (pairs, quads, 8, 16), the same value in
+// every lane of the row
+__device__ __forceinline__ double row_sum(double v) {
+    v += dpp_f64<0xB1>(v);                   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);                   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);                  // row_half_mirror
+    v += dpp_f64<0x140>(v);                  // row_mirror
+    return v;
+}
+
+// Sum over the 64 lanes of the wave in a fixed tree (pairs, quads, 8, 16
+// lanes by DPP, then rows by lane swaps), the same value in every lane and on
+// every run; VALU only (no LDS permutes).  Every lane must be active.
+__device__ __forceinline__ double wave_sum(double v) {
+    v = row_sum(v);
+    v = swap_sum<16>(v, v);
+    return swap_sum<32>(v, v);
+}
+
 // Sum over the workgroup of K values per lane, the same in every lane: a
-// butterfly within each wave, then the wave partials in wave order, so the
-// result is the same bit for bit on every run.
+// fixed-tree sum within each wave (wave_sum), then the wave partials in wave
+// order, so the result is the same bit for bit on every run.
 template <int K>
 __device__ __forceinline__ void block_sum(double (&v)[K], double* s_red) {
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
     const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     __syncthreads();
     if (__lane_id() == 0)
@@ -1172,15 +1223,11 @@ __device__ __forceinline__ void trace_batch_body(const TraceArgs& a, Surfaces su
     const bool store = (a.flags & RT_TRACE_STORE) != 0;
 
     for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
-        // instance of tile t: the last i with tile_start[i] <= t (uniform)
+        // instance of tile t, from the per-tile map (one scalar load; uniform)
         const cargs ka = kernel_args();
         const cptr64 ts = (cptr64)ka->tile_start;
-        uint64_t lo = 0, hi = a.n_inst;
-        while (hi - lo > 1) {
-            uint64_t mid = (lo + hi) >> 1;
-            if (ts[mid] <= t) lo = mid; else hi = mid;
-        }
-        const uint64_t inst = lo;
+        const uint64_t inst = (uint32_t)__builtin_amdgcn_readfirstlane(
+            (int)((const __attribute__((address_space(4))) uint32_t*)ka->tile_inst)[t]);
         const cptr64 rs = (cptr64)ka->ray_start;
         const uint64_t r0 = rs[inst], n_i = rs[inst + 1] - r0;
         const uint64_t r = (t - ts[inst]) * blockDim.x + threadIdx.x;

@@ -1215,6 +1215,7 @@ extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_des
     }
     a.stats = stats;
     a.tile_start = b->tile_start_dev;
+    a.tile_inst = b->tile_inst_dev;
     a.ray_start = b->ray_start_dev;
     a.tile_moments = b->moments_dev;
     a.n_inst = (uint64_t)b->n_inst;
@@ -1254,6 +1255,7 @@ extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_des
             rc = prepare_tiles(b);
             if (rc != RT_OK) return rc;
             a.tile_start = b->tile_start_dev;
+            a.tile_inst = b->tile_inst_dev;
             a.tile_moments = b->moments_dev;
             a.n_tiles = b->n_tiles;
         }
