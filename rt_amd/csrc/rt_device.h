@@ -191,6 +191,40 @@ __device__ __forceinline__ double fsqrt_pos(double x, double ry) {
     return fma(t, 0.5 * ry, s);
 }
 
+// sin and cos of an angle of moderate size (|x| < 2^20; the beam generators
+// use |x| <= 2 pi): Cody-Waite reduction to r = x - k pi/2, |r| <= pi/4, with
+// pi/2 split in two parts (the first of 31 significant bits, so k times it is
+// exact), then the minimax polynomials of fdlibm's __kernel_sin and
+// __kernel_cos (Sun Microsystems, 1993; coefficients checked against their
+// published bit patterns), and the quadrant k mod 4.  Within 1.5 ulp of the
+// correctly rounded values; no large-argument path, so few registers.
+__device__ __forceinline__ void sincos_bounded(double x, double& s_out, double& c_out) {
+    const double kd = rint(x * 6.36619772367581382433e-01);          // 2/pi
+    const int k = (int)kd;
+    const double r = fma(-kd, 6.07710050650619224932e-11, fma(-kd, 1.57079632673412561417e+00, x));
+    const double z = r * r;
+    // __kernel_sin: r + r^3 (S1 + z (S2 + ... + z^4 S6))
+    const double ps = fma(z, fma(z, fma(z, fma(z, 1.58969099521155010221e-10, -2.50507602534068634195e-08),
+                                        2.75573137070700676789e-06),
+                                 -1.98412698298579493134e-04),
+                          8.33333333332248946124e-03);
+    const double sr = fma(z * r, fma(z, ps, -1.66666666666666324348e-01), r);
+    // __kernel_cos: w + (((1 - w) - z/2) + z^2 (C1 + z (C2 + ... + z^5 C6))), w = 1 - z/2
+    const double pc = z * fma(z, fma(z, fma(z, fma(z, fma(z, -1.13596475577881948265e-11, 2.08757232129817482790e-09),
+                                                    -2.75573143513906633035e-07),
+                                             2.48015872894767294178e-05),
+                                      -1.38888888888741095749e-03),
+                               4.16666666666666019037e-02);
+    const double hz = 0.5 * z, w = 1.0 - hz;
+    const double cr = w + (((1.0 - w) - hz) + z * pc);
+    // quadrant: (s, c) = (sr, cr), (cr, -sr), (-sr, -cr), (-cr, sr) for k mod 4 = 0..3
+    const bool swap = (k & 1) != 0;
+    const double s0 = swap ? cr : sr, c0 = swap ? sr : cr;
+    const uint64_t sgn_s = (uint64_t)(k & 2) << 62, sgn_c = (uint64_t)((k + 1) & 2) << 62;
+    s_out = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, s0) ^ sgn_s);
+    c_out = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, c0) ^ sgn_c);
+}
+
 // ---------------------------------------------------------------- Philox
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -234,7 +268,7 @@ __device__ __forceinline__ void gen_position(cptr B, uint64_t seed, uint64_t r, 
         double phi = kPi2Beam * u2;
         double R = (B[RT_B_P2] - B[RT_B_P1]) * fsqrt(u1) + B[RT_B_P1];
         double s, c;
-        sincos(phi, &s, &c);
+        sincos_bounded(phi, s, c);
         ray.X = R * c + B[RT_B_CEN + 0];
         ray.Y = R * s + B[RT_B_CEN + 1];
     } else {
@@ -268,7 +302,7 @@ __device__ __forceinline__ void gen_direction(cptr B, uint64_t seed, uint64_t r,
     } else if (TYPE == RT_BEAM_CONIC) {                // conic_cosins_kernel, beam.cu:586-611
         double t = atan2(ray.Y, ray.X);
         double st, ct;
-        sincos(t, &st, &ct);
+        sincos_bounded(t, st, ct);
         ray.cZ = B[BU_CONE_COS];
         ray.cX = B[BU_CONE_SIN] * ct;
         ray.cY = B[BU_CONE_SIN] * st;
@@ -280,7 +314,7 @@ __device__ __forceinline__ void gen_direction(cptr B, uint64_t seed, uint64_t r,
         double u4 = u01(o[2], o[3]);
         double rad = fsqrt(-2.0 * log(u3));
         double sb, cb;
-        sincos(kTwoPi * u4, &sb, &cb);
+        sincos_bounded(kTwoPi * u4, sb, cb);
         double tX = tg * (rad * cb);
         double tY = tg * (rad * sb);
         // cos = 1/sqrt(1+t^2); sin = sign(t)/sqrt(1+1/t^2) = t * cos

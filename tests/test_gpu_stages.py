@@ -354,3 +354,24 @@ def test_spline_scale_and_grating_energy(hip):
         err = np.abs(ge[ok] - ref_e[ok])
         peak = float(np.max(np.abs(ref_e[ok])))
         assert float(np.max(err)) <= 1e-10 * peak, (order, float(np.max(err)), peak)
+
+
+@pytest.mark.parametrize("btype,ang", [(S.RT_BEAM_PARALLEL, 0.0), (S.RT_BEAM_GAUSS, 2.35482), (S.RT_BEAM_CONIC, 3.0)],
+                         ids=["parallel", "gauss", "conic"])
+def test_beam_angles_match_libm_within_ulps(hip, btype, ang):
+    """The beam generators' sin/cos (disk angle, Box-Muller angle, conic
+    azimuth) use a bounded-range reduction and polynomial of their own; the
+    positions and direction cosines agree with the oracle's (C libm) within a
+    few ulps of their scale: |dX|, |dY| <= 1e-15 x outer radius, |dc| <= 1e-15."""
+    sch = schemes.czerny_turner(40000, 5)
+    sch.beam.type = btype
+    p = list(sch.beam.params) + [0.0] * (4 - len(sch.beam.params))
+    p[3] = ang
+    sch.beam.params = tuple(p)
+    print(f"[beam angles] type {btype}", flush=True)
+    ref = O.gen_rays(sch.beam.pack(), 11, 0, sch.beam.n_rays)
+    g = beam_on_gpu(hip, sch, 11)
+    R = max(float(sch.beam.params[2]), 1.0)
+    assert float(np.max(np.abs(g[0:2] - ref[0:2]))) <= 1e-15 * R
+    assert float(np.max(np.abs(g[3:6] - ref[3:6]))) <= 1e-15
+    assert np.array_equal(g[6], ref[6])
