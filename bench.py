@@ -171,7 +171,9 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
                      "traffic": measured_traffic(block, waves)},
         "detail": {"kernel_ms": kernel_ms, "launch": {"block": block, "waves_per_simd": waves,
-                                                      "grid_mult": grid_mult, "autotune_sample_ms": tuned_ms},
+                                                      "grid_mult": grid_mult,
+                                                      "surface_unroll": "full" if ds.get_unroll() else "loop",
+                                                      "autotune_sample_ms": tuned_ms},
                    "ray_surface_per_step": ray_surf // args.steps,
                    "ccd_hits_per_step": int(st[S.ST_CCD_HITS]) // args.steps,
                    "flop_per_step": fl},

@@ -282,10 +282,20 @@ int rt_hip_set_ltab(int mode);
 int rt_hip_scheme_set_launch(rt_hip_scheme *scheme, int block, int waves, int grid_mult);
 int rt_hip_scheme_get_launch(const rt_hip_scheme *scheme, int *block, int *waves, int *grid_mult);
 
+/* Surface unroll depth of a scheme's traces: full != 0 (default) traces with
+   the kernel generated for the scheme's structure, the whole surface chain
+   unrolled into straight-line code; full == 0 with the generic kernel, a
+   runtime loop over the surface records (smaller code, for long schemes).
+   The autotuner times both.  No counterpart in the reference. */
+int rt_hip_scheme_set_unroll(rt_hip_scheme *scheme, int full);
+int rt_hip_scheme_get_unroll(const rt_hip_scheme *scheme, int *full);
+
 /* Autotuner: traces n_sample rays of the beam (generated from seed, seed + 1,
    ... for successive launches, with `flags`) under each candidate launch
-   (block 128/256, waves 0/5/6/7/8, grid multiple 1/2/4/8/16; those spilling
-   more than 64 bytes per lane to scratch are set aside when others exist) in
+   (block 128/256, waves 0/5/6/7/8, grid multiple 1/2/4/8/16 of the fully
+   unrolled generated kernel, and the runtime surface loop of the generic
+   kernel; those spilling more than 64 bytes per lane to scratch are set
+   aside when others exist) in
    3 interleaved rounds of 3 back-to-back launches, re-times the four fastest
    in 5 rounds of 20 (a sustained load runs at a lower clock than isolated
    launches, by an amount that depends on the launch), keeps the one with the

@@ -373,6 +373,9 @@ struct rt_hip_scheme {
     int block = rt::kBlock;          // lanes per workgroup
     int waves = 0;                   // minimum waves per SIMD asked of the compiler (0: its choice)
     int grid_mult = 2;               // persistent workgroups per CU = grid_mult x resident ones
+    // surface unroll depth: 1, the whole chain unrolled (the generated
+    // kernel); 0, a runtime loop over the records (the generic kernel)
+    int unroll = 1;
     Ltab lt;
 };
 
@@ -747,9 +750,9 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
     hipStream_t st = (hipStream_t)stream;
     const int beam_mode = gen ? (int)a.beam.v[RT_B_TYPE] : kLoad;
 
-    const uint64_t nk = gen ? ltab_nk(sch->lt, beam_desc) : 0;
+    const uint64_t nk = gen && sch->unroll ? ltab_nk(sch->lt, beam_desc) : 0;
     hipFunction_t fn = nullptr;
-    if (rt_jit_enabled())
+    if (rt_jit_enabled() && sch->unroll)
         fn = rt_jit_kernel(sch->signature, beam_mode | (nk ? RT_JIT_LTAB : 0), dump != nullptr, sch->block,
                            sch->waves, sch->device);
     if (fn) {
@@ -798,6 +801,18 @@ extern "C" int rt_hip_scheme_set_launch(rt_hip_scheme* sch, int block, int waves
     sch->block = block;
     sch->waves = waves;
     sch->grid_mult = grid_mult;
+    return RT_OK;
+}
+
+extern "C" int rt_hip_scheme_set_unroll(rt_hip_scheme* sch, int full) {
+    if (!sch) return set_error(RT_ERR_BAD_VALUE, "scheme is NULL");
+    sch->unroll = full ? 1 : 0;
+    return RT_OK;
+}
+
+extern "C" int rt_hip_scheme_get_unroll(const rt_hip_scheme* sch, int* full) {
+    if (!sch) return set_error(RT_ERR_BAD_VALUE, "scheme is NULL");
+    if (full) *full = sch->unroll;
     return RT_OK;
 }
 
@@ -864,12 +879,12 @@ extern "C" int rt_hip_autotune(rt_hip_scheme* sch, const double* beam_desc, uint
     const int blocks[] = {128, 256};
     const int wavess[] = {0, 5, 6, 7, 8};
     const int mults[] = {1, 2, 4, 8, 16};
-    struct Cand { int b, w, m; float ms; int regs, local; };
+    struct Cand { int b, w, m; float ms; int regs, local, u; };
     std::vector<Cand> cands;
-    const int bb0 = sch->block, bw0 = sch->waves, bm0 = sch->grid_mult;
+    const int bb0 = sch->block, bw0 = sch->waves, bm0 = sch->grid_mult, bu0 = sch->unroll;
     // mean time of `reps` back-to-back launches of candidate c
     auto timed = [&](const Cand& c, int reps, float& ms) -> int {
-        sch->block = c.b; sch->waves = c.w; sch->grid_mult = c.m;
+        sch->block = c.b; sch->waves = c.w; sch->grid_mult = c.m; sch->unroll = c.u;
         (void)hipEventRecord(e0, st);
         int r = RT_OK;
         for (int k = 0; k < reps && r == RT_OK; ++k) r = run();
@@ -882,13 +897,22 @@ extern "C" int rt_hip_autotune(rt_hip_scheme* sch, const double* beam_desc, uint
     };
     // every candidate launched once (compiling its kernel); its registers and
     // private (scratch) bytes per lane as the runtime reports them
+    // (surface unroll depth: the fully unrolled generated kernel under every
+    // launch, and the runtime loop of the generic kernel, whose launch is fixed)
     rc = RT_OK;
     const int beam_mode = (int)beam_desc[RT_B_TYPE];
+    {
+        Cand c{kBlock, 0, 1, 1e30f, 0, 0, 0};
+        sch->unroll = 0;
+        if ((rc = run()) == RT_OK && hipStreamSynchronize(st) != hipSuccess) rc = set_error(RT_ERR_FAILED, "autotune");
+        if (rc == RT_OK) cands.push_back(c);
+    }
+    sch->unroll = 1;
     for (int b : blocks)
         for (int w : wavess)
             for (int m : mults) {
-                if (rc != RT_OK) break;
-                Cand c{b, w, m, 1e30f, 0, 0};
+                if (rc != RT_OK || !rt_jit_enabled()) break;
+                Cand c{b, w, m, 1e30f, 0, 0, 1};
                 sch->block = b; sch->waves = w; sch->grid_mult = m;
                 if ((rc = run()) != RT_OK) break;
                 if (hipStreamSynchronize(st) != hipSuccess) { rc = set_error(RT_ERR_FAILED, "autotune"); break; }
@@ -928,7 +952,7 @@ extern "C" int rt_hip_autotune(rt_hip_scheme* sch, const double* beam_desc, uint
                 t[i].push_back(ms);
             }
     };
-    int bb = bb0, bw = bw0, bm = bm0;
+    int bb = bb0, bw = bw0, bm = bm0, bu = bu0;
     double best = 1e300;
     std::vector<std::vector<float>> t;
     if (rc == RT_OK && !cands.empty()) {
@@ -941,8 +965,8 @@ extern "C" int rt_hip_autotune(rt_hip_scheme* sch, const double* beam_desc, uint
     const bool log = lg && lg[0] == '1';
     if (log)
         for (const Cand& c : cands)
-            fprintf(stderr, "rt_hip autotune: %d,%d,%d registers %d, private %d B: screening %.4f ms\n", c.b, c.w,
-                    c.m, c.regs, c.local, c.ms);
+            fprintf(stderr, "rt_hip autotune: %d,%d,%d %s, registers %d, private %d B: screening %.4f ms\n", c.b,
+                    c.w, c.m, c.u ? "unrolled" : "loop", c.regs, c.local, c.ms);
     if (rc == RT_OK && !cands.empty()) {
         std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
         if (cands.size() > 4) cands.resize(4);
@@ -951,21 +975,21 @@ extern "C" int rt_hip_autotune(rt_hip_scheme* sch, const double* beam_desc, uint
         for (size_t i = 0; i < cands.size() && rc == RT_OK; ++i) {
             std::sort(t[i].begin(), t[i].end());
             const double med = t[i][t[i].size() / 2];
-            if (log) fprintf(stderr, "rt_hip autotune: %d,%d,%d median of 5 %.4f ms\n", cands[i].b, cands[i].w,
-                             cands[i].m, med);
+            if (log) fprintf(stderr, "rt_hip autotune: %d,%d,%d %s median of 5 %.4f ms\n", cands[i].b, cands[i].w,
+                             cands[i].m, cands[i].u ? "unrolled" : "loop", med);
             const bool better = best == 1e300 || med < best * (1.0 - 0.005) ||
                                 (med < best * (1.0 + 0.005) && (cands[i].local < best_local ||
                                                                  (cands[i].local == best_local && med < best)));
             if (better) {
                 best = med;
                 best_local = cands[i].local;
-                bb = cands[i].b; bw = cands[i].w; bm = cands[i].m;
+                bb = cands[i].b; bw = cands[i].w; bm = cands[i].m; bu = cands[i].u;
             }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    sch->block = bb; sch->waves = bw; sch->grid_mult = bm;
+    sch->block = bb; sch->waves = bw; sch->grid_mult = bm; sch->unroll = bu;
     if (rc != RT_OK) return rc;
     if (best_ms) *best_ms = best;
     return RT_OK;

@@ -45,6 +45,8 @@ def lib():
         L.rt_hip_set_ltab.argtypes = [C.c_int]
         L.rt_hip_scheme_set_launch.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
         L.rt_hip_scheme_get_launch.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.rt_hip_scheme_set_unroll.argtypes = [_vp, C.c_int]
+        L.rt_hip_scheme_get_unroll.argtypes = [_vp, C.POINTER(C.c_int)]
         L.rt_hip_autotune.argtypes = [_vp, _dp, C.c_uint64, C.c_size_t, C.c_int, C.POINTER(C.c_double), _vp]
         L.rt_hip_last_kernel.argtypes = [C.c_char_p, C.c_size_t]
         L.rt_hip_batch_create.argtypes = [_dp, C.c_int, C.c_int, _dp, C.c_size_t, C.POINTER(C.c_uint64),
@@ -193,6 +195,16 @@ class DeviceScheme:
     def set_launch(self, block: int = 256, waves: int = 0, grid_mult: int = 2):
         """Launch of the generated kernel (rt_hip_scheme_set_launch)."""
         check(lib().rt_hip_scheme_set_launch(self.handle, block, waves, grid_mult), "rt_hip_scheme_set_launch")
+
+    def set_unroll(self, full: bool):
+        """Surface unroll depth (rt_hip_scheme_set_unroll): True, the generated
+        kernel with the surface chain unrolled; False, the generic runtime loop."""
+        check(lib().rt_hip_scheme_set_unroll(self.handle, 1 if full else 0), "rt_hip_scheme_set_unroll")
+
+    def get_unroll(self) -> bool:
+        u = C.c_int()
+        check(lib().rt_hip_scheme_get_unroll(self.handle, C.byref(u)), "rt_hip_scheme_get_unroll")
+        return bool(u.value)
 
     def get_launch(self):
         b, w, m = C.c_int(), C.c_int(), C.c_int()

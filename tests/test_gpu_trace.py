@@ -217,11 +217,32 @@ def test_launch_configurations_agree_and_autotune(hip):
         assert np.array_equal(H.cpu().numpy().view(np.uint32), base["hist"])
         assert np.array_equal(st.cpu().numpy().view(np.uint64), base["stats"])
         ds.close()
+    # surface unroll depth 0: the runtime loop of the generic kernel; flags,
+    # hits and counters as the unrolled kernel, rays within the parity bar
+    ds = hip.DeviceScheme(sch)
+    ds.set_unroll(False)
+    assert ds.get_unroll() is False
+    n = sch.beam.n_rays
+    R = torch.zeros((7, n), dtype=torch.float64, device="cuda")
+    F = torch.zeros(n, dtype=torch.int8, device="cuda")
+    det = sch.detector()
+    H = torch.zeros(det.ccd["ydim"] * det.ccd["xdim"], dtype=torch.int32, device="cuda")
+    st = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
+    print("[launch] surface loop", flush=True)
+    ds.trace(R, F, seed=4, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE | S.RT_TRACE_HIST, hist=H, stats=st)
+    torch.cuda.synchronize()
+    assert hip.last_kernel() == "generic"
+    assert np.array_equal(F.cpu().numpy(), base["flag"])
+    assert rel_err(R.cpu().numpy(), base["rays"], base["flag"] == 1, length_scale(sch)) <= 2 * RTOL
+    assert np.array_equal(H.cpu().numpy().view(np.uint32), base["hist"])
+    assert np.array_equal(st.cpu().numpy().view(np.uint64), base["stats"])
+    ds.close()
     ds = hip.DeviceScheme(sch)
     print("[launch] autotune", flush=True)
     ms = ds.autotune(n_sample=200000, flags=S.RT_TRACE_STORE | S.RT_TRACE_HIST | S.RT_TRACE_FLUX)
     b, w, m = ds.get_launch()
     assert ms > 0 and b in (128, 256) and w in (0, 5, 6, 7, 8) and m in (1, 2, 4, 8, 16)
+    assert ds.get_unroll() in (True, False)
     ds.close()
 
 
