@@ -18,7 +18,10 @@ the units of the scheme description: angles in degrees, distances in mm,
 grating constants in um; ParamSpec.density(s) is a groove density in lines per
 um (1 / grating constant).  SpotLoss.value_and_grad() runs one forward trace
 (rays stored in HBM, spots reduced per group on the device) and one adjoint
-launch; SchemeLoss is the torch.autograd.Function on top.
+launch; SchemeLoss is the torch.autograd.Function on top.  SpotFunction
+differentiates any host function of the group spots the same way (seeds from
+its partial derivatives, spot_seeds), e.g. the linear dispersion
+(linear_dispersion: least-squares slope of the centroids against wavelength).
 """
 from __future__ import annotations
 
@@ -165,6 +168,76 @@ class SpotLoss:
         torch.cuda.synchronize()
         ds.close()
         return loss, self.spec.chain(sch, g.cpu().numpy())
+
+
+def spot_seeds(cnt, cx, cy, rms, d_cx, d_cy, d_rms):
+    """Adjoint seeds (n_groups x 3 float64: ax, ay, b) of a scalar function of
+    the group spots with partial derivatives d_cx, d_cy, d_rms per group:
+    with cx = mean X, rms = sqrt(var X + var Y) over a group's n live rays,
+    dcx/dX_r = 1/n and drms/dX_r = (X_r - cx)/(n rms), so
+    dF/dX_r = ax + b X_r with b = d_rms/(n rms), ax = d_cx/n - b cx (and
+    likewise for Y).  Groups without live rays get zero seeds; a group of zero
+    radius takes no radius term."""
+    cnt, cx, cy, rms = (np.asarray(t, dtype=np.float64) for t in (cnt, cx, cy, rms))
+    d_cx, d_cy, d_rms = (np.broadcast_to(np.asarray(t, dtype=np.float64), cnt.shape) for t in (d_cx, d_cy, d_rms))
+    live = cnt > 0
+    n = np.where(live, cnt, 1.0)
+    b = np.where(live & (rms > 0), d_rms / (n * np.where(rms > 0, rms, 1.0)), 0.0)
+    ax = np.where(live, d_cx / n - b * cx, 0.0)
+    ay = np.where(live, d_cy / n - b * cy, 0.0)
+    return np.ascontiguousarray(np.stack([ax, ay, b], axis=1))
+
+
+class SpotFunction:
+    """value and gradient of F(spots) for `spec` parameters of `base`, F any
+    scalar function of the per-group spots of the surviving rays, evaluated
+    on the host: f(cnt, cx, cy, rms) -> (F, dF/dcx, dF/dcy, dF/drms) (numpy
+    arrays over the groups).  One forward trace (rays stored on the device,
+    spots reduced per group on the device), the seeds of spot_seeds, one
+    adjoint launch."""
+
+    def __init__(self, base: "S.Scheme", spec: ParamSpec, *, n_groups: int, f, seed: int = 0):
+        self.loss = SpotLoss(base, spec, n_groups=n_groups, seed=seed)
+        self.f = f
+
+    def value_and_grad(self, values, grad=True):
+        import torch
+        L = self.loss
+        sch, ds, sp = L._forward(values)
+        h = sp.cpu().numpy()
+        cnt, cx, cy, rms = h[:, 0], h[:, 1], h[:, 2], h[:, 5]
+        val, d_cx, d_cy, d_rms = self.f(cnt, cx, cy, rms)
+        if not grad:
+            ds.close()
+            return float(val), None
+        seeds = torch.from_numpy(spot_seeds(cnt, cx, cy, rms, d_cx, d_cy, d_rms)).cuda()
+        g = ds.adjoint(seeds, L.ps, L.pf, seed=L.seed)
+        torch.cuda.synchronize()
+        ds.close()
+        return float(val), L.spec.chain(sch, g.cpu().numpy())
+
+
+def linear_dispersion(lam, axis: int = 0):
+    """F for SpotFunction: the linear dispersion, the least-squares slope of
+    the centroid coordinate (axis 0: x, 1: y) against the groups' wavelengths
+    `lam` over the groups with live rays (mm per um), and its derivatives:
+    D = sum_g (l_g - l_mean)(c_g - c_mean) / sum_g (l_g - l_mean)^2, so
+    dD/dc_g = (l_g - l_mean) / sum_k (l_k - l_mean)^2."""
+    lam = np.asarray(lam, dtype=np.float64)
+
+    def f(cnt, cx, cy, rms):
+        live = cnt > 0
+        c = cx if axis == 0 else cy
+        lm = lam[live].mean()
+        dl = np.where(live, lam - lm, 0.0)
+        sll = float(np.sum(dl * dl))
+        cm = c[live].mean()
+        D = float(np.sum(dl * np.where(live, c - cm, 0.0))) / sll
+        dD = dl / sll
+        zero = np.zeros_like(c)
+        return (D, dD, zero, zero) if axis == 0 else (D, zero, dD, zero)
+
+    return f
 
 
 def scheme_loss_function():

@@ -9,6 +9,8 @@
   oracle: |adjoint - FD| <= 1e-6 |FD| + the FD's rounding noise (8 eps |loss| /
   2h), for parameters whose FD runs keep the same surviving rays and Newton
   iteration counts (checked; at least 40 of the 48 must qualify).
+* The linear dispersion (SpotFunction over linear_dispersion) against the
+  same finite differences.
 * The torch.autograd.Function returns the same gradient.
 """
 import math
@@ -122,6 +124,71 @@ def test_spot_loss_gradient_matches_finite_differences(hip):
         assert abs(grad[j] - fd_cpu) <= 1e-6 * abs(fd_cpu) + noise, (j, grad[j], fd_cpu, noise)
         assert abs(grad[j] - fd_gpu) <= 1e-6 * abs(fd_gpu) + noise, (j, grad[j], fd_gpu, noise)
         checked += 1
+    assert checked >= 40, checked
+
+
+def _oracle_dispersion(sch, seed, n_groups, lam, axis):
+    r = O.trace(sch, seed=seed)
+    ok = r.flag == 1
+    g = np.arange(sch.beam.n_rays) % n_groups
+    cnt, cx, cy, rms = (np.zeros(n_groups) for _ in range(4))
+    for k in range(n_groups):
+        m = ok & (g == k)
+        cnt[k] = m.sum()
+        if cnt[k]:
+            cx[k], cy[k] = r.rays[0, m].mean(), r.rays[1, m].mean()
+    return diff.linear_dispersion(lam, axis)(cnt, cx, cy, rms)[0], r
+
+
+def test_linear_dispersion_gradient_matches_finite_differences(hip):
+    """Gradient of the linear dispersion (least-squares slope of the y
+    centroids against wavelength, mm per um) over the 48 design parameters,
+    through SpotFunction's seeds and the same adjoint: against central finite
+    differences of the GPU forward and of the oracle, h = 1e-7 relative, bar
+    1e-6 relative + the FD's rounding noise (8 eps times the slope's rounding
+    scale, max |centroid| sum |l - l_mean| / sum (l - l_mean)^2, over 2h), for
+    parameters whose FD runs keep the same surviving rays and Newton iteration
+    counts (at least 40)."""
+    base = schemes.echelle(n_per_lambda=48, nlambda=64)
+    spec = diff.echelle_design_params(base)
+    n_groups, seed, axis = 64, 9, 1
+    lam = np.array([O.gen_rays(base.beam.pack(), seed, k, 1)[6, 0] for k in range(n_groups)])
+    F = diff.SpotFunction(base, spec, n_groups=n_groups, f=diff.linear_dispersion(lam, axis), seed=seed)
+    v0 = spec.values(base)
+    val, grad = F.value_and_grad(v0)
+    oval, oref = _oracle_dispersion(spec.apply(base, v0), seed, n_groups, lam, axis)
+    print(f"[dispersion] {val:.12e} oracle {oval:.12e}", flush=True)
+    assert abs(val - oval) <= 1e-11 * abs(oval)
+    base_cnt = _counters(oref.stats, base.n_surf)
+    # rounding scale of D: centroid rounding (eps |c|) times the slope's
+    # sensitivity sum |l - l_mean| / sum (l - l_mean)^2
+    cnt, cx, cy, _ = (t.cpu().numpy() for t in F.loss.spots(v0))
+    c = (cx if axis == 0 else cy)[cnt > 0]
+    dl = lam[cnt > 0] - lam[cnt > 0].mean()
+    d_scale = max(abs(val), float(np.abs(c).max() * np.abs(dl).sum() / np.sum(dl * dl)))
+    checked = 0
+    for j, p in enumerate(spec.params):
+        x = v0[j]
+        h = 1e-7 * abs(x) if x != 0.0 else 1e-7
+        vp, vm = v0.copy(), v0.copy()
+        vp[j] += h
+        vm[j] -= h
+        dp, rp = _oracle_dispersion(spec.apply(base, vp), seed, n_groups, lam, axis)
+        dm, rm = _oracle_dispersion(spec.apply(base, vm), seed, n_groups, lam, axis)
+        same = (np.array_equal(rp.flag, oref.flag) and np.array_equal(rm.flag, oref.flag) and
+                np.array_equal(_counters(rp.stats, base.n_surf), base_cnt) and
+                np.array_equal(_counters(rm.stats, base.n_surf), base_cnt))
+        if not same:
+            continue
+        noise = 8 * np.finfo(float).eps * d_scale / (2 * h)
+        fd_cpu = (dp - dm) / (2 * h)
+        gp_, _ = F.value_and_grad(vp, grad=False)
+        gm_, _ = F.value_and_grad(vm, grad=False)
+        fd_gpu = (gp_ - gm_) / (2 * h)
+        assert abs(grad[j] - fd_cpu) <= 1e-6 * abs(fd_cpu) + noise, (j, grad[j], fd_cpu, noise)
+        assert abs(grad[j] - fd_gpu) <= 1e-6 * abs(fd_gpu) + noise, (j, grad[j], fd_gpu, noise)
+        checked += 1
+    print(f"[dispersion] {checked} parameters checked", flush=True)
     assert checked >= 40, checked
 
 
