@@ -326,6 +326,16 @@ int rt_hip_batch_create(const double *tables, int n_inst, int n_surf, const doub
 int rt_hip_batch_update(rt_hip_batch *batch, const double *tables, const double *spline_pool,
                         size_t pool_len);
 int rt_hip_batch_destroy(rt_hip_batch *batch);
+
+/* A beam per instance: beam_descs holds n_inst beam descriptions
+   (RT_BEAM_STRIDE doubles each, one beam type for all: the generated kernel
+   is specialised on it), e.g. a wavelength set per instance; instance i's
+   rays are generated from its own description (and seed + i) by the batched
+   trace and the batched adjoint, whose beam_desc argument must then have the
+   same type.  NULL returns to one beam for every instance.  Such batches
+   evaluate their splines per ray (no per-wavelength table).  No counterpart
+   in the reference. */
+int rt_hip_batch_set_beams(rt_hip_batch *batch, const double *beam_descs);
 int rt_hip_batch_info(const rt_hip_batch *batch, uint64_t *total_rays, uint64_t *n_tiles);
 int rt_hip_batch_set_launch(rt_hip_batch *batch, int block, int waves, int grid_mult);
 /* Trace every instance (flags: RT_TRACE_STORE optional; CCD frames are not

@@ -715,6 +715,9 @@ struct AdjArgs {
     // functions (column q at ltab[q * lt_nk ..]); generated kernels built for it
     const double* ltab;
     uint64_t lt_nk;
+    // batched adjoint with a beam per instance: n_inst records of
+    // RT_BEAM_STRIDE + 3 doubles (beam fields, then the uniform values), or null
+    const double* inst_beams;
 };
 
 // Column q of the current ray's per-wavelength table values (AdjArgs is the
@@ -1045,7 +1048,7 @@ __device__ __forceinline__ void adjoint_batch_body(const AdjArgs& a, Fwd forward
         cptr tbl = (cptr)ka->table + (uint32_t)inst * (uint32_t)ka->inst_stride;
         c.table = tbl;
         Ray ray;
-        cptr bm = (cptr)ka;
+        cptr bm = ka->inst_beams ? (cptr)ka->inst_beams + inst * (uint64_t)(RT_BEAM_STRIDE + 3) : (cptr)ka;
         uint64_t jl;
         gen_ray<BEAM>(bm, ka->seed + inst, r, ray, jl);
         c.lk = (uint32_t)jl;

@@ -389,6 +389,12 @@ extern "C" int rt_hip_batch_adjoint(const rt_hip_batch* b, const double* beam_de
     a.n_inst = (uint64_t)v.n_inst;
     a.n_tiles = v.n_tiles;
     a.inst_stride = (uint64_t)v.n_surf * rt::DEV_STRIDE;
+    if (v.beams_dev) {
+        if (beam_type != v.beam_type)
+            return rt_set_error(RT_ERR_BAD_VALUE, "beam type %d differs from the instances' beams (%d)", beam_type,
+                                v.beam_type);
+        a.inst_beams = v.beams_dev;
+    }
 
     const uint64_t nk = rt_batch_ltab_nk(b, beam_desc);
     hipFunction_t fn = nullptr;

@@ -852,7 +852,12 @@ struct TraceArgs {
     const double* ltab;
     uint64_t lt_nk;
     const uint32_t* tile_inst;    // batched traces: the instance of each tile
+    // batched traces with a beam per instance: n_inst records of kBeamRec
+    // doubles (beam fields, then the BeamUniform values), or null (the beam
+    // above for every instance)
+    const double* inst_beams;
 };
+constexpr int kBeamRec = (int)(sizeof(BeamArg) / sizeof(double));
 
 struct TraceCtx {
     cptr table;                   // device records (per-iteration copy of TraceArgs::table)
@@ -1273,7 +1278,9 @@ __device__ __forceinline__ void trace_batch_body(const TraceArgs& a, Surfaces su
         bool alive = valid;
         c.lk = 0;
         if (valid) {
-            cptr bm = (cptr)((const __attribute__((address_space(4))) char*)ka + __builtin_offsetof(TraceArgs, beam));
+            cptr bm = ka->inst_beams
+                          ? (cptr)ka->inst_beams + inst * (uint64_t)kBeamRec
+                          : (cptr)((const __attribute__((address_space(4))) char*)ka + __builtin_offsetof(TraceArgs, beam));
             uint64_t j;
             gen_ray<BEAM>(bm, ka->seed + inst, r, ray, j);
             c.lk = (uint32_t)j;

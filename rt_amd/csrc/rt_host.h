@@ -45,6 +45,8 @@ struct BatchView {
     uint64_t n_tiles;
     const double* dtables;           // host copy of the device records
     const uint64_t* ray_start;       // host copy
+    const double* beams_dev;         // a beam per instance (kBeamRec doubles each), or null
+    int beam_type;                   // their beam type (-1: none)
 };
 int rt_batch_view(const rt_hip_batch* b, BatchView* v);
 

@@ -1017,6 +1017,9 @@ struct rt_hip_batch {
     std::vector<int> descend_key;
     int* descend_dev = nullptr;      // map_start[n_surf + 1] then the (j, field) pairs
     Ltab lt;
+    // a beam per instance (rt_hip_batch_set_beams): n_inst x kBeamRec records
+    double* beams_dev = nullptr;
+    int beam_type = -1;
 };
 
 struct rt_hip_graph {
@@ -1035,6 +1038,7 @@ void free_batch(rt_hip_batch* b) {
     (void)hipFree(b->descend_dev);
     (void)hipFree(b->tile_start_dev);
     (void)hipFree(b->moments_dev);
+    (void)hipFree(b->beams_dev);
     b->lt.release();
     delete b;
 }
@@ -1181,6 +1185,38 @@ extern "C" int rt_hip_batch_update(rt_hip_batch* b, const double* tables, const 
     return ltab_setup(b->lt, b->signature, b->dtables.data(), b->n_inst, b->n_surf);
 }
 
+extern "C" int rt_hip_batch_set_beams(rt_hip_batch* b, const double* beam_descs) {
+    if (!b) return set_error(RT_ERR_BAD_VALUE, "batch is NULL");
+    if (!beam_descs) {
+        (void)hipFree(b->beams_dev);
+        b->beams_dev = nullptr;
+        b->beam_type = -1;
+        return RT_OK;
+    }
+    std::vector<double> recs((size_t)b->n_inst * rt::kBeamRec);
+    const int type = (int)beam_descs[RT_B_TYPE];
+    for (int i = 0; i < b->n_inst; ++i) {
+        const double* d = beam_descs + (size_t)i * RT_BEAM_STRIDE;
+        int rc = validate_beam(d);
+        if (rc != RT_OK) return rc;
+        if ((int)d[RT_B_TYPE] != type)
+            return set_error(RT_ERR_BAD_VALUE, "instance %d: beam type %d differs from instance 0 (%d)", i,
+                             (int)d[RT_B_TYPE], type);
+        TraceArgs a;
+        memset(&a, 0, sizeof a);
+        memcpy(a.beam.v, d, sizeof a.beam.v);
+        beam_uniforms(a);
+        memcpy(recs.data() + (size_t)i * rt::kBeamRec, &a.beam, sizeof a.beam);
+    }
+    int rc = ensure_device();
+    if (rc != RT_OK) return rc;
+    if (!b->beams_dev) HIP_TRY(hipMalloc(&b->beams_dev, recs.size() * sizeof(double)), RT_ERR_BAD_ALLOC);
+    HIP_TRY(hipMemcpy(b->beams_dev, recs.data(), recs.size() * sizeof(double), hipMemcpyHostToDevice),
+            RT_ERR_FAILED);
+    b->beam_type = type;
+    return RT_OK;
+}
+
 extern "C" int rt_hip_batch_destroy(rt_hip_batch* b) {
     free_batch(b);
     return RT_OK;
@@ -1250,7 +1286,13 @@ extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_des
     const size_t lds = b->pool_len * sizeof(double);
     hipStream_t st = (hipStream_t)stream;
     const int beam_mode = (int)a.beam.v[RT_B_TYPE];
-    const uint64_t nk = ltab_nk(b->lt, beam_desc);
+    if (b->beams_dev) {
+        if (beam_mode != b->beam_type)
+            return set_error(RT_ERR_BAD_VALUE, "beam type %d differs from the instances' beams (%d)", beam_mode,
+                             b->beam_type);
+        a.inst_beams = b->beams_dev;
+    }
+    const uint64_t nk = b->beams_dev ? 0 : ltab_nk(b->lt, beam_desc);
     hipFunction_t fn = nullptr;
     if (rt_jit_enabled())
         fn = rt_jit_kernel(b->signature, beam_mode | (nk ? RT_JIT_LTAB : 0), false, b->block, b->waves, b->device,
@@ -1317,7 +1359,8 @@ extern "C" int rt_hip_batch_graph_create(const rt_hip_batch* b, const double* be
     if (rt_jit_enabled()) {
         int rc = validate_beam(beam_desc);
         if (rc != RT_OK) return rc;
-        (void)rt_jit_kernel(b->signature, (int)beam_desc[RT_B_TYPE] | (ltab_nk(b->lt, beam_desc) ? RT_JIT_LTAB : 0),
+        (void)rt_jit_kernel(b->signature,
+                            (int)beam_desc[RT_B_TYPE] | (rt_batch_ltab_nk(b, beam_desc) ? RT_JIT_LTAB : 0),
                             false, b->block, b->waves, b->device, true);
     }
     rt_hip_graph* g = new (std::nothrow) rt_hip_graph();
@@ -1370,6 +1413,8 @@ int rt_batch_view(const rt_hip_batch* b, BatchView* v) {
     v->ray_start_dev = b->ray_start_dev;
     v->tile_start_dev = b->tile_start_dev;
     v->tile_inst_dev = b->tile_inst_dev;
+    v->beams_dev = b->beams_dev;
+    v->beam_type = b->beam_type;
     v->n_tiles = b->n_tiles;
     v->dtables = b->dtables.data();
     v->ray_start = b->ray_start.data();
@@ -1389,7 +1434,9 @@ int rt_scheme_ltab_fill(const rt_hip_scheme* s, const double* beam_desc, uint64_
     return ltab_fill(s->lt, a, s->pool_dev, nk, stream);
 }
 
-uint64_t rt_batch_ltab_nk(const rt_hip_batch* b, const double* beam_desc) { return ltab_nk(b->lt, beam_desc); }
+uint64_t rt_batch_ltab_nk(const rt_hip_batch* b, const double* beam_desc) {
+    return b->beams_dev ? 0 : ltab_nk(b->lt, beam_desc);
+}
 
 int rt_batch_ltab_fill(const rt_hip_batch* b, const double* beam_desc, uint64_t nk, hipStream_t stream,
                        const double** tab) {

@@ -53,6 +53,7 @@ def lib():
                                           C.POINTER(_vp)]
         L.rt_hip_batch_update.argtypes = [_vp, _dp, _dp, C.c_size_t]
         L.rt_hip_batch_destroy.argtypes = [_vp]
+        L.rt_hip_batch_set_beams.argtypes = [_vp, _dp]
         L.rt_hip_batch_info.argtypes = [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.rt_hip_batch_set_launch.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
         L.rt_hip_batch_trace.argtypes = [_vp, _dp, C.c_uint64, C.c_int] + [_vp] * 8 + [_vp, _vp, _vp]
@@ -343,6 +344,17 @@ class DeviceBatch:
         n, t = C.c_uint64(), C.c_uint64()
         check(lib().rt_hip_batch_info(self.handle, C.byref(n), C.byref(t)), "rt_hip_batch_info")
         return n.value, t.value
+
+    def set_beams(self, beams):
+        """rt_hip_batch_set_beams: a Beam per instance (one beam type), or None
+        for the first instance's beam everywhere."""
+        if beams is None:
+            self._beams = None
+            check(lib().rt_hip_batch_set_beams(self.handle, None), "rt_hip_batch_set_beams")
+            return
+        assert len(beams) == self.n_inst
+        self._beams = np.ascontiguousarray(np.concatenate([b.pack() for b in beams]))
+        check(lib().rt_hip_batch_set_beams(self.handle, _dptr(self._beams)), "rt_hip_batch_set_beams")
 
     def _args(self, beam, rays, flag):
         b = np.ascontiguousarray((beam if beam is not None else self.instances[0].beam).pack())

@@ -142,3 +142,50 @@ def test_batch_graph_replays_match_eager(hip):
         torch.cuda.synchronize()
     g.close()
     db.close()
+
+
+def test_batch_with_a_beam_per_instance(hip):
+    """rt_hip_batch_set_beams: every instance its own wavelength set (range and
+    number of wavelengths) in one launch; instance i equals the oracle's trace
+    of that instance with its own beam and seed + i (same bars)."""
+    import copy
+    inst, counts = make_batch(n_inst=5, seed=8)
+    beams = []
+    for i, sch in enumerate(inst):
+        b = copy.deepcopy(sch.beam)
+        b.range = (0.40 + 0.03 * i, 0.62 + 0.05 * i)
+        b.nlambda = (8, 16, 32, 64, 5)[i]
+        beams.append(b)
+    seed = 12
+    db = hip.DeviceBatch(inst, counts)
+    db.set_beams(beams)
+    total = int(sum(counts))
+    R = torch.zeros((7, total), dtype=torch.float64, device="cuda")
+    F = torch.zeros(total, dtype=torch.int8, device="cuda")
+    print(f"[batch beams] n_inst={len(inst)} rays={total}", flush=True)
+    db.trace(seed=seed, flags=S.RT_TRACE_STORE, rays=R, flag=F)
+    torch.cuda.synchronize()
+    rays, flag = R.cpu().numpy(), F.cpu().numpy()
+    t = np.asarray(inst[0].table).reshape(inst[0].n_surf, S.SURF_STRIDE)
+    L = max(1.0, float(np.max(np.abs(t[:, S.S_DIST:S.S_DIST + 3]))))
+    floor = np.array([L, L, L, 1.0, 1.0, 1.0, 0.0])[:, None]
+    off = 0
+    for i, (sch, n) in enumerate(zip(inst, counts)):
+        own = copy.deepcopy(sch)
+        own.beam = beams[i]
+        ref = O.trace(own, seed=seed + i, n=n)
+        gr, gf = rays[:, off:off + n], flag[off:off + n]
+        assert np.array_equal(gr[6], ref.rays[6]), f"instance {i}: wavelengths differ"
+        assert np.array_equal(gf, ref.flag), f"instance {i}: flags differ"
+        ok = ref.flag == 1
+        if ok.any():
+            scale = np.maximum(np.maximum(np.abs(gr), np.abs(ref.rays)), floor)
+            assert float(np.max((np.abs(gr - ref.rays) / scale)[:, ok])) <= RTOL, i
+        off += n
+    # back to one beam: the first instance's for all
+    db.set_beams(None)
+    db.trace(seed=seed, flags=S.RT_TRACE_STORE, rays=R, flag=F)
+    torch.cuda.synchronize()
+    ref0 = O.trace(inst[1], seed=seed + 1, n=counts[1])
+    assert np.array_equal(R.cpu().numpy()[6, counts[0]:counts[0] + counts[1]], ref0.rays[6])
+    db.close()

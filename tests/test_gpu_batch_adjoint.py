@@ -229,3 +229,30 @@ def test_adjoint_table_matches_per_ray_splines(hip):
     finally:
         hip.set_jit(prev_jit)
     db.close()
+
+
+def test_batch_adjoint_with_a_beam_per_instance(hip):
+    """The batched adjoint of instances with their own wavelength sets
+    (rt_hip_batch_set_beams) equals the single-scheme adjoint of each instance
+    with its own beam, within 1e-11 of the instance's largest component."""
+    import copy
+    print("[batch adjoint beams]", flush=True)
+    base, inst, counts, ps, pf, db = setup_batch(hip, n_inst=3, seed=31)
+    beams = []
+    for i, sch in enumerate(inst):
+        b = copy.deepcopy(sch.beam)
+        b.range = (0.45 + 0.05 * i, 0.65 + 0.05 * i)
+        beams.append(b)
+    db.set_beams(beams)
+    gen = torch.Generator().manual_seed(6)
+    seeds = (torch.rand((len(inst), G, 3), generator=gen, dtype=torch.float64) * 1e-3).cuda()
+    gb = db.adjoint(seeds, ps, pf, seed=5).cpu().numpy()
+    for i in range(len(inst)):
+        own = copy.deepcopy(inst[i])
+        own.beam = beams[i]
+        ds = hip.DeviceScheme(own)
+        g1 = ds.adjoint(seeds[i].contiguous(), ps, pf, seed=5 + i, n=int(counts[i])).cpu().numpy()
+        ds.close()
+        scale = max(float(np.abs(g1).max()), 1e-300)
+        assert np.abs(gb[i] - g1).max() <= 1e-11 * scale, i
+    db.close()
