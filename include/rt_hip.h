@@ -332,7 +332,8 @@ int rt_hip_batch_destroy(rt_hip_batch *batch);
    is specialised on it), e.g. a wavelength set per instance; instance i's
    rays are generated from its own description (and seed + i) by the batched
    trace and the batched adjoint, whose beam_desc argument must then have the
-   same type.  NULL returns to one beam for every instance.  Such batches
+   same type.  NULL returns to one beam for every instance (a graph captured
+   before keeps reading the beams it was captured with).  Such batches
    evaluate their splines per ray (no per-wavelength table).  No counterpart
    in the reference. */
 int rt_hip_batch_set_beams(rt_hip_batch *batch, const double *beam_descs);

@@ -1017,8 +1017,11 @@ struct rt_hip_batch {
     std::vector<int> descend_key;
     int* descend_dev = nullptr;      // map_start[n_surf + 1] then the (j, field) pairs
     Ltab lt;
-    // a beam per instance (rt_hip_batch_set_beams): n_inst x kBeamRec records
+    // a beam per instance (rt_hip_batch_set_beams): n_inst x kBeamRec records,
+    // used while beams_on (kept allocated until the batch is destroyed: a
+    // captured graph may hold its address)
     double* beams_dev = nullptr;
+    bool beams_on = false;
     int beam_type = -1;
 };
 
@@ -1188,8 +1191,7 @@ extern "C" int rt_hip_batch_update(rt_hip_batch* b, const double* tables, const 
 extern "C" int rt_hip_batch_set_beams(rt_hip_batch* b, const double* beam_descs) {
     if (!b) return set_error(RT_ERR_BAD_VALUE, "batch is NULL");
     if (!beam_descs) {
-        (void)hipFree(b->beams_dev);
-        b->beams_dev = nullptr;
+        b->beams_on = false;
         b->beam_type = -1;
         return RT_OK;
     }
@@ -1214,6 +1216,7 @@ extern "C" int rt_hip_batch_set_beams(rt_hip_batch* b, const double* beam_descs)
     HIP_TRY(hipMemcpy(b->beams_dev, recs.data(), recs.size() * sizeof(double), hipMemcpyHostToDevice),
             RT_ERR_FAILED);
     b->beam_type = type;
+    b->beams_on = true;
     return RT_OK;
 }
 
@@ -1286,13 +1289,13 @@ extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_des
     const size_t lds = b->pool_len * sizeof(double);
     hipStream_t st = (hipStream_t)stream;
     const int beam_mode = (int)a.beam.v[RT_B_TYPE];
-    if (b->beams_dev) {
+    if (b->beams_on) {
         if (beam_mode != b->beam_type)
             return set_error(RT_ERR_BAD_VALUE, "beam type %d differs from the instances' beams (%d)", beam_mode,
                              b->beam_type);
         a.inst_beams = b->beams_dev;
     }
-    const uint64_t nk = b->beams_dev ? 0 : ltab_nk(b->lt, beam_desc);
+    const uint64_t nk = b->beams_on ? 0 : ltab_nk(b->lt, beam_desc);
     hipFunction_t fn = nullptr;
     if (rt_jit_enabled())
         fn = rt_jit_kernel(b->signature, beam_mode | (nk ? RT_JIT_LTAB : 0), false, b->block, b->waves, b->device,
@@ -1413,7 +1416,7 @@ int rt_batch_view(const rt_hip_batch* b, BatchView* v) {
     v->ray_start_dev = b->ray_start_dev;
     v->tile_start_dev = b->tile_start_dev;
     v->tile_inst_dev = b->tile_inst_dev;
-    v->beams_dev = b->beams_dev;
+    v->beams_dev = b->beams_on ? b->beams_dev : nullptr;
     v->beam_type = b->beam_type;
     v->n_tiles = b->n_tiles;
     v->dtables = b->dtables.data();
@@ -1435,7 +1438,7 @@ int rt_scheme_ltab_fill(const rt_hip_scheme* s, const double* beam_desc, uint64_
 }
 
 uint64_t rt_batch_ltab_nk(const rt_hip_batch* b, const double* beam_desc) {
-    return b->beams_dev ? 0 : ltab_nk(b->lt, beam_desc);
+    return b->beams_on ? 0 : ltab_nk(b->lt, beam_desc);
 }
 
 int rt_batch_ltab_fill(const rt_hip_batch* b, const double* beam_desc, uint64_t nk, hipStream_t stream,
