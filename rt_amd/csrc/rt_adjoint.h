@@ -793,7 +793,8 @@ __device__ __forceinline__ void fwd_step(const SD& d, int s, Ray& r, bool& alive
         t.st(k0 + 3, r.cX); t.st(k0 + 4, r.cY); t.st(k0 + 5, r.cZ);
     }
     surface_step(d, s, r, alive, c, lv, col);
-    if (alive) t.st(k0 + 6, (double)c.last_it);
+    // the Newton count (row 6) only where the intersection iterates
+    if ((d.ikind() == IK_CONIC_NEWTON || d.ikind() == IK_TORIC_NEWTON) && alive) t.st(k0 + 6, (double)c.last_it);
 }
 
 // backward through one surface: gp receives the surface's parameter
@@ -810,7 +811,7 @@ __device__ __forceinline__ bool bwd_step(const SD& d, const Ray& r, St& ad, cons
 RT_UNROLL
     for (int k = 0; k < PG_N; ++k) gp[k] = 0.0;
     const St entry = {t.ld(k0 + 0), t.ld(k0 + 1), t.ld(k0 + 2), t.ld(k0 + 3), t.ld(k0 + 4), t.ld(k0 + 5)};
-    const int nit = (int)t.ld(k0 + 6);
+    const int nit = (d.ikind() == IK_CONIC_NEWTON || d.ikind() == IK_TORIC_NEWTON) ? (int)t.ld(k0 + 6) : 0;
     // mu and L exactly as the forward pass formed them
     double mu = 0.0, L = 0.0;
     if (d.type() == RT_LENS || d.type() == RT_GRATING) {

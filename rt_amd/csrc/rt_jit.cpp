@@ -274,12 +274,18 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
        << "    [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, const rt::adj::LaneRows& t) {\n";
     if (use_tab) os << "      double lv[" << nc << "];\n";
     for (int s = 0; s < n_surf; ++s) {
+        // refractive-index columns only: the adjoint accumulates no flux, so
+        // the detector's QE and spectrum are never read
+        bool tab_s = false;
         if (use_tab)
             for (size_t q = 0; q < nc; ++q)
-                if (col_surf[q] == s) os << "      rt::adj::load_lval_adj(lv, c, " << q << ");\n";
+                if (col_surf[q] == s && col_role[q] <= 1) {
+                    os << "      rt::adj::load_lval_adj(lv, c, " << q << ");\n";
+                    tab_s = true;
+                }
         os << "      rt::adj::fwd_step(" << types[s] << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s
            << ", r, alive, c, t, " << s * rt::adj::kTape;
-        if (use_tab && col_base[(size_t)s] >= 0) os << ", lv, " << col_base[(size_t)s];
+        if (tab_s) os << ", lv, " << col_base[(size_t)s];
         os << ");\n";
     }
     os << "    },\n"

@@ -123,11 +123,12 @@ def test_generated_adjoint_kernel_compiles():
     assert text.count("rt::adj::fwd_step(") == sch.n_surf
     assert text.count("rt::adj::bwd_step(") == sch.n_surf
     # reading the per-wavelength table (0x400): the three glass columns in the
-    # forward (plus the detector QE) and n1/n2 of the six lens surfaces in the
-    # backward, where a back surface's n1 is its front surface's n2 column
+    # forward (not the detector QE: no flux in the adjoint) and n1/n2 of the
+    # six lens surfaces in the backward, where a back surface's n1 is its
+    # front surface's n2 column
     rc, size, text = lib.jit_check(sch, S.RT_BEAM_GAUSS | 0x200 | 0x400)
     assert rc == 0, text[-2000:]
-    assert text.count("rt::adj::load_lval_adj(lv, c, ") == 4
+    assert text.count("rt::adj::load_lval_adj(lv, c, ") == 3
     assert text.count("rt::adj::load_lval_adj(lb, c, ") == 6
 
 
