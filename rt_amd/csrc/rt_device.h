@@ -198,23 +198,33 @@ __device__ __forceinline__ double fsqrt_pos(double x, double ry) {
 // __kernel_cos (Sun Microsystems, 1993; coefficients checked against their
 // published bit patterns), and the quadrant k mod 4.  Within 1.5 ulp of the
 // correctly rounded values; no large-argument path, so few registers.
+// A constant operand made at its use: an empty asm keeps the compiler from
+// hoisting it out of the ray loop, where a polynomial's coefficients would
+// otherwise hold about 20 registers for the whole loop (and push the trace
+// into scratch at 6 waves per SIMD); it is rematerialised in scalar registers
+// (scalar moves, issued beside the vector work).
+__device__ __forceinline__ double kconst(double v) {
+    __asm__ volatile("" : "+s"(v));
+    return v;
+}
+
 __device__ __forceinline__ void sincos_bounded(double x, double& s_out, double& c_out) {
-    const double kd = rint(x * 6.36619772367581382433e-01);          // 2/pi
+    const double kd = rint(x * kconst(6.36619772367581382433e-01));          // 2/pi
     const int k = (int)kd;
-    const double r = fma(-kd, 6.07710050650619224932e-11, fma(-kd, 1.57079632673412561417e+00, x));
+    const double r = fma(-kd, kconst(6.07710050650619224932e-11), fma(-kd, kconst(1.57079632673412561417e+00), x));
     const double z = r * r;
     // __kernel_sin: r + r^3 (S1 + z (S2 + ... + z^4 S6))
-    const double ps = fma(z, fma(z, fma(z, fma(z, 1.58969099521155010221e-10, -2.50507602534068634195e-08),
-                                        2.75573137070700676789e-06),
-                                 -1.98412698298579493134e-04),
-                          8.33333333332248946124e-03);
-    const double sr = fma(z * r, fma(z, ps, -1.66666666666666324348e-01), r);
+    const double ps = fma(z, fma(z, fma(z, fma(z, kconst(1.58969099521155010221e-10), kconst(-2.50507602534068634195e-08)),
+                                        kconst(2.75573137070700676789e-06)),
+                                 kconst(-1.98412698298579493134e-04)),
+                          kconst(8.33333333332248946124e-03));
+    const double sr = fma(z * r, fma(z, ps, kconst(-1.66666666666666324348e-01)), r);
     // __kernel_cos: w + (((1 - w) - z/2) + z^2 (C1 + z (C2 + ... + z^5 C6))), w = 1 - z/2
-    const double pc = z * fma(z, fma(z, fma(z, fma(z, fma(z, -1.13596475577881948265e-11, 2.08757232129817482790e-09),
-                                                    -2.75573143513906633035e-07),
-                                             2.48015872894767294178e-05),
-                                      -1.38888888888741095749e-03),
-                               4.16666666666666019037e-02);
+    const double pc = z * fma(z, fma(z, fma(z, fma(z, fma(z, kconst(-1.13596475577881948265e-11), kconst(2.08757232129817482790e-09)),
+                                                    kconst(-2.75573143513906633035e-07)),
+                                             kconst(2.48015872894767294178e-05)),
+                                      kconst(-1.38888888888741095749e-03)),
+                               kconst(4.16666666666666019037e-02));
     const double hz = 0.5 * z, w = 1.0 - hz;
     const double cr = w + (((1.0 - w) - hz) + z * pc);
     // quadrant: (s, c) = (sr, cr), (cr, -sr), (-sr, -cr), (-cr, sr) for k mod 4 = 0..3
