@@ -235,6 +235,39 @@ __device__ __forceinline__ void sincos_bounded(double x, double& s_out, double& 
     c_out = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, c0) ^ sgn_c);
 }
 
+// Natural logarithm of a normal x in (0, 1] (the Box-Muller radius): fdlibm's
+// __ieee754_log (Sun Microsystems, 1993): x = 2^k (1 + f) with 1 + f in
+// [sqrt(2)/2, sqrt(2)), s = f / (2 + f), log(1 + f) = f - s (f - R(s^2))
+// with the minimax polynomial R of coefficients Lg1..Lg7 (checked against
+// their published bit patterns), plus k ln 2 in two parts.  Within 1 ulp of
+// the correctly rounded value; log(1) = 0.  Coefficients rematerialised at
+// use (kconst).
+__device__ __forceinline__ double log_unit(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    int hx = (int)(uint32_t)(u >> 32);
+    int k = (hx >> 20) - 1023;
+    hx &= 0x000fffff;
+    const int i = (hx + 0x95f64) & 0x100000;
+    const double xr = __builtin_bit_cast(double, ((uint64_t)(uint32_t)(hx | (i ^ 0x3ff00000)) << 32) | (uint32_t)u);
+    k += i >> 20;
+    const double f = xr - 1.0;
+    const double s = fdiv_fin(f, 2.0 + f);
+    const double dk = (double)k;
+    const double z = s * s, w = z * z;
+    const double t1 = w * fma(w, fma(w, kconst(1.531383769920937332e-01), kconst(2.222219843214978396e-01)),
+                              kconst(3.999999999940941908e-01));
+    const double t2 = z * fma(w, fma(w, fma(w, kconst(1.479819860511658591e-01), kconst(1.818357216161805012e-01)),
+                                     kconst(2.857142874366239149e-01)),
+                              kconst(6.666666666666735130e-01));
+    const double R = t2 + t1;
+    const double ln2_hi = kconst(6.93147180369123816490e-01), ln2_lo = kconst(1.90821492927058770002e-10);
+    if (((hx - 0x6147a) | (0x6b851 - hx)) > 0) {
+        const double hfsq = 0.5 * f * f;
+        return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+    }
+    return dk * ln2_hi - ((s * (f - R) - dk * ln2_lo) - f);
+}
+
 // ---------------------------------------------------------------- Philox
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -322,7 +355,7 @@ __device__ __forceinline__ void gen_direction(cptr B, uint64_t seed, uint64_t r,
         philox_stream(seed, r, 1u, o);
         double u3 = u01(o[0], o[1]);
         double u4 = u01(o[2], o[3]);
-        double rad = fsqrt(-2.0 * log(u3));
+        double rad = fsqrt(-2.0 * log_unit(u3));
         double sb, cb;
         sincos_bounded(kTwoPi * u4, sb, cb);
         double tX = tg * (rad * cb);
