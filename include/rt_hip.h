@@ -16,7 +16,11 @@
  *   - All floating-point data is FP64.  A ray is alive while flag != 0;
  *     the engine only ever writes flag = 0 (rt_engine_func.h semantics).
  *   - No function throws; no function keeps per-call global state, so
- *     distinct schemes may be traced concurrently on distinct streams.
+ *     distinct schemes may be traced concurrently on distinct streams.  A
+ *     scheme (or batch) object holds working buffers of its traces (the
+ *     per-wavelength table, the copies its kernels add statistics to before
+ *     they are folded into the caller's counters): the traces of one scheme
+ *     run in the order of one stream.
  *
  * Packed scheme table
  *   A scheme is `n_surf` records of RT_SURF_STRIDE doubles (layout: enum

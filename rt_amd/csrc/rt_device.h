@@ -901,6 +901,23 @@ struct TraceArgs {
     const double* inst_beams;
 };
 constexpr int kBeamRec = (int)(sizeof(BeamArg) / sizeof(double));
+// copies of the statistics counters the trace kernels add to (TraceArgs::stats
+// points at kStatCopies x RT_STATS_LEN counters, workgroup b adding to copy
+// b % kStatCopies)
+constexpr int kStatCopies = 64;
+
+// stats[k] += sum of the copies' counter k, and the copies cleared for the
+// next launch (one workgroup)
+__device__ __forceinline__ void stats_fold_body(unsigned long long* copies, unsigned long long* stats) {
+    for (int k = threadIdx.x; k < RT_STATS_LEN; k += blockDim.x) {
+        unsigned long long t = 0;
+        for (int c = 0; c < kStatCopies; ++c) {
+            t += copies[(size_t)c * RT_STATS_LEN + k];
+            copies[(size_t)c * RT_STATS_LEN + k] = 0;
+        }
+        stats[k] += t;
+    }
+}
 
 struct TraceCtx {
     cptr table;                   // device records (per-iteration copy of TraceArgs::table)
@@ -1205,8 +1222,12 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
     lds_add(&s_stats[RT_ST_CCD_EDGE], c.n_edge);
     __syncthreads();
     if (a.stats) {
+        // one of kStatCopies copies of the counters per workgroup (folded
+        // afterwards): thousands of workgroups adding to the same few lines at
+        // the end of a launch would serialise on them
+        unsigned long long* dst = a.stats + (size_t)(blockIdx.x % kStatCopies) * RT_STATS_LEN;
         for (int k = threadIdx.x; k < RT_STATS_LEN; k += blockDim.x)
-            if (s_stats[k]) atomicAdd(&a.stats[k], (unsigned long long)s_stats[k]);
+            if (s_stats[k]) atomicAdd(&dst[k], (unsigned long long)s_stats[k]);
     }
 }
 
@@ -1362,8 +1383,12 @@ __device__ __forceinline__ void trace_batch_body(const TraceArgs& a, Surfaces su
     lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
     __syncthreads();
     if (a.stats) {
+        // one of kStatCopies copies of the counters per workgroup (folded
+        // afterwards): thousands of workgroups adding to the same few lines at
+        // the end of a launch would serialise on them
+        unsigned long long* dst = a.stats + (size_t)(blockIdx.x % kStatCopies) * RT_STATS_LEN;
         for (int k = threadIdx.x; k < RT_STATS_LEN; k += blockDim.x)
-            if (s_stats[k]) atomicAdd(&a.stats[k], (unsigned long long)s_stats[k]);
+            if (s_stats[k]) atomicAdd(&dst[k], (unsigned long long)s_stats[k]);
     }
 }
 

@@ -377,6 +377,9 @@ struct rt_hip_scheme {
     // kernel); 0, a runtime loop over the records (the generic kernel)
     int unroll = 1;
     Ltab lt;
+    // kStatCopies x RT_STATS_LEN counters the trace kernels add to, folded into
+    // the caller's statistics after each trace (and cleared by the fold)
+    unsigned long long* stat_copies = nullptr;
 };
 
 namespace {
@@ -553,6 +556,10 @@ static void build_device_tables(rt_hip_scheme* s) {
 
 __global__ __launch_bounds__(256) void rt_ltab_kernel(rt::LtabArgs a) { rt::ltab_body(a); }
 
+__global__ __launch_bounds__(256) void rt_stats_fold_kernel(unsigned long long* copies, unsigned long long* stats) {
+    rt::stats_fold_body(copies, stats);
+}
+
 namespace {
 
 // The table's layout from the signature and its column offsets from the
@@ -576,6 +583,21 @@ int ltab_setup(Ltab& t, const std::string& sig, const double* dtables, int n_ins
     if (!t.tab_dev) HIP_TRY(hipMalloc(&t.tab_dev, nc * kLtabMax * sizeof(double)), RT_ERR_BAD_ALLOC);
     HIP_TRY(hipMemcpy(t.off_dev, off.data(), nc * sizeof(double), hipMemcpyHostToDevice), RT_ERR_FAILED);
     t.ok = true;
+    return RT_OK;
+}
+
+// the zeroed counter copies of a scheme or batch
+int stat_copies_alloc(unsigned long long** p) {
+    const size_t bytes = (size_t)rt::kStatCopies * RT_STATS_LEN * sizeof(unsigned long long);
+    HIP_TRY(hipMalloc(p, bytes), RT_ERR_BAD_ALLOC);
+    HIP_TRY(hipMemset(*p, 0, bytes), RT_ERR_FAILED);
+    return RT_OK;
+}
+
+// stats += the copies (cleared), on `st` after the trace
+int stats_fold(unsigned long long* copies, unsigned long long* stats, hipStream_t st) {
+    hipLaunchKernelGGL(rt_stats_fold_kernel, dim3(1), dim3(256), 0, st, copies, stats);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     return RT_OK;
 }
 
@@ -643,7 +665,9 @@ extern "C" int rt_hip_scheme_create(const double* surf_table, int n_surf, const 
         return set_error(RT_ERR_FAILED, "upload of scheme tables failed");
     }
     rc = ltab_setup(s->lt, s->signature, s->dtable.data(), 1, n_surf);
+    if (rc == RT_OK) rc = stat_copies_alloc(&s->stat_copies);
     if (rc != RT_OK) {
+        (void)hipFree(s->stat_copies);
         s->lt.release();
         (void)hipFree(s->table_dev);
         (void)hipFree(s->pool_dev);
@@ -672,6 +696,7 @@ extern "C" int rt_hip_jit_check(const double* surf_table, int n_surf, const doub
 
 extern "C" int rt_hip_scheme_destroy(rt_hip_scheme* s) {
     if (!s) return RT_OK;
+    (void)hipFree(s->stat_copies);
     s->lt.release();
     (void)hipFree(s->table_dev);
     (void)hipFree(s->pool_dev);
@@ -743,7 +768,8 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
     a.ray_begin = ray_begin;
     a.n = n_rays;
     a.X = X; a.Y = Y; a.Z = Z; a.cX = cX; a.cY = cY; a.cZ = cZ; a.L = lambda; a.F = flag;
-    a.hist = ccd_hist; a.flux = ccd_flux; a.stats = stats; a.dump = dump; a.dump_flag = dump_flag;
+    a.hist = ccd_hist; a.flux = ccd_flux; a.dump = dump; a.dump_flag = dump_flag;
+    a.stats = stats ? sch->stat_copies : nullptr;
 
     const int cus = rt_device_cus();
     size_t lds = sch->pool_len * sizeof(double);
@@ -787,6 +813,7 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
         g_last_kernel = "generic";
     }
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    if (stats) return stats_fold(sch->stat_copies, stats, st);
     return RT_OK;
 }
 
@@ -1017,6 +1044,7 @@ struct rt_hip_batch {
     std::vector<int> descend_key;
     int* descend_dev = nullptr;      // map_start[n_surf + 1] then the (j, field) pairs
     Ltab lt;
+    unsigned long long* stat_copies = nullptr;   // as rt_hip_scheme::stat_copies
     // a beam per instance (rt_hip_batch_set_beams): n_inst x kBeamRec records,
     // used while beams_on (kept allocated until the batch is destroyed: a
     // captured graph may hold its address)
@@ -1042,6 +1070,7 @@ void free_batch(rt_hip_batch* b) {
     (void)hipFree(b->tile_start_dev);
     (void)hipFree(b->moments_dev);
     (void)hipFree(b->beams_dev);
+    (void)hipFree(b->stat_copies);
     b->lt.release();
     delete b;
 }
@@ -1161,6 +1190,7 @@ extern "C" int rt_hip_batch_create(const double* tables, int n_inst, int n_surf,
     }
     rc = prepare_tiles(b);
     if (rc == RT_OK) rc = ltab_setup(b->lt, b->signature, b->dtables.data(), b->n_inst, b->n_surf);
+    if (rc == RT_OK) rc = stat_copies_alloc(&b->stat_copies);
     if (rc != RT_OK) { free_batch(b); return rc; }
     *out = b;
     return RT_OK;
@@ -1276,7 +1306,7 @@ extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_des
     if (flags & RT_TRACE_STORE) {
         a.X = X; a.Y = Y; a.Z = Z; a.cX = cX; a.cY = cY; a.cZ = cZ; a.L = lambda; a.F = flag;
     }
-    a.stats = stats;
+    a.stats = stats ? b->stat_copies : nullptr;
     a.tile_start = b->tile_start_dev;
     a.tile_inst = b->tile_inst_dev;
     a.ray_start = b->ray_start_dev;
@@ -1337,6 +1367,10 @@ extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_des
         g_last_kernel = "generic";
     }
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    if (stats) {
+        rc = stats_fold(b->stat_copies, stats, st);
+        if (rc != RT_OK) return rc;
+    }
     if (spots) {
         unsigned g = (unsigned)((b->n_inst + 255) / 256);
         hipLaunchKernelGGL(rt_batch_spots_kernel, dim3(g), dim3(256), 0, st, (const double*)b->moments_dev,
