@@ -129,16 +129,42 @@ __global__ __launch_bounds__(256) void rt_adjoint_reduce_chunks(const double* ga
     if (threadIdx.x == 0) part[j * kReduceChunks + b] = v;
 }
 
-// grad[i][j] = sum over instance i's tile-waves of part[tw][j] (blockIdx.x =
-// i * n_par + j), fixed order
+// grad[i][j] = sum over instance i's tile-waves tw of part[tw][j], one
+// workgroup per instance (blockIdx.x = i), fixed order. The columns are taken
+// in chunks of C = min(n_par, 256); lane k adds column c0 + k % C of rows
+// lo + k / C, lo + k / C + R, ... (R = 256 / C rows per pass, so a pass reads
+// R whole rows, coalesced) into 8 sums, row m into sum m mod 8 (8 loads in
+// flight per lane: one workgroup reads the largest instance's rows), the 8
+// sums are added as a fixed tree, then the R partial sums of each column in
+// row order.
 __global__ __launch_bounds__(256) void rt_adjoint_batch_reduce(const double* part, const uint64_t* tile_start,
                                                               uint64_t n_par, uint64_t wpg, double* grad) {
-    const uint64_t ij = blockIdx.x, i = ij / n_par, j = ij - i * n_par;
+    __shared__ double s[256];
+    const uint64_t i = blockIdx.x, k = threadIdx.x;
     const uint64_t lo = tile_start[i] * wpg, hi = tile_start[i + 1] * wpg;
-    double v = 0.0;
-    for (uint64_t tw = lo + threadIdx.x; tw < hi; tw += blockDim.x) v += part[tw * n_par + j];
-    v = block_sum_256(v);
-    if (threadIdx.x == 0) grad[ij] = v;
+    for (uint64_t c0 = 0; c0 < n_par; c0 += 256) {
+        const uint64_t C = n_par - c0 < 256 ? n_par - c0 : 256, R = 256 / C;
+        const uint64_t col = c0 + k % C, r0 = k / C;
+        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (r0 < R) {
+            uint64_t tw = lo + r0;
+            for (; tw + 7 * R < hi; tw += 8 * R) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc[u] += part[(tw + u * R) * n_par + col];
+            }
+#pragma unroll
+            for (int u = 0; u < 7; ++u)
+                if (tw + u * R < hi) acc[u] += part[(tw + u * R) * n_par + col];
+        }
+        s[k] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        __syncthreads();
+        if (k < C) {
+            double t = 0.0;
+            for (uint64_t r = 0; r < R; ++r) t += s[k + r * C];
+            grad[i * n_par + col] = t;
+        }
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(256) void rt_adjoint_reduce_parts(const double* part, double* grad) {
@@ -234,6 +260,18 @@ int adjoint_waves() {
     return w ? atoi(w) : 3;
 }
 
+// workgroups launched per resident slot (the persistent grid is CUs x
+// resident x this, within the tape's 2 GiB buffer): 16 for batches, where the
+// hardware's dealing of the extra workgroups balances instances of uneven
+// cost (1024 echelle instances: 95.2 ms at 1, 85.7 at 8, 84.1 at 16), 1 for
+// one scheme, where more gains nothing and the tape stays small;
+// RT_HIP_ADJOINT_GRID_MULT overrides both
+uint64_t adjoint_grid_mult(uint64_t dflt) {
+    const char* m = getenv("RT_HIP_ADJOINT_GRID_MULT");
+    const int v = m ? atoi(m) : (int)dflt;
+    return v >= 1 && v <= 64 ? (uint64_t)v : dflt;
+}
+
 }  // namespace
 
 extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc, uint64_t seed, size_t n_rays,
@@ -287,7 +325,7 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, kAdjBlock, lds);
     if (oe != hipSuccess || resident < 1) resident = 1;
     uint64_t want = (n_rays + kAdjBlock - 1) / kAdjBlock;
-    uint64_t cap = (uint64_t)rt_device_cus() * resident;
+    uint64_t cap = (uint64_t)rt_device_cus() * resident * adjoint_grid_mult(1);
     // the tape (n_surf * kTape rows) and the accumulators (n_params rows) of
     // G lanes are addressed as buffer resources of at most 2^31 - 1 bytes
     const uint64_t rows_max = std::max<uint64_t>((uint64_t)n_surf * kTape, (uint64_t)std::max(n_params, 1));
@@ -409,7 +447,7 @@ extern "C" int rt_hip_batch_adjoint(const rt_hip_batch* b, const double* beam_de
     hipError_t oe = fn ? hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, kAdjBlock, lds)
                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, kAdjBlock, lds);
     if (oe != hipSuccess || resident < 1) resident = 1;
-    uint64_t cap = (uint64_t)rt_device_cus() * resident;
+    uint64_t cap = (uint64_t)rt_device_cus() * resident * adjoint_grid_mult(16);
     const uint64_t lanes_max = 0x7fffffffull / ((uint64_t)v.n_surf * kTape * sizeof(double));
     if (cap > lanes_max / kAdjBlock) cap = lanes_max / kAdjBlock;
     const unsigned grid = (unsigned)(v.n_tiles < cap ? v.n_tiles : cap);
@@ -454,8 +492,10 @@ extern "C" int rt_hip_batch_adjoint(const rt_hip_batch* b, const double* beam_de
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kAdjBlock), lds, st, a);
     }
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
-    hipLaunchKernelGGL(rt_adjoint_batch_reduce, dim3((unsigned)((uint64_t)v.n_inst * n_params)), dim3(256), 0, st,
-                       (const double*)B.gacc, v.tile_start_dev, (uint64_t)n_params, wpg, grad);
-    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    if (n_params > 0) {
+        hipLaunchKernelGGL(rt_adjoint_batch_reduce, dim3((unsigned)v.n_inst), dim3(256), 0, st,
+                           (const double*)B.gacc, v.tile_start_dev, (uint64_t)n_params, wpg, grad);
+        HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    }
     return RT_OK;
 }
