@@ -330,8 +330,13 @@ static bool compile_code(const std::string& src, std::vector<char>& code, std::s
         log = "hiprtcCreateProgram failed";
         return false;
     }
+    // the atomic optimizer is off: it wraps every statistics add (already made
+    // by one lane of a ballot count) in a second lane election and a multiply
+    // by the active-lane count, about 130 VALU instructions per pass over the
+    // echelle's surfaces; the counts are integer sums, the same either way
     std::vector<const char*> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-munsafe-fp-atomics",
-                                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+                                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                                     "-mllvm", "-amdgpu-atomic-optimizer-strategy=None"};
     hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
     size_t log_size = 0;
     hiprtcGetProgramLogSize(prog, &log_size);
