@@ -36,6 +36,7 @@ def lib():
         L.oracle_trace.argtypes = [_dp, C.c_int, _dp, _dp, C.c_uint64, C.c_size_t, C.c_size_t,
                                    _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i8p, C.c_int,
                                    _u32p, _dp, _u64p, _dp, _i8p, C.c_int]
+        L.oracle_trace_iters.argtypes = L.oracle_trace.argtypes + [_u32p, C.c_int]
         L.oracle_beam_transform.argtypes = [C.c_size_t, _dp, _dp] + [_dp] * 6
         L.oracle_normalize.argtypes = [C.c_size_t, _dp, _dp, _dp]
         L.oracle_surface_intersection.argtypes = [_dp, C.c_size_t] + [_dp] * 6 + [_u64p, _i8p, _u64p]
@@ -101,9 +102,12 @@ class TraceResult:
 
 
 def trace(scheme, seed=0, begin=0, n=None, rays=None, flag=None, flags=None, hist=False,
-          flux=False, dump=False, nthreads=0):
+          flux=False, dump=False, nthreads=0, iters=None, iters_mode=0):
     """Run the oracle's fused trace. `scheme` is an rt_amd.scheme.Scheme.
-    With rays=None the beam is generated from (scheme.beam, seed)."""
+    With rays=None the beam is generated from (scheme.beam, seed).
+    iters (n_surf x n uint32) with iters_mode 1 receives the Newton iteration
+    count of every (surface, ray); with iters_mode 2 it forces them (finite
+    differences at fixed iteration counts)."""
     from rt_amd import scheme as S
     table = np.ascontiguousarray(scheme.table, dtype=np.float64)
     pool = np.ascontiguousarray(scheme.pool, dtype=np.float64)
@@ -134,9 +138,12 @@ def trace(scheme, seed=0, begin=0, n=None, rays=None, flag=None, flags=None, his
         D = np.zeros((scheme.n_surf, 7, n))
         DF = np.zeros((scheme.n_surf, n), dtype=np.int8)
     rows = [R[i] for i in range(7)]
-    rc = lib().oracle_trace(_p(table), scheme.n_surf, _p(pool), _p(beam), seed, begin, n,
-                            *[_p(r) for r in rows], _p(flag, _i8p), f,
-                            _p(H, _u32p), _p(F), _p(st, _u64p), _p(D), _p(DF, _i8p), nthreads)
+    if iters is not None:
+        assert iters.dtype == np.uint32 and iters.shape == (scheme.n_surf, n) and iters.flags.c_contiguous
+    rc = lib().oracle_trace_iters(_p(table), scheme.n_surf, _p(pool), _p(beam), seed, begin, n,
+                                  *[_p(r) for r in rows], _p(flag, _i8p), f,
+                                  _p(H, _u32p), _p(F), _p(st, _u64p), _p(D), _p(DF, _i8p), nthreads,
+                                  _p(iters, _u32p), int(iters_mode) if iters is not None else 0)
     return TraceResult(R, flag, H, F, st, D, DF, rc)
 
 

@@ -295,8 +295,12 @@ static void toric_derivs(const double *p, double X, double Y, double Z,
     *dFdZ = 1.0 - cR * Z;
 }
 
-/* return 1 if alive after intersection; *niter counts Newton iterations */
-static int intersect_ray(const double *p, ray_t *r, uint64_t *niter)
+/* return 1 if alive after intersection; *niter counts Newton iterations.
+   forced > 0 (test infrastructure for finite differences at fixed iteration
+   counts): the Newton loops run exactly `forced` iterations instead of
+   stopping on the step size, so the returned iterate is a smooth function of
+   the scheme parameters; forced == 0 is the reference's stopping rule. */
+static int intersect_ray(const double *p, ray_t *r, uint64_t *niter, uint64_t forced)
 {
     int cls = (int)p[RT_S_CLASS];
     int N = (int)p[RT_S_NPOLY];
@@ -358,7 +362,7 @@ static int intersect_ray(const double *p, ray_t *r, uint64_t *niter)
             if (fabs(dF) < NEWTON_EPS) { alive = 0; break; }
             s2 = s1 - F / dF;
             ++it;
-        } while ((fabs(s2 - s1) >= NEWTON_EPS) && (it <= NEWTON_MAX_ITER));
+        } while (forced ? (it < forced) : ((fabs(s2 - s1) >= NEWTON_EPS) && (it <= NEWTON_MAX_ITER)));
         r->X = Xi; r->Y = Yi; r->Z = Zi;
         *niter = it;
         if (it > NEWTON_MAX_ITER) alive = 0;
@@ -392,7 +396,7 @@ static int intersect_ray(const double *p, ray_t *r, uint64_t *niter)
             if (fabs(dF) < NEWTON_EPS) { alive = 0; break; }
             s2 = s1 - F / dF;
             ++it;
-        } while ((fabs(s2 - s1) >= NEWTON_EPS) && (it <= NEWTON_MAX_ITER));
+        } while (forced ? (it < forced) : ((fabs(s2 - s1) >= NEWTON_EPS) && (it <= NEWTON_MAX_ITER)));
         r->X = Xi; r->Y = Yi; r->Z = Zi;
         *niter = it;
         if (it > NEWTON_MAX_ITER) alive = 0;
@@ -620,7 +624,7 @@ typedef struct {
 static void trace_one(const double *table, int n_surf, const double *pool,
                       ray_t *r, int *alive_io, int flags, uint64_t ray_local,
                       size_t n, uint32_t *hist, double *flux, stats_t *st,
-                      double *dump, signed char *dump_flag)
+                      double *dump, signed char *dump_flag, uint32_t *iters, int iters_mode)
 {
     int alive = *alive_io;
     for (int s = 0; s < n_surf; ++s) {
@@ -632,7 +636,9 @@ static void trace_one(const double *table, int n_surf, const double *pool,
             if (type != RT_AUXTYPE) {
                 uint64_t it = 0;
                 st->v[RT_ST_SURF + 4 * s + 0] += 1;
-                alive = intersect_ray(p, r, &it);                     /* :1002 */
+                uint64_t forced = (iters && iters_mode == 2) ? iters[(size_t)s * n + ray_local] : 0;
+                alive = intersect_ray(p, r, &it, forced);             /* :1002 */
+                if (iters && iters_mode == 1) iters[(size_t)s * n + ray_local] = (uint32_t)it;
                 st->v[RT_ST_SURF + 4 * s + 1] += it;
                 if (!alive) {
                     st->v[RT_ST_SURF + 4 * s + 2] += 1;
@@ -726,6 +732,18 @@ int oracle_trace(const double *table, int n_surf, const double *pool,
                  uint32_t *hist, double *flux, uint64_t *stats,
                  double *dump, signed char *dump_flag, int nthreads)
 {
+    return oracle_trace_iters(table, n_surf, pool, beam, seed, ray_begin, n, X, Y, Z, cX, cY, cZ, lambda, flag,
+                              flags, hist, flux, stats, dump, dump_flag, nthreads, NULL, 0);
+}
+
+int oracle_trace_iters(const double *table, int n_surf, const double *pool,
+                       const double *beam, uint64_t seed, size_t ray_begin, size_t n,
+                       double *X, double *Y, double *Z, double *cX, double *cY, double *cZ,
+                       double *lambda, signed char *flag, int flags,
+                       uint32_t *hist, double *flux, uint64_t *stats,
+                       double *dump, signed char *dump_flag, int nthreads,
+                       uint32_t *iters, int iters_mode)
+{
     if (n_surf < 0 || n_surf > RT_MAX_SURF) return RT_ERR_BAD_VALUE;
     int gen = (flags & RT_TRACE_GENERATE) != 0;
     int store = (flags & RT_TRACE_STORE) != 0;
@@ -759,7 +777,8 @@ int oracle_trace(const double *table, int n_surf, const double *pool,
                 alive = flag[i] != 0;
             }
             st.v[RT_ST_RAYS] += 1;
-            trace_one(table, n_surf, pool, &r, &alive, flags, i, n, hist, flux, &st, dump, dump_flag);
+            trace_one(table, n_surf, pool, &r, &alive, flags, i, n, hist, flux, &st, dump, dump_flag, iters,
+                      iters_mode);
             if (alive) st.v[RT_ST_GOOD] += 1;
             if (store || (gen && X)) {
                 if (X) { X[i] = r.X; Y[i] = r.Y; Z[i] = r.Z; cX[i] = r.cX; cY[i] = r.cY; cZ[i] = r.cZ; lambda[i] = r.lam; }
@@ -802,7 +821,7 @@ int oracle_surface_intersection(const double *rec, size_t n,
     for (size_t i = 0; i < n; ++i) {
         ray_t r = {X[i], Y[i], Z[i], cX[i], cY[i], cZ[i], 0.0};
         uint64_t it = 0;
-        int ok = intersect_ray(rec, &r, &it);
+        int ok = intersect_ray(rec, &r, &it, 0);
         X[i] = r.X; Y[i] = r.Y; Z[i] = r.Z;
         if (newton_iters) *newton_iters += it;
         if (!ok) { flag[i] = 0; if (n_bad) ++*n_bad; }

@@ -46,6 +46,18 @@ int oracle_trace(const double *table, int n_surf, const double *pool,
                  uint32_t *hist, double *flux, uint64_t *stats,
                  double *dump, signed char *dump_flag, int nthreads);
 
+/* oracle_trace with the Newton iteration counts of every (surface, ray):
+   iters is n_surf x n uint32 (row s at s * n); iters_mode 1 records them,
+   2 forces each intersection to run exactly the given number of iterations
+   (finite differences at fixed iteration counts), 0 ignores iters. */
+int oracle_trace_iters(const double *table, int n_surf, const double *pool,
+                       const double *beam, uint64_t seed, size_t ray_begin, size_t n,
+                       double *X, double *Y, double *Z, double *cX, double *cY, double *cZ,
+                       double *lambda, signed char *flag, int flags,
+                       uint32_t *hist, double *flux, uint64_t *stats,
+                       double *dump, signed char *dump_flag, int nthreads,
+                       uint32_t *iters, int iters_mode);
+
 /* per-range beam generators (the engine's beam_random, beam_*_cosins and
    *_range_beam split by component) */
 int oracle_beam_positions(const double *beam, uint64_t seed, size_t begin, size_t n,

@@ -174,27 +174,40 @@ __global__ __launch_bounds__(256) void rt_adjoint_reduce_parts(const double* par
     if (threadIdx.x == 0) grad[j] = v;
 }
 
-struct AdjBuffers {
-    int device = -1;
-    double* tape = nullptr;
-    size_t tape_len = 0;
-    double* gacc = nullptr;
-    size_t gacc_len = 0;
-    double* part = nullptr;
-    size_t part_len = 0;
-    int* params = nullptr;
-    size_t params_len = 0;
+// The tape, the per-wave gradient accumulators, the chunk partials and the
+// row table of an adjoint, in its scheme's (or batch's) working buffers: a
+// buffer only grows, and the one it replaced stays allocated for the owner's
+// life (a graph captured with it may be replayed).  The row table is re-made
+// in a new buffer when its contents change.
+struct AdjBufs {
+    double* tape;
+    double* gacc;
+    double* part;
+    int* params;
 };
-thread_local AdjBuffers g_adj;
 
-bool grow(double*& p, size_t& have, size_t want) {
-    if (have >= want) return true;
-    (void)hipFree(p);
-    p = nullptr;
-    have = 0;
-    if (hipMalloc(&p, want * sizeof(double)) != hipSuccess) return false;
-    have = want;
-    return true;
+int adj_buffers(WorkBufs* w, size_t tape, size_t gacc, size_t part, const std::vector<int>& rows, bool upload_rows,
+                AdjBufs& out) {
+    std::vector<int>& rows_key = w->params_key;
+    int rc = w->ensure(WorkBufs::TAPE, tape * sizeof(double));
+    if (rc == RT_OK) rc = w->ensure(WorkBufs::GACC, gacc * sizeof(double));
+    if (rc == RT_OK && part) rc = w->ensure(WorkBufs::PART, part * sizeof(double));
+    if (rc != RT_OK) return rc;
+    if (upload_rows && (rows != rows_key || !w->p[WorkBufs::PARAMS])) {
+        rows_key.clear();
+        rc = w->fresh(WorkBufs::PARAMS, rows.size() * sizeof(int));
+        if (rc != RT_OK) return rc;
+        // read by the generic kernel only (the generated one has the rows in
+        // its code): a synchronous upload, so no host buffer outlives the call
+        HIP_TRY(hipMemcpy(w->p[WorkBufs::PARAMS], rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice),
+                RT_ERR_FAILED);
+        rows_key = rows;
+    }
+    out.tape = w->get<double>(WorkBufs::TAPE);
+    out.gacc = w->get<double>(WorkBufs::GACC);
+    out.part = w->get<double>(WorkBufs::PART);
+    out.params = w->get<int>(WorkBufs::PARAMS);
+    return RT_OK;
 }
 
 }  // namespace
@@ -335,29 +348,11 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     const unsigned grid = (unsigned)(want < cap ? want : cap);
     const uint64_t G = (uint64_t)grid * kAdjBlock;
 
-    AdjBuffers& B = g_adj;
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev), RT_ERR_FAILED);
-    if (B.device != dev) {
-        (void)hipFree(B.tape); (void)hipFree(B.gacc); (void)hipFree(B.part); (void)hipFree(B.params);
-        B = AdjBuffers();
-        B.device = dev;
-    }
-    if (!grow(B.tape, B.tape_len, (size_t)n_surf * kTape * G) ||
-        !grow(B.gacc, B.gacc_len, (size_t)std::max(n_params, 1) * (G / 64)) ||
-        !grow(B.part, B.part_len, (size_t)std::max(n_params, 1) * kReduceChunks))
-        return rt_set_error(RT_ERR_BAD_ALLOC, "adjoint buffers");
-    if (B.params_len < host.size()) {
-        (void)hipFree(B.params);
-        B.params = nullptr;
-        B.params_len = 0;
-        HIP_TRY(hipMalloc(&B.params, host.size() * sizeof(int)), RT_ERR_BAD_ALLOC);
-        B.params_len = host.size();
-    }
+    AdjBufs B;
+    rc = adj_buffers(rt_scheme_work(sch), (size_t)n_surf * kTape * G, (size_t)std::max(n_params, 1) * (G / 64),
+                     (size_t)std::max(n_params, 1) * kReduceChunks, host, !fn, B);
+    if (rc != RT_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
-    // the row table is read by the generic kernel only (the generated one has
-    // the rows in its code): a synchronous upload, so no host buffer outlives the call
-    if (!fn) HIP_TRY(hipMemcpy(B.params, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice), RT_ERR_FAILED);
     a.prow = B.params;
     a.tape = B.tape;
     a.gacc = B.gacc;
@@ -457,25 +452,10 @@ extern "C" int rt_hip_batch_adjoint(const rt_hip_batch* b, const double* beam_de
         return rt_set_error(RT_ERR_BAD_VALUE, "batched adjoint: %llu tiles x %d parameters exceed one buffer",
                             (unsigned long long)v.n_tiles, n_params);
 
-    AdjBuffers& B = g_adj;
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev), RT_ERR_FAILED);
-    if (B.device != dev) {
-        (void)hipFree(B.tape); (void)hipFree(B.gacc); (void)hipFree(B.part); (void)hipFree(B.params);
-        B = AdjBuffers();
-        B.device = dev;
-    }
-    if (!grow(B.tape, B.tape_len, (size_t)v.n_surf * kTape * G) || !grow(B.gacc, B.gacc_len, part_len))
-        return rt_set_error(RT_ERR_BAD_ALLOC, "adjoint buffers");
-    if (B.params_len < host.size()) {
-        (void)hipFree(B.params);
-        B.params = nullptr;
-        B.params_len = 0;
-        HIP_TRY(hipMalloc(&B.params, host.size() * sizeof(int)), RT_ERR_BAD_ALLOC);
-        B.params_len = host.size();
-    }
+    AdjBufs B;
+    rc = adj_buffers(rt_batch_work(b), (size_t)v.n_surf * kTape * G, part_len, 0, host, !fn, B);
+    if (rc != RT_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
-    if (!fn) HIP_TRY(hipMemcpy(B.params, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice), RT_ERR_FAILED);
     a.prow = B.params;
     a.tape = B.tape;
     a.gacc = B.gacc;

@@ -907,7 +907,9 @@ constexpr int kBeamRec = (int)(sizeof(BeamArg) / sizeof(double));
 constexpr int kStatCopies = 64;
 
 // stats[k] += sum of the copies' counter k, and the copies cleared for the
-// next launch (one workgroup)
+// next launch (one workgroup).  The caller's counters are added to
+// atomically: traces of distinct schemes on distinct streams may fold into
+// one caller buffer at the same time.
 __device__ __forceinline__ void stats_fold_body(unsigned long long* copies, unsigned long long* stats) {
     for (int k = threadIdx.x; k < RT_STATS_LEN; k += blockDim.x) {
         unsigned long long t = 0;
@@ -915,7 +917,7 @@ __device__ __forceinline__ void stats_fold_body(unsigned long long* copies, unsi
             t += copies[(size_t)c * RT_STATS_LEN + k];
             copies[(size_t)c * RT_STATS_LEN + k] = 0;
         }
-        stats[k] += t;
+        if (t) atomicAdd(&stats[k], t);
     }
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 // Record an error message for rt_hip_last_error and return `code`.
 int rt_set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -15,6 +16,32 @@ int rt_set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2
             return rt_set_error(code, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
                                 __LINE__);                                                     \
     } while (0)
+
+// Device working memory of a scheme or batch (or of one host thread, for the
+// calls that belong to neither): the adjoint's tape, gradient partials and
+// row table, the spot partials, the descent map.  A buffer that must grow is
+// replaced by a new allocation and the old one is kept until release() (the
+// owner's destruction): a graph captured before may still hold its address,
+// so no buffer a call has handed to a kernel is ever freed while its owner
+// lives.  The calls of one owner share its buffers and therefore run in the
+// order of one stream.
+struct WorkBufs {
+    enum Slot { TAPE, GACC, PART, PARAMS, SPOT_PART, SPOT_CS, DESCEND, N_SLOTS };
+    void* p[N_SLOTS] = {};
+    size_t have[N_SLOTS] = {};
+    std::vector<void*> owned;
+    std::vector<int> params_key;     // contents of the PARAMS buffer
+    // p[s] holds at least `bytes` afterwards (RT_OK, or RT_ERR_BAD_ALLOC
+    // with p[s] unchanged)
+    int ensure(Slot s, size_t bytes);
+    // p[s] is a new buffer of `bytes` afterwards (for contents that change
+    // between calls while a captured graph may still read the old ones)
+    int fresh(Slot s, size_t bytes);
+    // frees every buffer ever allocated (the owner is being destroyed)
+    void release();
+    template <class T>
+    T* get(Slot s) const { return (T*)p[s]; }
+};
 
 // Current device selected and its properties cached (RT_OK or an error).
 int rt_ensure_device();
@@ -30,6 +57,8 @@ void rt_device_record(const double* pub, double* dev);
 struct rt_hip_scheme;
 const std::string& rt_scheme_signature(const rt_hip_scheme* s);
 int rt_scheme_device(const rt_hip_scheme* s);
+// the working buffers owned by a scheme / a batch
+WorkBufs* rt_scheme_work(const rt_hip_scheme* s);
 
 // What the batched adjoint needs of a batch (device pointers and structure).
 struct rt_hip_batch;
@@ -49,6 +78,7 @@ struct BatchView {
     int beam_type;                   // their beam type (-1: none)
 };
 int rt_batch_view(const rt_hip_batch* b, BatchView* v);
+WorkBufs* rt_batch_work(const rt_hip_batch* b);
 
 // The per-wavelength table of a scheme or batch (its spline-valued functions
 // at the grid wavelengths of a beam): the number of grid wavelengths it

@@ -55,6 +55,36 @@ int rt_set_error(int code, const char* fmt, ...) {
 }
 #define set_error rt_set_error
 
+// ------------------------------------------------------- working buffers
+
+int WorkBufs::ensure(Slot s, size_t bytes) {
+    if (p[s] && have[s] >= bytes) return RT_OK;
+    // geometric growth, so a sequence of growing calls allocates O(log) buffers
+    return fresh(s, std::max(bytes, have[s] + have[s] / 2));
+}
+
+int WorkBufs::fresh(Slot s, size_t want) {
+    void* q = nullptr;
+    if (hipMalloc(&q, want ? want : 8) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(RT_ERR_BAD_ALLOC, "hipMalloc of %zu bytes of working memory failed", want);
+    }
+    owned.push_back(q);
+    p[s] = q;
+    have[s] = want;
+    return RT_OK;
+}
+
+void WorkBufs::release() {
+    for (void* q : owned) (void)hipFree(q);
+    owned.clear();
+    params_key.clear();
+    for (int s = 0; s < N_SLOTS; ++s) {
+        p[s] = nullptr;
+        have[s] = 0;
+    }
+}
+
 // ----------------------------------------------------------------- kernel
 
 using rt::kBlock;
@@ -380,6 +410,8 @@ struct rt_hip_scheme {
     // kStatCopies x RT_STATS_LEN counters the trace kernels add to, folded into
     // the caller's statistics after each trace (and cleared by the fold)
     unsigned long long* stat_copies = nullptr;
+    // working memory of the scheme's adjoints and spot reductions
+    mutable WorkBufs work;
 };
 
 namespace {
@@ -696,6 +728,7 @@ extern "C" int rt_hip_jit_check(const double* surf_table, int n_surf, const doub
 
 extern "C" int rt_hip_scheme_destroy(rt_hip_scheme* s) {
     if (!s) return RT_OK;
+    s->work.release();
     (void)hipFree(s->stat_copies);
     s->lt.release();
     (void)hipFree(s->table_dev);
@@ -1040,9 +1073,11 @@ struct rt_hip_batch {
     uint64_t* tile_start_dev = nullptr;
     uint32_t* tile_inst_dev = nullptr;   // instance of each tile
     double* moments_dev = nullptr;
-    // descent map (rt_hip_batch_descend), kept until the parameter list changes
+    // descent map (rt_hip_batch_descend) in work's DESCEND slot: map_start[n_surf + 1]
+    // then the (j, field) pairs, re-made (in a new buffer) when the parameter list changes
     std::vector<int> descend_key;
-    int* descend_dev = nullptr;      // map_start[n_surf + 1] then the (j, field) pairs
+    // working memory of the batch's adjoints, spot reductions and descent steps
+    mutable WorkBufs work;
     Ltab lt;
     unsigned long long* stat_copies = nullptr;   // as rt_hip_scheme::stat_copies
     // a beam per instance (rt_hip_batch_set_beams): n_inst x kBeamRec records,
@@ -1066,7 +1101,7 @@ void free_batch(rt_hip_batch* b) {
     (void)hipFree(b->pool_dev);
     (void)hipFree(b->ray_start_dev);
     (void)hipFree(b->tile_inst_dev);
-    (void)hipFree(b->descend_dev);
+    b->work.release();
     (void)hipFree(b->tile_start_dev);
     (void)hipFree(b->moments_dev);
     (void)hipFree(b->beams_dev);
@@ -1127,9 +1162,10 @@ int prepare_tiles(rt_hip_batch* b) {
         uint64_t n = b->ray_start[i + 1] - b->ray_start[i];
         ts[i + 1] = ts[i] + (n + b->block - 1) / b->block;
     }
-    (void)hipFree(b->tile_start_dev);
-    (void)hipFree(b->tile_inst_dev);
-    (void)hipFree(b->moments_dev);
+    // arrays of another workgroup size stay allocated until the batch is
+    // destroyed: a graph captured with them may still be replayed
+    for (void* q : {(void*)b->tile_start_dev, (void*)b->tile_inst_dev, (void*)b->moments_dev})
+        if (q) b->work.owned.push_back(q);
     b->tile_start_dev = nullptr;
     b->tile_inst_dev = nullptr;
     b->moments_dev = nullptr;
@@ -1484,6 +1520,8 @@ int rt_batch_ltab_fill(const rt_hip_batch* b, const double* beam_desc, uint64_t 
     return ltab_fill(b->lt, a, b->pool_dev, nk, stream);
 }
 int rt_scheme_device(const rt_hip_scheme* s) { return s->device; }
+WorkBufs* rt_scheme_work(const rt_hip_scheme* s) { return &s->work; }
+WorkBufs* rt_batch_work(const rt_hip_batch* b) { return &b->work; }
 
 extern "C" int rt_hip_scheme_device_tables(const rt_hip_scheme* s, const double** table_dev, const double** pool_dev,
                                            int* n_surf, size_t* pool_len, const double** host_table) {
@@ -1496,41 +1534,64 @@ extern "C" int rt_hip_scheme_device_tables(const rt_hip_scheme* s, const double*
     return RT_OK;
 }
 
-// Spots of the alive rays per ray group; partial moments in a per-thread
-// device buffer grown on demand (kept for the life of the thread).
+// Spots of the alive rays per ray group; partial moments in a buffer of the
+// calling thread (calls that belong to no scheme): it only grows, and a
+// buffer it replaced stays allocated until the thread ends, so a graph that
+// captured a call keeps valid memory.  Calls of one thread share it: they
+// must run in the order of one stream.
+namespace {
+struct ThreadWork {
+    int device = -1;
+    WorkBufs w;
+    ~ThreadWork() { w.release(); }
+};
+thread_local ThreadWork g_thread_work;
+
+WorkBufs* thread_work(int dev) {
+    if (g_thread_work.device != dev) {
+        // another device: the old buffers belong to that device's context
+        g_thread_work.w.release();
+        g_thread_work.device = dev;
+    }
+    return &g_thread_work.w;
+}
+
+int group_spots_launch(const double* X, const double* Y, const signed char* flag, uint64_t n, uint64_t G,
+                       uint64_t n_chunks, double* part, double* spots, hipStream_t st) {
+    // with n == 0 the single chunk is empty and its partials are written as zeros
+    const uint64_t threads = n_chunks * G;
+    hipLaunchKernelGGL(rt_group_moments_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, X, Y,
+                       flag, n, G, n_chunks, part);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    hipLaunchKernelGGL(rt_group_spots_kernel, dim3((unsigned)G), dim3(256), 0, st, (const double*)part, n_chunks,
+                       G, spots);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    return RT_OK;
+}
+
+// chunks of kSpotRays strides of the groups over n rays
+uint64_t spot_chunks(uint64_t n, uint64_t G) {
+    const uint64_t strides = (n + G - 1) / G;
+    return strides > 0 ? (strides + kSpotRays - 1) / kSpotRays : 1;
+}
+}  // namespace
+
 extern "C" int rt_hip_group_spots(const double* X, const double* Y, const signed char* flag, size_t n,
                                   size_t n_groups, double* spots, void* stream) {
     if (!spots || n_groups == 0) return set_error(RT_ERR_BAD_VALUE, "spots NULL or no groups");
     if (n > 0 && (!X || !Y || !flag)) return set_error(RT_ERR_BAD_VALUE, "NULL ray array");
     int rc = ensure_device();
     if (rc != RT_OK) return rc;
-    hipStream_t st = (hipStream_t)stream;
-    // chunks of kSpotRays strides of the groups
     const uint64_t G = n_groups;
-    const uint64_t strides = (n + G - 1) / G;
-    const uint64_t n_chunks = strides > 0 ? (strides + kSpotRays - 1) / kSpotRays : 1;
-    // per-thread partials, kept for the life of the thread on its device
-    struct Part { double* p = nullptr; size_t cap = 0; int device = -1; };
-    static thread_local Part part;
+    const uint64_t n_chunks = spot_chunks(n, G);
+    if (n_chunks > (1ull << 40) / G) return set_error(RT_ERR_BAD_VALUE, "too many rays or groups");
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev), RT_ERR_FAILED);
-    const size_t need = n_chunks * G * 5;
-    if (part.device != dev || part.cap < need) {
-        if (part.p && part.device == dev) (void)hipFree(part.p);
-        part = Part();
-        HIP_TRY(hipMalloc(&part.p, need * sizeof(double)), RT_ERR_BAD_ALLOC);
-        part.cap = need;
-        part.device = dev;
-    }
-    // with n == 0 the single chunk is empty and its partials are written as zeros
-    const uint64_t threads = n_chunks * G;
-    hipLaunchKernelGGL(rt_group_moments_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, X, Y,
-                       flag, (uint64_t)n, G, n_chunks, part.p);
-    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
-    hipLaunchKernelGGL(rt_group_spots_kernel, dim3((unsigned)G), dim3(256), 0, st, (const double*)part.p, n_chunks,
-                       G, spots);
-    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
-    return RT_OK;
+    WorkBufs* w = thread_work(dev);
+    rc = w->ensure(WorkBufs::SPOT_PART, n_chunks * G * 5 * sizeof(double));
+    if (rc != RT_OK) return rc;
+    return group_spots_launch(X, Y, flag, n, G, n_chunks, w->get<double>(WorkBufs::SPOT_PART), spots,
+                              (hipStream_t)stream);
 }
 
 extern "C" int rt_hip_spot_loss(const double* spots, size_t n_groups, const double* target, double weight,
@@ -1561,26 +1622,12 @@ extern "C" int rt_hip_batch_group_spots(const rt_hip_batch* b, const double* X, 
         const uint64_t n = b->ray_start[i + 1] - b->ray_start[i], strides = (n + G - 1) / G;
         chunks += strides > 0 ? (strides + kSpotRays - 1) / kSpotRays : 1;
     }
-    struct Buf { double* part = nullptr; size_t cap = 0; uint64_t* cs = nullptr; size_t cs_cap = 0; int device = -1; };
-    static thread_local Buf buf;
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev), RT_ERR_FAILED);
-    if (buf.device != dev) buf = Buf();
-    if (buf.cap < chunks * G * 5) {
-        if (buf.part) (void)hipFree(buf.part);
-        buf.part = nullptr;
-        buf.cap = 0;
-        HIP_TRY(hipMalloc(&buf.part, chunks * G * 5 * sizeof(double)), RT_ERR_BAD_ALLOC);
-        buf.cap = chunks * G * 5;
-    }
-    if (buf.cs_cap < (size_t)b->n_inst + 1) {
-        if (buf.cs) (void)hipFree(buf.cs);
-        buf.cs = nullptr;
-        buf.cs_cap = 0;
-        HIP_TRY(hipMalloc(&buf.cs, ((size_t)b->n_inst + 1) * sizeof(uint64_t)), RT_ERR_BAD_ALLOC);
-        buf.cs_cap = (size_t)b->n_inst + 1;
-    }
-    buf.device = dev;
+    // the batch's own working memory (calls on one batch are stream-ordered)
+    rc = b->work.ensure(WorkBufs::SPOT_PART, chunks * G * 5 * sizeof(double));
+    if (rc == RT_OK) rc = b->work.ensure(WorkBufs::SPOT_CS, ((size_t)b->n_inst + 1) * sizeof(uint64_t));
+    if (rc != RT_OK) return rc;
+    struct { double* part; uint64_t* cs; } buf = {b->work.get<double>(WorkBufs::SPOT_PART),
+                                                  b->work.get<uint64_t>(WorkBufs::SPOT_CS)};
     hipLaunchKernelGGL(rt_batch_chunks_kernel, dim3(1), dim3(1024), 0, st, (const uint64_t*)b->ray_start_dev,
                        (uint64_t)b->n_inst, G, buf.cs);
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
@@ -1650,18 +1697,20 @@ extern "C" int rt_hip_batch_descend(rt_hip_batch* b, const int* param_surface, c
         for (int s = 0; s < b->n_surf; ++s) host[s + 1] = host[s] + (int)per[s].size();
         for (int s = 0; s < b->n_surf; ++s)
             for (auto& jf : per[s]) { host.push_back(jf.first); host.push_back(jf.second); }
-        (void)hipFree(b->descend_dev);
-        b->descend_dev = nullptr;
+        // a new buffer for a new list: a graph captured with the old one keeps reading it
         b->descend_key.clear();
-        HIP_TRY(hipMalloc(&b->descend_dev, host.size() * sizeof(int)), RT_ERR_BAD_ALLOC);
-        HIP_TRY(hipMemcpy(b->descend_dev, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice),
+        int rc = b->work.fresh(WorkBufs::DESCEND, host.size() * sizeof(int));
+        if (rc != RT_OK) return rc;
+        HIP_TRY(hipMemcpy(b->work.p[WorkBufs::DESCEND], host.data(), host.size() * sizeof(int),
+                          hipMemcpyHostToDevice),
                 RT_ERR_FAILED);
         b->descend_key = key;
     }
+    const int* map = b->work.get<const int>(WorkBufs::DESCEND);
     const uint64_t threads = (uint64_t)b->n_inst * b->n_surf;
     hipLaunchKernelGGL(rt_batch_descend_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
                        (hipStream_t)stream, b->tables_dev, (uint64_t)b->n_inst, b->n_surf,
-                       (const int*)b->descend_dev, (const int*)b->descend_dev + b->n_surf + 1, grad, lr, n_params);
+                       map, map + b->n_surf + 1, grad, lr, n_params);
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     return RT_OK;
 }

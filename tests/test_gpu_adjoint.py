@@ -267,3 +267,76 @@ def test_spot_loss_and_seeds(hip):
         assert abs(float(gl.item()) - loss) <= 1e-12 * abs(loss)
         want = np.stack([ax, ay, b], axis=1)
         assert np.all(np.abs(gs - want) <= 1e-12 * np.maximum(np.abs(want), 1e-30))
+
+
+def _oracle_loss_fixed(sch, seed, n_groups, target, weight, iters, mode):
+    """the spot loss of the oracle's trace with the Newton iteration counts
+    recorded (mode 1) or held at the given ones (mode 2); vectorised over
+    groups (two-pass centroid and spread)"""
+    r = O.trace(sch, seed=seed, nthreads=16, iters=iters, iters_mode=mode)
+    ok = r.flag == 1
+    g = (np.arange(sch.beam.n_rays) % n_groups)[ok]
+    x, y = r.rays[0][ok], r.rays[1][ok]
+    cnt = np.bincount(g, minlength=n_groups).astype(np.float64)
+    cx = np.bincount(g, weights=x, minlength=n_groups) / np.maximum(cnt, 1)
+    cy = np.bincount(g, weights=y, minlength=n_groups) / np.maximum(cnt, 1)
+    m2 = np.bincount(g, weights=(x - cx[g]) ** 2 + (y - cy[g]) ** 2, minlength=n_groups)
+    live = cnt > 0
+    rms = np.sqrt(m2[live] / cnt[live])
+    loss = float(np.sum(rms + weight * ((cx[live] - target[live, 0]) ** 2 + (cy[live] - target[live, 1]) ** 2)))
+    return loss, r
+
+
+def test_c4_spot_loss_gradient_full_size(hip):
+    """The differentiable configuration at its stated size: the echelle at
+    15625 rays x 64 wavelengths (1e6 rays), the spot loss of 64 wavelength
+    groups (RMS radius + 0.5 x squared deviation of the centroid from the
+    dispersion line), the adjoint over the 48 design parameters against
+    central finite differences of the oracle with h = 1e-7 relative, bar
+    1e-6 relative + the FD rounding noise.  At 1e6 rays a perturbation of
+    1e-7 makes some ray's Newton loop stop one iteration earlier or later,
+    which moves that ray by up to the 1e-7 mm stopping tolerance (a jump, not
+    a derivative); the finite differences therefore hold every ray's
+    iteration counts at the nominal trace's (oracle_trace_iters), which is
+    the function the adjoint differentiates.  Parameters whose perturbed
+    traces change the set of surviving rays are skipped; at least 44 of the 48
+    must qualify."""
+    base = schemes.echelle(n_per_lambda=15625, nlambda=64)
+    assert base.beam.n_rays == 1_000_000
+    spec = diff.echelle_design_params(base)
+    assert len(spec) == 48
+    n_groups, seed, weight = 64, 9, 0.5
+    v0 = spec.values(base)
+    loss0 = diff.SpotLoss(base, spec, n_groups=n_groups, seed=seed)
+    cnt, cx, cy, rms = (t.cpu().numpy() for t in loss0.spots(v0))
+    lam = np.array([O.gen_rays(base.beam.pack(), seed, k, 1)[6, 0] for k in range(n_groups)])
+    target = np.stack([np.polyval(np.polyfit(lam, cx, 1), lam), np.polyval(np.polyfit(lam, cy, 1), lam)], axis=1)
+    L = diff.SpotLoss(base, spec, n_groups=n_groups, target=target, weight=weight, seed=seed)
+    print("[C4] adjoint at 1e6 rays", flush=True)
+    val, grad = L.value_and_grad(v0)
+    iters = np.zeros((base.n_surf, base.beam.n_rays), dtype=np.uint32)
+    oval, oref = _oracle_loss_fixed(spec.apply(base, v0), seed, n_groups, target, weight, iters, 1)
+    assert abs(val - oval) <= 1e-11 * abs(oval), (val, oval)
+    checked = 0
+    worst = 0.0
+    for j, p in enumerate(spec.params):
+        x = v0[j]
+        h = 1e-7 * abs(x) if x != 0.0 else 1e-7
+        vp, vm = v0.copy(), v0.copy()
+        vp[j] += h
+        vm[j] -= h
+        lp, rp = _oracle_loss_fixed(spec.apply(base, vp), seed, n_groups, target, weight, iters.copy(), 2)
+        lm, rm = _oracle_loss_fixed(spec.apply(base, vm), seed, n_groups, target, weight, iters.copy(), 2)
+        if not (np.array_equal(rp.flag, oref.flag) and np.array_equal(rm.flag, oref.flag)):
+            print(f"[C4 fd] parameter {j} ({p.surface}, {p.field}): surviving rays change, skipped", flush=True)
+            continue
+        noise = 8 * np.finfo(float).eps * abs(val) / (2 * h)
+        fd = (lp - lm) / (2 * h)
+        rel = abs(grad[j] - fd) / max(abs(fd), 1e-300)
+        worst = max(worst, rel)
+        print(f"[C4 fd] {j:2d} s{p.surface} f{p.field} adjoint {grad[j]:+.9e} fd {fd:+.9e} rel {rel:.1e}",
+              flush=True)
+        assert abs(grad[j] - fd) <= 1e-6 * abs(fd) + noise, (j, grad[j], fd, noise)
+        checked += 1
+    print(f"[C4] {checked} of 48 parameters checked, worst relative difference {worst:.2e}", flush=True)
+    assert checked >= 44, checked

@@ -256,3 +256,79 @@ def test_batch_adjoint_with_a_beam_per_instance(hip):
         scale = max(float(np.abs(g1).max()), 1e-300)
         assert np.abs(gb[i] - g1).max() <= 1e-11 * scale, i
     db.close()
+
+
+def test_design_graph_survives_later_buffer_growth(hip):
+    """A captured design step keeps valid working memory when later eager calls
+    need larger buffers: between replays, an eager adjoint of the same batch
+    over more parameters (larger gradient partials), an eager adjoint of a
+    larger batch and a single-scheme adjoint run; every replay still equals
+    the eager step of an identical batch bit for bit."""
+    print("[design graph growth]", flush=True)
+
+    def make(n_inst, seed):
+        base, inst, counts, _, _, db = setup_batch(hip, n_inst=n_inst, seed=seed)
+        return base, inst, counts, db
+
+    base, inst, counts, db_e = make(2, 17)
+    _, _, _, db_g = make(2, 17)
+    ps, pf = _design_params(base)
+    ccd = base.n_surf - 1
+    lr = torch.tensor([1e-4 if (s == ccd and f == S.S_DIST + 2) else 0.0 for s, f in zip(ps, pf)],
+                      dtype=torch.float64, device="cuda")
+    # a shorter list for the captured step, so the later eager call over the full one grows the partials
+    keep = [j for j, (s, f) in enumerate(zip(ps, pf)) if s >= ccd - 3]
+    ps_s, pf_s, lr_s = ps[keep], pf[keep], lr[keep].contiguous()
+    total = int(sum(counts))
+    n = len(inst)
+
+    def buffers(npar):
+        return dict(rays=torch.zeros((7, total), dtype=torch.float64, device="cuda"),
+                    flag=torch.zeros(total, dtype=torch.int8, device="cuda"),
+                    spots=torch.zeros((n, G, hip.SPOT_STRIDE), dtype=torch.float64, device="cuda"),
+                    seeds=torch.zeros((n * G, 3), dtype=torch.float64, device="cuda"),
+                    loss=torch.zeros(1, dtype=torch.float64, device="cuda"),
+                    grad=torch.zeros((n, npar), dtype=torch.float64, device="cuda"))
+
+    def step(db, b):
+        db.trace(seed=11, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, rays=b["rays"], flag=b["flag"])
+        db.group_spots(b["rays"], b["flag"], G, out=b["spots"])
+        hip.spot_loss(b["spots"].view(n * G, hip.SPOT_STRIDE), seeds=b["seeds"], loss=b["loss"])
+        db.adjoint(b["seeds"].view(n, G, 3), ps_s, pf_s, seed=11, grad=b["grad"])
+        db.descend(ps_s, pf_s, b["grad"], lr_s)
+
+    be, bg = buffers(len(ps_s)), buffers(len(ps_s))
+    eager = []
+    for _ in range(4):
+        step(db_e, be)
+        eager.append((float(be["loss"].item()), be["grad"].clone()))
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step(db_g, bg)
+    torch.cuda.synchronize()
+    got = [(float(bg["loss"].item()), bg["grad"].clone())]
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
+        step(db_g, bg)
+    big_base, big_inst, big_counts, db_big = make(5, 19)
+    seeds_big = torch.full((len(big_inst), G, 3), 1e-4, dtype=torch.float64, device="cuda")
+    seeds_g = torch.full((n, G, 3), 1e-4, dtype=torch.float64, device="cuda")
+    ds = hip.DeviceScheme(base)
+    for k in range(3):
+        with torch.cuda.stream(s):
+            g.replay()
+        torch.cuda.synchronize()
+        got.append((float(bg["loss"].item()), bg["grad"].clone()))
+        # eager calls between replays that need larger working buffers
+        db_g.adjoint(seeds_g, ps, pf, seed=3 + k)                 # more parameters on the same batch
+        db_big.adjoint(seeds_big, ps, pf, seed=3 + k)             # another, larger batch
+        ds.adjoint(seeds_g[0].contiguous(), ps, pf, seed=3 + k)   # a single scheme
+        torch.cuda.synchronize()
+    for k, ((le, ge), (lg, gg)) in enumerate(zip(eager, got)):
+        assert le == lg, (k, le, lg)
+        assert torch.equal(ge, gg), k
+    ds.close()
+    db_big.close()
+    db_e.close()
+    db_g.close()
