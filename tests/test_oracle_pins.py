@@ -473,6 +473,151 @@ def test_grating_energy_sums_to_one_over_orders():
     assert abs(best - 2 * d * math.sin(blaze) / 0.5) < 1.0   # peak at the blaze order
 
 
+def _facet_model_energy(lam, order, blaze, alpha, gamma, d, rule):
+    """Scalar blaze efficiency of a sawtooth grating from the facet geometry,
+    built from vectors in the dispersion plane (an independent statement of
+    the model grating_energy_distr follows, surface.cu:1988-2019,2098-2153):
+
+    * unit vectors from the grating toward the source, s_in = (sin a, cos a),
+      and along the diffracted light, s_out = (sin b, cos b), with the
+      grating equation sin a + sin b = m lam / (d cos gamma) (Littrow: b = a);
+    * the facet tilted by the blaze angle: normal n_f = (sin c, cos c), facet
+      direction f = (cos c, -sin c);
+    * the illuminated facet length l: the part of the facet whose projection
+      across the diffracted beam equals one period's, l = rule d |x . s_perp| /
+      |f . s_perp| with s_perp perpendicular to s_out;
+    * the phase difference across it, U = pi l f . (s_in + s_out) / lam (half
+      the path difference in waves: zero at the facet's specular direction);
+    * the obliquity factor (s_out . z)(s_in . n_f) / (s_out . n_f);
+    * I = (obliquity sin(U)/U)^2, normalised over the propagating orders
+      (|sin b| <= 1)."""
+    c = blaze
+    s_in = np.array([math.sin(alpha), math.cos(alpha)])
+    n_f = np.array([math.sin(c), math.cos(c)])
+    f = np.array([math.cos(c), -math.sin(c)])
+    z = np.array([0.0, 1.0])
+    x = np.array([1.0, 0.0])
+
+    def intensity(m):
+        sb = m * lam / (d * math.cos(gamma)) - math.sin(alpha)
+        cb = math.sqrt(max(0.0, 1.0 - sb * sb))
+        s_out = np.array([sb, cb])
+        s_perp = np.array([cb, -sb])
+        length = rule * d * abs(x @ s_perp) / abs(f @ s_perp)
+        U = math.pi * length * (f @ (s_in + s_out)) / lam
+        sinc = math.sin(U) / U if U != 0.0 else 1.0
+        obl = (s_out @ z) * (s_in @ n_f) / (s_out @ n_f)
+        return (obl * sinc) ** 2
+
+    lo = math.ceil(d * math.cos(gamma) * (math.sin(alpha) - 1.0) / lam - 1e-12)
+    hi = math.floor(d * math.cos(gamma) * (math.sin(alpha) + 1.0) / lam + 1e-12)
+    norm = sum(intensity(m) for m in range(lo, hi + 1))
+    return intensity(order) / norm
+
+
+@pytest.mark.parametrize("d,blaze_deg,alpha_deg,gamma_deg,rule", [
+    (1000 / 31.6, 63.4, 63.4, 0.0, 1.0),        # R2 echelle in Littrow
+    (1000 / 31.6, 63.4, 61.0, 0.0, 0.85),       # off Littrow, part of each facet ruled
+    (1000 / 600.0, 17.5, 25.0, 0.0, 1.0),       # first-order grating
+    (1000 / 300.0, 8.6, 12.0, 5.0, 0.7),        # out-of-plane incidence (gamma)
+])
+def test_grating_energy_matches_the_facet_model(d, blaze_deg, alpha_deg, gamma_deg, rule):
+    """grating_energy_distr against the facet model above for every
+    propagating order at several wavelengths.  The reference evaluates sin(U)/U
+    by the polynomial 1 - 0.16605 U^2 + 0.00761 U^4 for |U| < pi/2
+    (surface.cu:2001-2004), within 1.2e-4 of sin(U)/U there, so the bar is
+    1e-3 relative (plus 1e-9 absolute for orders near a zero of the sinc).
+    A wrong factor in U (rule fraction, pi, lambda, grating constant), a sine
+    and cosine exchanged, or a wrong obliquity term fails it."""
+    blaze, alpha, gamma = (math.radians(v) for v in (blaze_deg, alpha_deg, gamma_deg))
+    for lam in (0.41, 0.55, 0.73):
+        lo = math.ceil(d * math.cos(gamma) * (math.sin(alpha) - 1.0) / lam - 1e-12)
+        hi = math.floor(d * math.cos(gamma) * (math.sin(alpha) + 1.0) / lam + 1e-12)
+        for m in range(lo, hi + 1):
+            got = O.grating_energy(np.array([lam]), m, blaze, alpha, gamma, d, 0.0, rule)[0]
+            want = _facet_model_energy(lam, m, blaze, alpha, gamma, d, rule)
+            assert abs(got - want) <= 1e-3 * want + 1e-9, (lam, m, got, want)
+
+
+def test_grating_energy_littrow_blaze_known_answers():
+    """In Littrow at the blaze wavelength of order m (lambda_b = 2 d sin c / m)
+    the diffracted order m leaves along the facet normal: U = 0 and its
+    intensity is the obliquity term alone, cos^2 c; the neighbouring orders
+    m +- 1 lie near zeros of the sinc (U ~ -+pi(1 - O(angle change))), so they
+    carry little energy; adjacent orders trade the energy between their blaze
+    wavelengths."""
+    d, c = 1000 / 31.6, math.radians(63.4)
+    m = 90
+    lam_b = 2 * d * math.sin(c) / m
+    e = {k: O.grating_energy(np.array([lam_b]), k, c, c, 0.0, d)[0] for k in (m - 1, m, m + 1)}
+    lo = math.ceil(d * (math.sin(c) - 1.0) / lam_b - 1e-12)
+    hi = math.floor(d * (math.sin(c) + 1.0) / lam_b + 1e-12)
+    # unnormalised order-m intensity cos^2 c: the other orders' share follows from the
+    # normalisation, so compare ratios
+    assert e[m] > 0.8
+    assert e[m - 1] < 0.02 * e[m] and e[m + 1] < 0.02 * e[m]
+    want_m = _facet_model_energy(lam_b, m, c, c, 0.0, d, 1.0)
+    assert abs(e[m] - want_m) <= 1e-3 * want_m
+    # exact U = 0 for order m: the intensity ratio of m to an order far off the
+    # blaze, with the obliquity cos^2 c for m, fixes the normalisation
+    far = lo + 3
+    sb = far * lam_b / d - math.sin(c)
+    b = math.asin(sb)
+    U = math.pi * d * math.cos(b) / math.cos(c - b) * (math.sin(c - c) + math.sin(c - b)) / lam_b
+    sinc = math.sin(U) / U if abs(U) >= math.pi / 2 else (0.00761 * U * U - 0.16605) * U * U + 1
+    i_far = (math.cos(b) * math.cos(c - c) / math.cos(c - b) * sinc) ** 2
+    e_far = O.grating_energy(np.array([lam_b]), far, c, c, 0.0, d)[0]
+    assert abs((e[m] / e_far) - (math.cos(c) ** 2 / i_far)) <= 1e-9 * (e[m] / e_far)
+    # adjacent orders trade places between their blaze wavelengths: at the
+    # wavelength where their angles straddle the blaze angle symmetrically
+    # (sin b_m + sin b_(m+1) = 2 sin c) both carry a large, similar share (the
+    # facet's obliquity and illuminated length make the two sides unequal by
+    # ~10%), and each dominates at its own blaze wavelength
+    lam_x = 4 * d * math.sin(c) / (2 * m + 1)
+    em = O.grating_energy(np.array([lam_x]), m, c, c, 0.0, d)[0]
+    em1 = O.grating_energy(np.array([lam_x]), m + 1, c, c, 0.0, d)[0]
+    assert min(em, em1) > 0.3 and abs(em - em1) <= 0.15 * max(em, em1), (em, em1)
+    lam_b1 = 2 * d * math.sin(c) / (m + 1)
+    assert O.grating_energy(np.array([lam_b1]), m + 1, c, c, 0.0, d)[0] > \
+        50 * O.grating_energy(np.array([lam_b1]), m, c, c, 0.0, d)[0]
+    assert hi > lo
+
+
+def test_flux_weight_is_photons_from_codata():
+    """The CCD flux weight (compute_ccd_image, ccd.cu:227,286-287,360):
+    photon rate = spectrum x lambda / (h c) x QE x pixel area, with the
+    spectrum in erg/s/cm^2/um-like units, lambda in um, the pixel area in cm^2
+    (mm^2 x 1e-2).  h and c from CODATA (scipy.constants) give hc in erg um to
+    within 1e-6 of the reference's constant; for a monochromatic beam with a
+    constant QE and spectrum the frame's sum is N_hits x that weight, and
+    each lit pixel holds an integer multiple of it.  Pixels of unequal sides
+    catch a squared side, a scaled hc or a dropped 1e-2."""
+    import dataclasses
+
+    from scipy import constants as K
+
+    from rt_amd import schemes
+    hc = K.h * K.c * 1e7 * 1e6                          # J m -> erg um
+    assert abs(hc / 1.98644560233e-12 - 1.0) < 1e-6
+    base = schemes.czerny_turner(n_per_lambda=3000, nlambda=1)
+    surfs = list(base.surfaces)
+    xpix, ypix, qe, spec = 0.0135, 0.02, 0.8, 2.5
+    surfs[-1] = dataclasses.replace(surfs[-1], size=(1024 * xpix, 700 * ypix),
+                                    ccd=dict(xdim=1024, ydim=700, xpix=xpix, ypix=ypix), qe=qe, spectrum=spec)
+    beam = dataclasses.replace(base.beam, range_distr=S.RT_RANGE_MONO, range=(0.54, 0.56), composite=False,
+                               nlambda=1)
+    sch = S.Scheme(surfs, beam, name="ct-mono")
+    res = O.trace(sch, seed=4, hist=True, flux=True)
+    lam = (0.56 + 0.54) / 2.0
+    w = spec * lam / hc * qe * (xpix * ypix * 1e-2)
+    hits = int(res.stats[S.ST_CCD_HITS])
+    assert hits > 2000
+    assert abs(res.flux.sum() - hits * w) <= 1e-6 * hits * w
+    lit = res.hist > 0
+    assert np.allclose(res.flux[lit], res.hist[lit] * w, rtol=1e-6, atol=0)
+    assert np.all(res.flux[~lit] == 0.0)
+
+
 # ---------------------------------------------------------------- stage order
 
 def test_fused_trace_is_the_stage_composition():
