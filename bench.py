@@ -8,8 +8,14 @@ CCD.  All buffers are resident in HBM before the timed region.
 
 value = sum over non-auxiliary surfaces of the rays alive on entering that
 surface's intersection stage (read from the kernel's own counters), divided
-by the timed wall time of the K steps.  The CPU baseline is the oracle's
-fused trace on a bounded sample of the same rays on the host cores.
+by the timed wall time of the K steps.  Each step is also bracketed by HIP
+events on its stream: the roofline uses the median step time (K >= 50).
+The CPU baseline is the oracle's fused trace (built for speed: -O3, AVX2,
+strict IEEE) on the same rays on the host cores, the median of 5 timed passes
+after a warm-up pass.  The default line also carries, in detail, the
+adjoint/forward time ratios of the differentiable configurations (one scheme
+at 1e6 rays; 1024 tolerance instances), (loss forward + adjoint) / plain
+forward on the same rays.
 """
 from __future__ import annotations
 
@@ -48,12 +54,15 @@ def measured_traffic(block, waves):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n-per-lambda", type=int, default=4883)
     ap.add_argument("--nlambda", type=int, default=2048)
     ap.add_argument("--cpu-sample", type=int, default=0, help="rays traced by the CPU baseline (0: the whole workload)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="leave the adjoint ratios out of the default line")
+    ap.add_argument("--adjoint-instances", type=int, default=1024,
+                    help="tolerance instances of the batched adjoint ratio in the default line (0: none)")
     ap.add_argument("--launch", default="", metavar="BLOCK,WAVES,GRID_MULT",
                     help="launch configuration of the generated kernel instead of the autotuner's choice")
     ap.add_argument("--batch", type=int, default=0, metavar="N_INST",
@@ -129,26 +138,28 @@ def main():
     for k in range(args.warmup):
         step(1000 + k)
     torch.cuda.synchronize()
+    generated = lib.last_kernel() == "generated"
+    table = generated and os.environ.get("RT_HIP_LTAB", "1") != "0"
     stats.zero_()
     hist.zero_()
     flux.zero_()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev0.record(stream)
+    evs[0].record(stream)
     for k in range(args.steps):
         step(k)
-    ev1.record(stream)
+        evs[k + 1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    gpu_ms = ev0.elapsed_time(ev1)
+    step_ms = np.array([evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)])
     st = stats.cpu().numpy().view(np.uint64)
     ray_surf = int(sum(int(st[S.ST_SURF + 4 * s]) for s in range(sch.n_surf)))
     ms_per_step = wall * 1e3 / args.steps
     value = ray_surf / wall
-    kernel_ms = gpu_ms / args.steps
-    fl = flops.trace_flops(sch, st) / args.steps
+    kernel_ms = float(np.median(step_ms))
+    fl = flops.trace_flops(sch, st, generated_kernel=generated, table=table) / args.steps
+    fl_splines = flops.trace_flops(sch, st, generated_kernel=generated, table=False) / args.steps
     achieved_tflops = fl / (kernel_ms * 1e-3) / 1e12
 
     out = {
@@ -170,31 +181,75 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
                      "traffic": measured_traffic(block, waves)},
-        "detail": {"kernel_ms": kernel_ms, "launch": {"block": block, "waves_per_simd": waves,
-                                                      "grid_mult": grid_mult,
-                                                      "surface_unroll": "full" if ds.get_unroll() else "loop",
-                                                      "autotune_sample_ms": tuned_ms},
+        "detail": {"kernel_ms_median": kernel_ms, "kernel_ms_mean": float(step_ms.mean()),
+                   "kernel_ms_min": float(step_ms.min()), "timed_launches": args.steps,
+                   "launch": {"block": block, "waves_per_simd": waves, "grid_mult": grid_mult,
+                              "surface_unroll": "full" if ds.get_unroll() else "loop",
+                              "kernel": "generated" if generated else "generic",
+                              "per_wavelength_table": table, "autotune_sample_ms": tuned_ms},
                    "ray_surface_per_step": ray_surf // args.steps,
                    "ccd_hits_per_step": int(st[S.ST_CCD_HITS]) // args.steps,
-                   "flop_per_step": fl},
+                   "ccd_edge_per_step": int(st[S.ST_CCD_EDGE]) // args.steps,
+                   "flop_per_step": fl,
+                   "flop_per_step_splines_counted": fl_splines,
+                   "frac_splines_counted": fl_splines / (kernel_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
     }
+    ds.close()
+    del rays, flag, hist, flux
+
+    if not args.no_adjoint:
+        out["detail"]["adjoint_single"] = adjoint_ratio(args.warmup, max(args.steps, 20))
+        if args.adjoint_instances:
+            out["detail"]["adjoint_batched"] = batch_adjoint_ratio(args.adjoint_instances, args.warmup, 5,
+                                                                   args.max_rays or 1_000_000)
 
     if not args.no_cpu:
         from oracle import oracle as O
         m = min(args.cpu_sample, n) if args.cpu_sample > 0 else n
         threads = min(HOST_CPU_SHARE, len(os.sched_getaffinity(0)))
-        t0 = time.perf_counter()
-        r = O.trace(sch, seed=0, n=m, nthreads=threads)
-        cpu_s = time.perf_counter() - t0
+        O.trace(sch, seed=0, n=m, nthreads=threads, fast=True)             # warm-up pass
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            r = O.trace(sch, seed=0, n=m, nthreads=threads, fast=True)
+            times.append(time.perf_counter() - t0)
+        cpu_s = float(np.median(times))
         cst = r.stats
         cpu_rs = int(sum(int(cst[S.ST_SURF + 4 * s]) for s in range(sch.n_surf)))
         out["cpu_baseline"] = {"value": cpu_rs / cpu_s, "unit": "ray*surface/s", "cores": threads,
                                "kind": "port",
-                               "sample": (f"the whole workload ({m} rays)" if m == n else f"first {m} rays of the same beam") + ", oracle fused trace, one pass"}
+                               "sample": (f"the whole workload ({m} rays)" if m == n
+                                          else f"first {m} rays of the same beam") +
+                                         ", oracle fused trace built -O3 -march=x86-64-v3 (strict IEEE), "
+                                         f"median of 5 passes after a warm-up on {threads} threads of "
+                                         f"{cpu_model()}",
+                               "pass_seconds": times}
     print(json.dumps(out))
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def bench_adjoint(args):
+    r = adjoint_ratio(args.warmup, args.steps)
+    print(json.dumps({
+        "metric": "adjoint/forward time ratio (echelle, 1e6 rays, 48 parameters)",
+        "value": r["ratio"], "unit": "x", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": False, "dtype": "f64",
+        "data": "synthetic (Philox beam)",
+        "config": {"workload": "echelle-adjoint", "rays": r["rays"], "wavelengths": 64,
+                   "parameters": r["parameters"]},
+        "detail": r}))
+
+
+def adjoint_ratio(warmup, steps):
     """Adjoint/forward time ratio on the differentiable configuration: the
     echelle at 15625 rays x 64 wavelengths (1e6 rays), loss = sum over the
     wavelengths of the RMS spot radius, gradient over 48 design parameters.
@@ -235,19 +290,15 @@ def bench_adjoint(args):
         return e0.elapsed_time(e1) / k
 
     plain = lambda: ds.trace(None, None, seed=5, n=n, flags=S.RT_TRACE_GENERATE)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         plain(); loss_forward(); ds.adjoint(seeds, ps, pf, seed=5)
-    t_plain = timed(plain, args.steps)
-    t_loss = timed(loss_forward, args.steps)
-    t_adj = timed(lambda: ds.adjoint(seeds, ps, pf, seed=5), args.steps)
-    print(json.dumps({
-        "metric": "adjoint/forward time ratio (echelle, 1e6 rays, 48 parameters)",
-        "value": (t_loss + t_adj) / t_plain, "unit": "x", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "higher_is_better": False, "dtype": "f64",
-        "data": "synthetic (Philox beam)",
-        "config": {"workload": "echelle-adjoint", "rays": n, "wavelengths": 64, "parameters": len(spec)},
-        "detail": {"forward_ms": t_plain, "loss_forward_ms": t_loss, "adjoint_ms": t_adj,
-                   "adjoint_kernel_ratio": t_adj / t_plain}}))
+    t_plain = timed(plain, steps)
+    t_loss = timed(loss_forward, steps)
+    t_adj = timed(lambda: ds.adjoint(seeds, ps, pf, seed=5), steps)
+    ds.close()
+    return {"ratio": (t_loss + t_adj) / t_plain, "rays": n, "parameters": len(spec), "forward_ms": t_plain,
+            "loss_forward_ms": t_loss, "adjoint_ms": t_adj, "adjoint_kernel_ratio": t_adj / t_plain,
+            "definition": "(loss forward + adjoint) / plain forward, same rays and surfaces"}
 
 
 def bench_batch(args):
@@ -295,6 +346,19 @@ def bench_batch(args):
 
 
 def bench_batch_adjoint(args):
+    r = batch_adjoint_ratio(args.batch_adjoint, args.warmup, args.steps, args.max_rays or 1_000_000)
+    print(json.dumps({
+        "metric": f"batched adjoint/forward time ratio ({r['instances']} echelle tolerance instances, "
+                  "48 parameters)",
+        "value": r["ratio"], "unit": "x", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": False, "dtype": "f64",
+        "data": "synthetic (Philox beams, seed + instance)",
+        "config": {"workload": "echelle-tolerance-batch-adjoint", "instances": r["instances"], "rays": r["rays"],
+                   "wavelengths": 64, "parameters": r["parameters"]},
+        "detail": r}))
+
+
+def batch_adjoint_ratio(n_inst, warmup, steps, max_rays):
     """Batched adjoint/forward ratio: N_INST tolerance instances of the echelle
     (64 wavelengths, ray counts log-uniform in [1e4, --max-rays]), the spot
     loss per instance (sum over wavelengths of the RMS spot radius), gradient of
@@ -310,8 +374,7 @@ def bench_batch_adjoint(args):
     from rt_amd import scheme as S
     base = schemes.echelle(n_per_lambda=4883, nlambda=64)
     spec = diff.echelle_design_params(base)
-    inst, counts = schemes.tolerance_instances(base, args.batch_adjoint, seed=1, min_rays=10_000,
-                                               max_rays=args.max_rays or 1_000_000)
+    inst, counts = schemes.tolerance_instances(base, n_inst, seed=1, min_rays=10_000, max_rays=max_rays)
     db = lib.DeviceBatch(inst, counts)
     total = int(sum(counts))
     rays = torch.zeros((7, total), dtype=torch.float64, device="cuda")
@@ -345,28 +408,24 @@ def bench_batch_adjoint(args):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / k
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         plain(); loss_forward(); adjoint()
     torch.cuda.synchronize()
     stats.zero_()
-    t_plain = timed(plain, args.steps)
+    t_plain = timed(plain, steps)
     st = stats.cpu().numpy().view(np.uint64)
-    ray_surf = int(sum(int(st[S.ST_SURF + 4 * s]) for s in range(base.n_surf))) // args.steps
-    t_loss = timed(loss_forward, args.steps)
-    t_adj = timed(adjoint, args.steps)
+    ray_surf = int(sum(int(st[S.ST_SURF + 4 * s]) for s in range(base.n_surf))) // steps
+    t_loss = timed(loss_forward, steps)
+    t_adj = timed(adjoint, steps)
     g = grad.cpu().numpy()
-    print(json.dumps({
-        "metric": f"batched adjoint/forward time ratio ({len(inst)} echelle tolerance instances, 48 parameters)",
-        "value": (t_loss + t_adj) / t_plain, "unit": "x", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "higher_is_better": False, "dtype": "f64",
-        "data": "synthetic (Philox beams, seed + instance)",
-        "config": {"workload": "echelle-tolerance-batch-adjoint", "instances": len(inst), "rays": total,
-                   "min_rays": int(min(counts)), "max_rays": int(max(counts)), "wavelengths": 64,
-                   "parameters": len(ps)},
-        "detail": {"plain_forward_ms": t_plain, "loss_forward_ms": t_loss, "adjoint_ms": t_adj,
-                   "adjoint_kernel_ratio": t_adj / t_plain, "ray_surface_per_step": ray_surf,
-                   "forward_ray_surface_per_s": ray_surf / (t_plain * 1e-3),
-                   "finite_gradients": bool(np.all(np.isfinite(g)))}}))
+    db.close()
+    return {"ratio": (t_loss + t_adj) / t_plain, "instances": len(inst), "rays": total,
+            "min_rays": int(min(counts)), "max_rays": int(max(counts)), "parameters": len(ps),
+            "plain_forward_ms": t_plain, "loss_forward_ms": t_loss, "adjoint_ms": t_adj,
+            "adjoint_kernel_ratio": t_adj / t_plain, "ray_surface_per_step": ray_surf,
+            "forward_ray_surface_per_s": ray_surf / (t_plain * 1e-3),
+            "finite_gradients": bool(np.all(np.isfinite(g))),
+            "definition": "(loss forward + adjoint) / plain forward, same rays and surfaces"}
 
 
 def bench_design(args):

@@ -27,6 +27,14 @@ public:
     /* value at xval; throws bad_tab_func(OutOfRange) outside the table */
     real_t operator[](const real_t &xval);
 
+    /* Read access for packing the function into the engine's spline pool
+       (rt_amd's Scheme): not in common-lib's interface, inline so the
+       class layout and exported symbols are unchanged. */
+    bool IsSingleValue() const { return SingleValue; }
+    const std::vector<real_t> &Abscissae() const { return x; }
+    const std::vector<real_t> &Values() const { return y; }
+    const std::vector<real_t> &SecondDerivatives() const { return dy2; }
+
     class bad_tab_func : public std::exception {
     public:
         bad_tab_func(TabulatedFunctionError err);

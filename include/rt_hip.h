@@ -190,6 +190,61 @@ int rt_hip_trace(const rt_hip_scheme *scheme, const double *beam_desc, uint64_t 
                  double *dump_dev, signed char *dump_flag_dev,
                  void *stream);
 
+/* ---- Scheme::run's saved beam and log (common-lib/scheme.cpp:983-1066) -- */
+
+/* Loss codes of a recording trace: 0 for a ray alive after the last surface,
+   RT_LOSS_CODE(s, k) for a ray lost at surface s in stage k, RT_LOSS_DEAD
+   for a ray that entered dead.  Extents: RT_EXT_PER_SURF doubles per
+   surface, min and max of cZ after the transform's normalisation, of X and Y
+   after the intersection, of X and Y after the aperture, of cZ after the
+   action's normalisation, over the rays alive at that point (NaN where none
+   is): what Scheme::run logs (scheme.cpp:993-1042). */
+enum rt_loss_stage { RT_LOSS_INTERSECTION = 0, RT_LOSS_APERTURE = 1, RT_LOSS_ACTION = 2 };
+#define RT_LOSS_CODE(s, k) (1 + 3 * (s) + (k))
+#define RT_LOSS_DEAD 255
+enum { RT_EXT_PER_SURF = 12 };
+
+/*
+ * rt_hip_trace with RT_TRACE_GENERATE / RT_TRACE_STORE semantics for rays
+ * [0, n_rays), plus the loss code of every ray (loss, n bytes, device) and,
+ * when `extents` (device, n_surf x RT_EXT_PER_SURF doubles) is not NULL, the
+ * logged extents.  No CCD frame (RT_TRACE_HIST / RT_TRACE_FLUX are refused).
+ * The extents are combined per workgroup with integer min / max on
+ * order-preserving keys (bit-exact whatever the order).  Replaces the
+ * stage loop of Scheme::run (scheme.cpp:983-1046) with its log
+ * (GetBeamDim / GetBeamCosins, beam.cpp:490-523, over all good rays).
+ */
+int rt_hip_trace_record(const rt_hip_scheme *scheme, const double *beam_desc, uint64_t seed, size_t n_rays,
+                        double *X_dev, double *Y_dev, double *Z_dev, double *cX_dev, double *cY_dev,
+                        double *cZ_dev, double *lambda_dev, signed char *flag_dev, int flags,
+                        unsigned long long *stats_dev, unsigned char *loss_dev, double *extents_dev,
+                        void *stream);
+
+/*
+ * The order Beam::Rearrange (common-lib/beam.cpp:574-598) leaves a beam in
+ * after every stage of Scheme::run that lost rays (in stage order), from the
+ * loss codes of a recording trace: perm_dev (device, n uint64) receives the
+ * ray at each position, the good rays first; counts (host, 256 uint64)
+ * receives the number of rays of every loss code (counts[0]: good rays).
+ * Waits for the stream (the counts drive the stage launches).  Refuses
+ * RT_LOSS_DEAD rays.
+ */
+int rt_hip_rearrange(const unsigned char *loss_dev, size_t n, uint64_t *perm_dev, uint64_t *counts, void *stream);
+
+/*
+ * The arrays of Beam::Save (beam.cpp:636-715) in file order: out_dev[k * n +
+ * p] = array k (X, Y, Z, cX, cY, cZ, lambda) of ray perm_dev[p].  A lost
+ * ray's cosines first go renorm[loss code] times through Beam::Normalize's
+ * scaled-norm formula (beam.cpp:541-569, reference operation order, IEEE
+ * division and square root, no contraction): the reference normalises every
+ * ray, lost ones included, twice per later surface (once for an AUX one).
+ * renorm: host, 256 entries.
+ */
+int rt_hip_beam_order(size_t n, const uint64_t *perm_dev, const unsigned char *loss_dev,
+                      const unsigned char *renorm, const double *X_dev, const double *Y_dev,
+                      const double *Z_dev, const double *cX_dev, const double *cY_dev, const double *cZ_dev,
+                      const double *lambda_dev, double *out_dev, void *stream);
+
 /* ---- per-stage kernels (the plugin API of rt_engine_func.h runs on these) --
    All ray arrays are device SoA buffers of n elements; `rec` is one packed
    surface record on the host; n_bad_dev / iters_dev (nullable) are device
@@ -245,6 +300,20 @@ int rt_hip_ccd_accumulate(size_t n, const unsigned *ccdX, const unsigned *ccdY, 
                           unsigned long long *offchip_dev, void *stream);
 /* surface_QE multiply (rt_engine_func.h:73; rt_cuda_engine_surface.cu:1775-1784): v *= f */
 int rt_hip_scale(size_t n, double *v, const double *f, void *stream);
+
+/* ccd_frame's work per result file (ccd/ccd_frame.cpp:190-195) in one
+   kernel: each ray's pixel (ccd_coordinates, rt_cuda_engine_ccd.cu:30-45),
+   its weight spec(lambda) lambda / hc QE(lambda) xpix ypix 1e-2 (ccd.cu:
+   286-287,360; QE and spectrum as host splines [K, x, y, y2], or NULL for
+   the constants) added into image_dev (ydim x xdim FP64, device) with atomics
+   and a bounds check (compute_ccd_image, ccd.cu:219-232).  counts_dev
+   (device, 3 uint64, added to, nullable): rays added, rays off the frame,
+   rays whose lambda is outside a table (the reference's bad_tab_func; their
+   weight is not added).  Waits for the stream. */
+int rt_hip_ccd_flux(size_t n, const double *X_dev, const double *Y_dev, const double *lambda_dev,
+                    const double *qe_spline, double qe_const, const double *spec_spline, double spec_const,
+                    double xsize, double ysize, double xpix, double ypix, size_t xdim, size_t ydim,
+                    double *image_dev, unsigned long long *counts_dev, void *stream);
 /* grating_energy_distr (rt_engine_func.h:75-78; rt_cuda_engine_surface.cu:2098-2290) */
 int rt_hip_grating_energy(size_t n, const double *lambda, long order, double blaze, double alpha,
                           double gamma, double gr_const, double rule_start, double rule_stop,
@@ -445,7 +514,7 @@ int rt_hip_last_kernel(char *buf, size_t len);
    table); `text` receives the generated source
    and the compiler log, *code_size the code object size.  RT_OK when it
    compiles. */
-enum { RT_JIT_BATCH = 0x100, RT_JIT_ADJOINT = 0x200, RT_JIT_LTAB = 0x400 };
+enum { RT_JIT_BATCH = 0x100, RT_JIT_ADJOINT = 0x200, RT_JIT_LTAB = 0x400, RT_JIT_RECORD = 0x800 };
 int rt_hip_jit_check(const double *surf_table, int n_surf, const double *spline_pool,
                      size_t pool_len, int beam_mode, char *text, size_t len, size_t *code_size);
 

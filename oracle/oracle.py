@@ -22,33 +22,54 @@ _i8p = C.POINTER(C.c_int8)
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.environ.get("RT_ORACLE_LIB") or os.path.join(_HERE, "librt_oracle.so")
-        if not os.path.exists(path):
-            raise RuntimeError("oracle/librt_oracle.so is not built (run `make -C oracle`)")
-        L = C.CDLL(path)
-        L.oracle_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
-        L.oracle_u01.argtypes = [C.c_uint32, C.c_uint32]
-        L.oracle_u01.restype = C.c_double
-        L.oracle_spline_init.argtypes = [C.c_size_t, _dp, _dp, _dp]
-        L.oracle_spline_eval.argtypes = [C.c_size_t, _dp, _dp, _dp, C.c_double, C.POINTER(C.c_int)]
-        L.oracle_spline_eval.restype = C.c_double
-        L.oracle_gen_ray.argtypes = [_dp, C.c_uint64, C.c_uint64] + [_dp] * 7
-        L.oracle_trace.argtypes = [_dp, C.c_int, _dp, _dp, C.c_uint64, C.c_size_t, C.c_size_t,
-                                   _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i8p, C.c_int,
-                                   _u32p, _dp, _u64p, _dp, _i8p, C.c_int]
-        L.oracle_trace_iters.argtypes = L.oracle_trace.argtypes + [_u32p, C.c_int]
-        L.oracle_beam_transform.argtypes = [C.c_size_t, _dp, _dp] + [_dp] * 6
-        L.oracle_normalize.argtypes = [C.c_size_t, _dp, _dp, _dp]
-        L.oracle_surface_intersection.argtypes = [_dp, C.c_size_t] + [_dp] * 6 + [_u64p, _i8p, _u64p]
-        L.oracle_apply_mask.argtypes = [_dp, C.c_size_t, _dp, _dp, _u64p, _i8p]
-        L.oracle_surface_reflection.argtypes = [_dp, C.c_size_t] + [_dp] * 6
-        L.oracle_surface_refraction.argtypes = [_dp, C.c_size_t] + [_dp] * 7 + [_u64p, _i8p]
-        L.oracle_surface_diffraction.argtypes = [_dp, C.c_size_t] + [_dp] * 8 + [_u64p, _i8p]
-        L.oracle_ccd_coordinates.argtypes = [C.c_size_t, _dp, _dp, C.c_double, C.c_double,
-                                             C.c_double, C.c_double, _u32p, _u32p]
-        L.oracle_grating_energy_distr.argtypes = [C.c_size_t, _dp, C.c_long] + [C.c_double] * 6 + [_dp]
-        _LIB = L
+        _LIB = _load(os.environ.get("RT_ORACLE_LIB") or os.path.join(_HERE, "librt_oracle.so"))
     return _LIB
+
+
+_FAST = None
+
+
+def fast_lib():
+    """The oracle built for speed (oracle/librt_oracle_fast.so: -O3, AVX2,
+    strict IEEE): the timed CPU baseline of bench.py."""
+    global _FAST
+    if _FAST is None:
+        _FAST = _load(os.path.join(_HERE, "librt_oracle_fast.so"))
+    return _FAST
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built (run `make -C oracle`)")
+    L = C.CDLL(path)
+    L.oracle_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+    L.oracle_u01.argtypes = [C.c_uint32, C.c_uint32]
+    L.oracle_u01.restype = C.c_double
+    L.oracle_spline_init.argtypes = [C.c_size_t, _dp, _dp, _dp]
+    L.oracle_spline_eval.argtypes = [C.c_size_t, _dp, _dp, _dp, C.c_double, C.POINTER(C.c_int)]
+    L.oracle_spline_eval.restype = C.c_double
+    L.oracle_gen_ray.argtypes = [_dp, C.c_uint64, C.c_uint64] + [_dp] * 7
+    L.oracle_trace.argtypes = [_dp, C.c_int, _dp, _dp, C.c_uint64, C.c_size_t, C.c_size_t,
+                               _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i8p, C.c_int,
+                               _u32p, _dp, _u64p, _dp, _i8p, C.c_int]
+    L.oracle_trace_iters.argtypes = L.oracle_trace.argtypes + [_u32p, C.c_int]
+    L.oracle_beam_transform.argtypes = [C.c_size_t, _dp, _dp] + [_dp] * 6
+    L.oracle_normalize.argtypes = [C.c_size_t, _dp, _dp, _dp]
+    L.oracle_surface_intersection.argtypes = [_dp, C.c_size_t] + [_dp] * 6 + [_u64p, _i8p, _u64p]
+    L.oracle_apply_mask.argtypes = [_dp, C.c_size_t, _dp, _dp, _u64p, _i8p]
+    L.oracle_surface_reflection.argtypes = [_dp, C.c_size_t] + [_dp] * 6
+    L.oracle_surface_refraction.argtypes = [_dp, C.c_size_t] + [_dp] * 7 + [_u64p, _i8p]
+    L.oracle_surface_diffraction.argtypes = [_dp, C.c_size_t] + [_dp] * 8 + [_u64p, _i8p]
+    L.oracle_ccd_coordinates.argtypes = [C.c_size_t, _dp, _dp, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, _u32p, _u32p]
+    L.oracle_grating_energy_distr.argtypes = [C.c_size_t, _dp, C.c_long] + [C.c_double] * 6 + [_dp]
+    _u8p = C.POINTER(C.c_uint8)
+    L.oracle_trace_record.argtypes = ([_dp, C.c_int, _dp, _dp, C.c_uint64, C.c_size_t] + [_dp] * 7 +
+                                      [_i8p, _u64p, _u8p, _dp, C.c_int])
+    L.oracle_rearrange.argtypes = [_u8p, C.c_size_t, _u64p, _u64p]
+    L.oracle_beam_order.argtypes = [C.c_size_t, _u64p, _u8p, _u8p] + [_dp] * 8
+    L.oracle_beam_order.restype = None
+    return L
 
 
 def _p(a, t=_dp):
@@ -102,7 +123,7 @@ class TraceResult:
 
 
 def trace(scheme, seed=0, begin=0, n=None, rays=None, flag=None, flags=None, hist=False,
-          flux=False, dump=False, nthreads=0, iters=None, iters_mode=0):
+          flux=False, dump=False, nthreads=0, iters=None, iters_mode=0, fast=False):
     """Run the oracle's fused trace. `scheme` is an rt_amd.scheme.Scheme.
     With rays=None the beam is generated from (scheme.beam, seed).
     iters (n_surf x n uint32) with iters_mode 1 receives the Newton iteration
@@ -140,7 +161,8 @@ def trace(scheme, seed=0, begin=0, n=None, rays=None, flag=None, flags=None, his
     rows = [R[i] for i in range(7)]
     if iters is not None:
         assert iters.dtype == np.uint32 and iters.shape == (scheme.n_surf, n) and iters.flags.c_contiguous
-    rc = lib().oracle_trace_iters(_p(table), scheme.n_surf, _p(pool), _p(beam), seed, begin, n,
+    rc = (fast_lib() if fast else lib()).oracle_trace_iters(_p(table), scheme.n_surf, _p(pool), _p(beam), seed,
+                                                            begin, n,
                                   *[_p(r) for r in rows], _p(flag, _i8p), f,
                                   _p(H, _u32p), _p(F), _p(st, _u64p), _p(D), _p(DF, _i8p), nthreads,
                                   _p(iters, _u32p), int(iters_mode) if iters is not None else 0)
@@ -241,3 +263,87 @@ def grating_energy(lam, order, blaze, alpha, gamma, gr_const, rule_start=0.0, ru
     lib().oracle_grating_energy_distr(len(lam), _p(lam), order, blaze, alpha, gamma, gr_const,
                                       rule_start, rule_stop, _p(e))
     return e
+
+
+# ---- Scheme::run's saved beam (rt_hip_trace_record, rt_hip_rearrange,
+# rt_hip_beam_order semantics)
+
+EXT_PER_SURF = 12
+LOSS_DEAD = 255
+
+
+class RecordResult:
+    """rays (7 x n), flag (n), loss codes (n, uint8), extents (n_surf x 12),
+    stats (uint64 counters), status"""
+
+    def __init__(self, rays, flag, loss, extents, stats, status):
+        self.rays, self.flag, self.loss, self.extents, self.stats = rays, flag, loss, extents, stats
+        self.status = status
+
+
+def trace_record(scheme, seed=0, n=None, beam=None, nthreads=0):
+    """oracle_trace_record: the generated beam of `seed` traced with loss
+    codes and the logged extents"""
+    table = np.ascontiguousarray(scheme.table, dtype=np.float64)
+    pool = np.ascontiguousarray(scheme.pool, dtype=np.float64)
+    b = np.ascontiguousarray(scheme.beam.pack() if beam is None else beam, dtype=np.float64)
+    n = scheme.beam.n_rays if n is None else n
+    rays = np.zeros((7, n))
+    flag = np.zeros(n, dtype=np.int8)
+    loss = np.zeros(n, dtype=np.uint8)
+    ext = np.zeros((scheme.n_surf, EXT_PER_SURF))
+    stats = np.zeros(144, dtype=np.uint64)
+    u8 = C.POINTER(C.c_uint8)
+    rc = lib().oracle_trace_record(_p(table), scheme.n_surf, _p(pool), _p(b), seed, n,
+                                   *[_p(rays[k]) for k in range(7)], _p(flag, _i8p), _p(stats, _u64p),
+                                   _p(loss, u8), _p(ext), nthreads)
+    return RecordResult(rays, flag, loss, ext, stats, rc)
+
+
+def rearrange(loss):
+    """oracle_rearrange: (perm, counts[256])"""
+    loss = np.ascontiguousarray(loss, dtype=np.uint8)
+    n = len(loss)
+    perm = np.zeros(n, dtype=np.uint64)
+    counts = np.zeros(256, dtype=np.uint64)
+    rc = lib().oracle_rearrange(_p(loss, C.POINTER(C.c_uint8)), n, _p(perm, _u64p), _p(counts, _u64p))
+    if rc != 0:
+        raise RuntimeError(f"oracle_rearrange failed ({rc})")
+    return perm, counts
+
+
+def renorm_table(scheme):
+    """How many more times Beam::Normalize (beam.cpp:527-571) touches a ray
+    lost with each code: Scheme::run normalises every ray after each
+    surface's transform and, on surfaces other than AUX, after its action
+    (scheme.cpp:993,1040); a ray lost at surface s meets that surface's
+    second call and every call of the surfaces after it."""
+    types = [int(scheme.table[s * 56 + 1]) for s in range(scheme.n_surf)]
+    table = np.zeros(256, dtype=np.uint8)
+    for s in range(scheme.n_surf):
+        later = sum(1 if t == 6 else 2 for t in types[s + 1:])
+        for k in range(3):
+            table[1 + 3 * s + k] = 1 + later
+    return table
+
+
+def beam_order(perm, loss, renorm, rays):
+    """oracle_beam_order: the 7 x n arrays of Beam::Save in file order"""
+    n = len(perm)
+    out = np.zeros((7, n))
+    rays = np.ascontiguousarray(rays, dtype=np.float64)
+    u8 = C.POINTER(C.c_uint8)
+    lib().oracle_beam_order(n, _p(np.ascontiguousarray(perm, dtype=np.uint64), _u64p),
+                            _p(np.ascontiguousarray(loss, dtype=np.uint8), u8),
+                            _p(np.ascontiguousarray(renorm, dtype=np.uint8), u8),
+                            *[_p(rays[k]) for k in range(7)], _p(out))
+    return out
+
+
+def dat_bytes(n_rays, n_good, n_lambda, ordered):
+    """Beam::Save's bytes (beam.cpp:636-715): three size_t, then the seven
+    arrays over the good rays, then over the rest"""
+    hdr = np.array([n_rays, n_good, n_lambda], dtype=np.uint64).tobytes()
+    good = np.ascontiguousarray(ordered[:, :n_good]).tobytes()
+    rest = np.ascontiguousarray(ordered[:, n_good:]).tobytes()
+    return hdr + good + rest

@@ -621,18 +621,29 @@ typedef struct {
 
 #define SET_FN_ERR(e) do { if (e) { spl_oob = 1; } } while (0)
 
+/* extents of the scheme log (scheme.cpp:993-1042): e[2q] min, e[2q+1] max */
+static void ext_update(double *e, int q, double v)
+{
+    if (v < e[2 * q]) e[2 * q] = v;
+    if (v > e[2 * q + 1]) e[2 * q + 1] = v;
+}
+
 static void trace_one(const double *table, int n_surf, const double *pool,
                       ray_t *r, int *alive_io, int flags, uint64_t ray_local,
                       size_t n, uint32_t *hist, double *flux, stats_t *st,
-                      double *dump, signed char *dump_flag, uint32_t *iters, int iters_mode)
+                      double *dump, signed char *dump_flag, uint32_t *iters, int iters_mode,
+                      unsigned char *loss, double *ext)
 {
     int alive = *alive_io;
+    if (loss) *loss = alive ? 0 : RT_LOSS_DEAD;
     for (int s = 0; s < n_surf; ++s) {
         const double *p = table + (size_t)s * RT_SURF_STRIDE;
         int type = (int)p[RT_S_TYPE];
+        double *e = ext ? ext + (size_t)s * RT_EXT_PER_SURF : NULL;
         if (alive) {
             transform_ray(p + RT_S_ANG, p + RT_S_DIST, r);          /* scheme.cpp:990 */
             normalize_ray(&r->cX, &r->cY, &r->cZ);                   /* :993 */
+            if (e) ext_update(e, 0, r->cZ);                          /* :995 */
             if (type != RT_AUXTYPE) {
                 uint64_t it = 0;
                 st->v[RT_ST_SURF + 4 * s + 0] += 1;
@@ -642,8 +653,12 @@ static void trace_one(const double *table, int n_surf, const double *pool,
                 st->v[RT_ST_SURF + 4 * s + 1] += it;
                 if (!alive) {
                     st->v[RT_ST_SURF + 4 * s + 2] += 1;
+                    if (loss) *loss = (unsigned char)RT_LOSS_CODE(s, RT_LOSS_INTERSECTION);
                 } else {
+                    if (e) { ext_update(e, 1, r->X); ext_update(e, 2, r->Y); }   /* :1005-1008 */
                     alive = mask_ray(p, r->X, r->Y);                  /* :1014 */
+                    if (!alive && loss) *loss = (unsigned char)RT_LOSS_CODE(s, RT_LOSS_APERTURE);
+                    if (alive && e) { ext_update(e, 3, r->X); ext_update(e, 4, r->Y); }   /* :1017-1020 */
                     if (alive) {                                      /* :1035 */
                         int spl_oob = 0, e1 = 0, e2 = 0;
                         if (type == RT_MIRROR) {
@@ -671,6 +686,8 @@ static void trace_one(const double *table, int n_surf, const double *pool,
                         }
                         if (spl_oob) { alive = 0; st->v[RT_ST_SPLINE_OOB] += 1; }
                         if (alive) normalize_ray(&r->cX, &r->cY, &r->cZ);   /* :1040 */
+                        if (!alive && loss) *loss = (unsigned char)RT_LOSS_CODE(s, RT_LOSS_ACTION);
+                        if (alive && e) ext_update(e, 5, r->cZ);             /* :1042 */
                     }
                     if (!alive) st->v[RT_ST_SURF + 4 * s + 3] += 1;
                 }
@@ -778,7 +795,7 @@ int oracle_trace_iters(const double *table, int n_surf, const double *pool,
             }
             st.v[RT_ST_RAYS] += 1;
             trace_one(table, n_surf, pool, &r, &alive, flags, i, n, hist, flux, &st, dump, dump_flag, iters,
-                      iters_mode);
+                      iters_mode, NULL, NULL);
             if (alive) st.v[RT_ST_GOOD] += 1;
             if (store || (gen && X)) {
                 if (X) { X[i] = r.X; Y[i] = r.Y; Z[i] = r.Z; cX[i] = r.cX; cY[i] = r.cY; cZ[i] = r.cZ; lambda[i] = r.lam; }
@@ -792,6 +809,114 @@ int oracle_trace_iters(const double *table, int n_surf, const double *pool,
     }
     if (stats) for (int k = 0; k < RT_STATS_LEN; ++k) stats[k] += total.v[k];
     return total.v[RT_ST_SPLINE_OOB] ? RT_ERR_BAD_VALUE : RT_OK;
+}
+
+/* oracle_trace with the loss code of every ray and the logged extents
+   (rt_hip_trace_record semantics: beam generated from `seed`, rays 0..n-1) */
+int oracle_trace_record(const double *table, int n_surf, const double *pool,
+                        const double *beam, uint64_t seed, size_t n,
+                        double *X, double *Y, double *Z, double *cX, double *cY, double *cZ,
+                        double *lambda, signed char *flag, uint64_t *stats,
+                        unsigned char *loss, double *extents, int nthreads)
+{
+    if (n_surf < 0 || n_surf > RT_MAX_SURF || !beam || !loss) return RT_ERR_BAD_VALUE;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const size_t ne = (size_t)n_surf * RT_EXT_PER_SURF;
+    double ext_all[RT_MAX_SURF * RT_EXT_PER_SURF];
+    for (size_t k = 0; k < ne; ++k) ext_all[k] = (k & 1) ? -INFINITY : INFINITY;
+    stats_t total;
+    memset(&total, 0, sizeof(total));
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        stats_t st;
+        memset(&st, 0, sizeof(st));
+        double ext[RT_MAX_SURF * RT_EXT_PER_SURF];
+        for (size_t k = 0; k < ne; ++k) ext[k] = (k & 1) ? -INFINITY : INFINITY;
+#ifdef _OPENMP
+#pragma omp for schedule(static, 4096)
+#endif
+        for (long long ii = 0; ii < (long long)n; ++ii) {
+            size_t i = (size_t)ii;
+            ray_t r;
+            int alive = 1;
+            oracle_gen_ray(beam, seed, (uint64_t)i, &r.X, &r.Y, &r.Z, &r.cX, &r.cY, &r.cZ, &r.lam);
+            st.v[RT_ST_RAYS] += 1;
+            trace_one(table, n_surf, pool, &r, &alive, RT_TRACE_GENERATE, i, n, NULL, NULL, &st, NULL, NULL, NULL,
+                      0, &loss[i], ext);
+            if (alive) st.v[RT_ST_GOOD] += 1;
+            if (X) { X[i] = r.X; Y[i] = r.Y; Z[i] = r.Z; cX[i] = r.cX; cY[i] = r.cY; cZ[i] = r.cZ; lambda[i] = r.lam; }
+            if (flag) flag[i] = (signed char)alive;
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            for (int k = 0; k < RT_STATS_LEN; ++k) total.v[k] += st.v[k];
+            for (size_t k = 0; k < ne; ++k) {
+                if (k & 1) { if (ext[k] > ext_all[k]) ext_all[k] = ext[k]; }
+                else if (ext[k] < ext_all[k]) ext_all[k] = ext[k];
+            }
+        }
+    }
+    if (stats) for (int k = 0; k < RT_STATS_LEN; ++k) stats[k] += total.v[k];
+    if (extents)
+        for (size_t k = 0; k < ne; ++k) extents[k] = isinf(ext_all[k]) ? NAN : ext_all[k];
+    return total.v[RT_ST_SPLINE_OOB] ? RT_ERR_BAD_VALUE : RT_OK;
+}
+
+/* Beam::Rearrange (beam.cpp:574-598) after every stage of Scheme::run that
+   lost rays, in stage order: the reference's two-pointer loop on the flags of
+   [0, N_good), restated with indices (the reference walks char pointers).
+   perm[p] is the ray at position p; counts (256) the rays of every code. */
+int oracle_rearrange(const unsigned char *loss, size_t n, uint64_t *perm, uint64_t *counts)
+{
+    memset(counts, 0, 256 * sizeof(uint64_t));
+    for (size_t i = 0; i < n; ++i) counts[loss[i]] += 1;
+    if (counts[RT_LOSS_DEAD]) return RT_ERR_BAD_VALUE;
+    for (size_t i = 0; i < n; ++i) perm[i] = i;
+    char *fl = (char *)malloc(n ? n : 1);
+    if (!fl) return RT_ERR_BAD_ALLOC;
+    size_t N_good = n;
+    for (int code = 1; code < RT_LOSS_DEAD; ++code) {
+        if (!counts[code]) continue;
+        for (size_t p = 0; p < N_good; ++p) fl[p] = (loss[perm[p]] == code) ? 0 : 1;
+        long long start = 0, end = (long long)N_good - 1;
+        size_t i = 0;
+        while (start <= end) {
+            while (start <= end && fl[start] == 1) { ++start; ++i; }        /* find a zero from the front */
+            while (start <= end && fl[end] == 0) { --end; --N_good; }       /* find a one from the back */
+            if (start < end) {
+                fl[start] = 1;
+                fl[end] = 0;
+                uint64_t t = perm[i]; perm[i] = perm[N_good - 1]; perm[N_good - 1] = t;   /* swap_elements */
+            }
+        }
+    }
+    free(fl);
+    return RT_OK;
+}
+
+/* The arrays of Beam::Save (beam.cpp:636-715) in file order, a lost ray's
+   cosines first normalised renorm[code] more times (Beam::Normalize,
+   beam.cpp:527-571, runs over every ray); out: 7 x n */
+void oracle_beam_order(size_t n, const uint64_t *perm, const unsigned char *loss, const unsigned char *renorm,
+                       const double *X, const double *Y, const double *Z, const double *cX, const double *cY,
+                       const double *cZ, const double *lambda, double *out)
+{
+    for (size_t p = 0; p < n; ++p) {
+        const uint64_t r = perm[p];
+        double x = cX[r], y = cY[r], z = cZ[r];
+        for (int j = 0; j < renorm[loss[r]]; ++j) normalize_ray(&x, &y, &z);
+        out[0 * n + p] = X[r]; out[1 * n + p] = Y[r]; out[2 * n + p] = Z[r];
+        out[3 * n + p] = x; out[4 * n + p] = y; out[5 * n + p] = z;
+        out[6 * n + p] = lambda[r];
+    }
 }
 
 /* ------------------------------------------------------ per-stage API */

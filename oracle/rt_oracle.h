@@ -96,6 +96,21 @@ int oracle_grating_energy_distr(size_t n, const double *lambda, long order,
                                 double blaze, double alpha, double gamma, double gr_const,
                                 double rule_start, double rule_stop, double *energy);
 
+/* rt_hip_trace_record semantics: loss code per ray, extents (n_surf x 12) */
+int oracle_trace_record(const double *table, int n_surf, const double *pool,
+                        const double *beam, uint64_t seed, size_t n,
+                        double *X, double *Y, double *Z, double *cX, double *cY, double *cZ,
+                        double *lambda, signed char *flag, uint64_t *stats,
+                        unsigned char *loss, double *extents, int nthreads);
+
+/* Beam::Rearrange after every losing stage (two-pointer loop, beam.cpp:574-598) */
+int oracle_rearrange(const unsigned char *loss, size_t n, uint64_t *perm, uint64_t *counts);
+
+/* the saved beam in file order with the reference's renormalisation of lost rays */
+void oracle_beam_order(size_t n, const uint64_t *perm, const unsigned char *loss, const unsigned char *renorm,
+                       const double *X, const double *Y, const double *Z, const double *cX, const double *cY,
+                       const double *cZ, const double *lambda, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -899,6 +899,11 @@ struct TraceArgs {
     // doubles (beam fields, then the BeamUniform values), or null (the beam
     // above for every instance)
     const double* inst_beams;
+    // recording traces (kernels built with kRecLoss / kRecExt): the loss code
+    // of every ray, and the extents keys (n_surf x kExtPerSurf, ord_key
+    // encoded, combined with atomic min / max)
+    unsigned char* loss;
+    unsigned long long* ext;
 };
 constexpr int kBeamRec = (int)(sizeof(BeamArg) / sizeof(double));
 // copies of the statistics counters the trace kernels add to (TraceArgs::stats
@@ -921,10 +926,36 @@ __device__ __forceinline__ void stats_fold_body(unsigned long long* copies, unsi
     }
 }
 
+// Recording modes of the trace kernels (template argument MODE of
+// surface_step and trace_body), for Scheme::run's saved beam and log:
+// kRecLoss keeps every ray's loss code (0 alive at the end, 1 + 3 s + k for a
+// ray lost at surface s in stage k: 0 intersection, 1 aperture mask, 2 action;
+// kLossDead for a ray that entered dead); kRecExt keeps, per surface, the
+// extents the reference logs (scheme.cpp:993-1042): min and max of cZ after
+// the first normalisation, of X and Y after the intersection, of X and Y after
+// the mask, of cZ after the action's normalisation, over the rays alive there.
+constexpr int kRecLoss = 1;
+constexpr int kRecExt = 2;
+constexpr unsigned char kLossDead = 255;
+constexpr int kExtPerSurf = 12;
+
+// order-preserving integer key of a double (unsigned compare = double compare)
+__device__ __forceinline__ unsigned long long ord_key(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double key_value(unsigned long long k) {
+    const unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __builtin_bit_cast(double, u);
+}
+
 struct TraceCtx {
     cptr table;                   // device records (per-iteration copy of TraceArgs::table)
     const double* pool;           // LDS copy
     unsigned* st;                 // per-block LDS counters (RT_STATS_LEN)
+    unsigned long long* ext;      // per-block LDS extents keys (recording kernels)
+    unsigned char loss;           // loss code of the current ray (recording kernels)
     unsigned* hist;
     double* flux;
     bool do_hist, do_flux;
@@ -961,6 +992,22 @@ __device__ __forceinline__ void hist_add_wave(unsigned* hist, uint32_t pix, bool
     if (mine) atomicAdd(&hist[pix], 1u);
 }
 
+// Extents: the wave's min and max of v over the lanes with `on`, folded into
+// the block's keys e[2 q] (min) and e[2 q + 1] (max).  Called by every lane
+// of the wave (converged code).
+__device__ __forceinline__ void ext_add(unsigned long long* e, int q, bool on, double v) {
+    double lo = on ? v : __builtin_inf(), hi = on ? v : -__builtin_inf();
+    for (int m = 32; m >= 1; m >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, m));
+        hi = fmax(hi, __shfl_xor(hi, m));
+    }
+    const bool any = __ballot(on) != 0;
+    if (any && __lane_id() == 0) {
+        atomicMin(&e[2 * q], ord_key(lo));
+        atomicMax(&e[2 * q + 1], ord_key(hi));
+    }
+}
+
 // One surface of the Scheme::run stage loop (common-lib/scheme.cpp:983-1046)
 // for one ray: transform, normalize, intersection, aperture mask, action
 // (reflection, refraction or diffraction), normalize, then the CCD frame on
@@ -971,7 +1018,7 @@ __device__ __forceinline__ void hist_add_wave(unsigned* hist, uint32_t pix, bool
 // lv[col], lv[col + 1], ... in the order n1 (unless reused), n2, QE,
 // spectrum, a NaN meaning a wavelength outside the table; col < 0 evaluates
 // the splines here.
-template <class SD>
+template <class SD, int MODE = 0>
 __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& alive, TraceCtx& c,
                                              const double* lv = nullptr, int col = -1) {
     // the record's loads are issued at this surface, not hoisted ahead of
@@ -984,18 +1031,23 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
         transform_ray(d, r);                                     // scheme.cpp:990
         normalize_ray(r.cX, r.cY, r.cZ);                         // :993
     }
+    if constexpr ((MODE & kRecExt) != 0) ext_add(c.ext + s * kExtPerSurf, 0, alive, r.cZ);   // :995
     if (d.type() == RT_AUXTYPE) return;                          // :1000
     unsigned long long entering = __ballot(alive);
     if (__lane_id() == 0 && entering) atomicAdd(&c.st[RT_ST_SURF + 4 * s], (unsigned)__popcll(entering));
     unsigned it = 0;
-    bool lost_int = false, lost_act = false;
+    bool lost_int = false, lost_mask = false, lost_act = false;
     if (alive) {
         alive = intersect_ray(d, r, it);                         // :1002
         lost_int = !alive;
     }
     c.last_it = it;
+    const bool met = alive;           // alive after the intersection
+    bool passed = false;              // alive after the aperture mask
     if (alive) {
         alive = mask_ray(d, r.X, r.Y);                           // :1014
+        passed = alive;
+        lost_mask = !alive;
         if (alive) {                                             // :1035
             if (d.type() == RT_MIRROR) {
                 reflect_ray(d, r);
@@ -1031,11 +1083,26 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
                 }
             }
             if (alive) normalize_ray(r.cX, r.cY, r.cZ);          // :1040
+            lost_act = !alive;
         }
-        lost_act = !alive;
+    }
+    if constexpr ((MODE & kRecExt) != 0) {
+        // the action changes only the cosines, so X, Y are still those of
+        // the intersection (:1005-1008) and of the mask (:1017-1020)
+        unsigned long long* e = c.ext + s * kExtPerSurf;
+        ext_add(e, 1, met, r.X);
+        ext_add(e, 2, met, r.Y);
+        ext_add(e, 3, passed, r.X);
+        ext_add(e, 4, passed, r.Y);
+        ext_add(e, 5, alive, r.cZ);                              // :1042
+    }
+    if constexpr ((MODE & kRecLoss) != 0) {
+        if (lost_int) c.loss = (unsigned char)(1 + 3 * s);
+        if (lost_mask) c.loss = (unsigned char)(2 + 3 * s);
+        if (lost_act) c.loss = (unsigned char)(3 + 3 * s);
     }
     if (d.ikind() == IK_CONIC_NEWTON || d.ikind() == IK_TORIC_NEWTON) lds_add(&c.st[RT_ST_SURF + 4 * s + 1], it);
-    unsigned long long li = __ballot(lost_int), la = __ballot(lost_act);
+    unsigned long long li = __ballot(lost_int), la = __ballot(lost_mask || lost_act);
     if (__lane_id() == 0) {
         if (li) atomicAdd(&c.st[RT_ST_SURF + 4 * s + 2], (unsigned)__popcll(li));
         if (la) atomicAdd(&c.st[RT_ST_SURF + 4 * s + 3], (unsigned)__popcll(la));
@@ -1150,17 +1217,22 @@ __device__ __forceinline__ void ltab_body(const LtabArgs& a) {
 // loaded otherwise), `surfaces` applies the scheme, counters are flushed per
 // block.  `surfaces(r, alive, ctx, i)` is the runtime loop over the table or
 // a generated straight-line sequence of StatSurf steps.
-template <int BEAM, class Surfaces>
+template <int BEAM, int MODE = 0, class Surfaces>
 __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces) {
     extern __shared__ double s_pool[];
     __shared__ unsigned s_stats[RT_STATS_LEN];
+    __shared__ unsigned long long s_ext[(MODE & kRecExt) ? RT_MAX_SURF * kExtPerSurf : 1];
     for (int i = threadIdx.x; i < RT_STATS_LEN; i += blockDim.x) s_stats[i] = 0u;
     for (int i = threadIdx.x; i < a.pool_len; i += blockDim.x) s_pool[i] = a.pool[i];
+    if constexpr ((MODE & kRecExt) != 0)
+        for (int i = threadIdx.x; i < RT_MAX_SURF * kExtPerSurf; i += blockDim.x) s_ext[i] = (i & 1) ? 0ull : ~0ull;
     __syncthreads();
 
     TraceCtx c;
     c.pool = s_pool;
     c.st = s_stats;
+    c.ext = s_ext;
+    c.loss = 0;
     c.hist = a.hist;
     c.flux = a.flux;
     c.do_hist = (a.flags & RT_TRACE_HIST) && a.hist;
@@ -1200,7 +1272,9 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
             r.lam = ka->L[i];
             alive = ka->F[i] != 0;
         }
+        if constexpr ((MODE & kRecLoss) != 0) c.loss = alive ? 0 : kLossDead;
         surfaces(r, alive, c, i);
+        if constexpr ((MODE & kRecLoss) != 0) kernel_args()->loss[i] = c.loss;
         if (store || BEAM >= 0) {
             // the ray pointers are read from the argument segment where they are
             // used, not held in SGPRs across the surface chain
@@ -1230,6 +1304,16 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
         unsigned long long* dst = a.stats + (size_t)(blockIdx.x % kStatCopies) * RT_STATS_LEN;
         for (int k = threadIdx.x; k < RT_STATS_LEN; k += blockDim.x)
             if (s_stats[k]) atomicAdd(&dst[k], (unsigned long long)s_stats[k]);
+    }
+    if constexpr ((MODE & kRecExt) != 0) {
+        for (int k = threadIdx.x; k < a.n_surf * kExtPerSurf; k += blockDim.x) {
+            const unsigned long long v = s_ext[k];
+            if (k & 1) {
+                if (v != 0ull) atomicMax(&a.ext[k], v);
+            } else if (v != ~0ull) {
+                atomicMin(&a.ext[k], v);
+            }
+        }
     }
 }
 

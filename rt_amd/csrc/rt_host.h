@@ -26,7 +26,8 @@ int rt_set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2
 // lives.  The calls of one owner share its buffers and therefore run in the
 // order of one stream.
 struct WorkBufs {
-    enum Slot { TAPE, GACC, PART, PARAMS, SPOT_PART, SPOT_CS, DESCEND, N_SLOTS };
+    enum Slot { TAPE, GACC, PART, PARAMS, SPOT_PART, SPOT_CS, DESCEND, EXT_KEYS, SAVE_HIST, SAVE_LPOS, SAVE_TILES,
+                SAVE_GPOS, N_SLOTS };
     void* p[N_SLOTS] = {};
     size_t have[N_SLOTS] = {};
     std::vector<void*> owned;

@@ -148,8 +148,12 @@ extern "C" int rt_hip_set_jit(int mode) {
     return prev;
 }
 
-static std::string generate_source(const std::string& sig, int beam_mode, bool dump, int block, int waves,
+// variant: RT_JIT_VAR_DUMP (per-surface state dump), RT_JIT_VAR_LOSS and
+// RT_JIT_VAR_EXT (the recording modes rt::kRecLoss, rt::kRecExt)
+static std::string generate_source(const std::string& sig, int beam_mode, int variant, int block, int waves,
                                    bool batch = false) {
+    const bool dump = (variant & RT_JIT_VAR_DUMP) != 0;
+    const int rec = ((variant & RT_JIT_VAR_LOSS) ? rt::kRecLoss : 0) | ((variant & RT_JIT_VAR_EXT) ? rt::kRecExt : 0);
     // RT_JIT_LTAB: the spline-valued functions come from the per-wavelength
     // table (TraceArgs::ltab), loaded at the start of each ray
     const bool ltab = beam_mode >= 0 && (beam_mode & RT_JIT_LTAB) != 0;
@@ -162,8 +166,9 @@ static std::string generate_source(const std::string& sig, int beam_mode, bool d
        << "extern \"C\" __global__ __launch_bounds__(" << block;
     if (waves > 0) os << ", " << waves;      // minimum waves per SIMD the register allocation must allow
     os << ") void rt_trace_jit(rt::TraceArgs a) {\n"
-       << "  rt::" << (batch ? "trace_batch_body<" : "trace_body<") << beam_mode
-       << ">(a, [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, uint64_t i) {\n";
+       << "  rt::" << (batch ? "trace_batch_body<" : "trace_body<") << beam_mode;
+    if (rec) os << ", " << rec;
+    os << ">(a, [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, uint64_t i) {\n";
     if (use_tab) os << "    double lv[" << col_surf.size() << "];\n";
     size_t pos = 0;
     int s = 0;
@@ -173,8 +178,9 @@ static std::string generate_source(const std::string& sig, int beam_mode, bool d
         if (use_tab)
             for (size_t q = 0; q < col_surf.size(); ++q)
                 if (col_surf[q] == s) os << "    rt::load_lval(lv, c, " << q << ");\n";
-        os << "    rt::surface_step(" << type << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s
-           << ", r, alive, c";
+        os << "    rt::surface_step";
+        if (rec) os << "<" << type << ", " << rec << ">";
+        os << "(" << type << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s << ", r, alive, c";
         if (use_tab && col_base[(size_t)s] >= 0) os << ", lv, " << col_base[(size_t)s];
         os << ");\n";
         if (dump) os << "    rt::dump_state(a, " << s << ", r, alive, i);\n";
@@ -382,16 +388,16 @@ static bool compile(const std::string& src, Entry& e, const char* name = "rt_tra
     return true;
 }
 
-hipFunction_t rt_jit_kernel(const std::string& sig, int beam_mode, bool dump, int block, int waves, int device,
+hipFunction_t rt_jit_kernel(const std::string& sig, int beam_mode, int variant, int block, int waves, int device,
                             bool batch) {
     std::ostringstream key;
-    key << device << "|" << beam_mode << "|" << block << "|" << waves << "|" << (dump ? "dump|" : "")
+    key << device << "|" << beam_mode << "|" << block << "|" << waves << "|v" << variant << "|"
         << (batch ? "batch|" : "") << sig;
     std::lock_guard<std::mutex> lk(g_mu);
     Entry& e = g_cache[key.str()];
     if (e.fn) return e.fn;
     if (e.failed) return nullptr;
-    if (!compile(generate_source(sig, beam_mode, dump, block, waves, batch), e)) {
+    if (!compile(generate_source(sig, beam_mode, variant, block, waves, batch), e)) {
         e.failed = true;
         fprintf(stderr, "rt_hip: generated kernel failed to build, using the generic kernel:\n%s\n",
                 e.log.c_str());
@@ -429,7 +435,8 @@ extern "C" int rt_jit_check_dtable(const double* dtable, int n_surf, int beam_mo
     // a negative mode (rays given) carries no flags
     const bool batch = beam_mode >= 0 && (beam_mode & RT_JIT_BATCH) != 0;
     const bool adjoint = beam_mode >= 0 && (beam_mode & RT_JIT_ADJOINT) != 0;
-    if (beam_mode >= 0) beam_mode &= ~(RT_JIT_BATCH | RT_JIT_ADJOINT);
+    const bool record = beam_mode >= 0 && (beam_mode & RT_JIT_RECORD) != 0;
+    if (beam_mode >= 0) beam_mode &= ~(RT_JIT_BATCH | RT_JIT_ADJOINT | RT_JIT_RECORD);
     const bool ltab = beam_mode >= 0 && (beam_mode & RT_JIT_LTAB) != 0;
     if (adjoint) beam_mode &= ~RT_JIT_LTAB;
     std::string src;
@@ -446,7 +453,8 @@ extern "C" int rt_jit_check_dtable(const double* dtable, int n_surf, int beam_mo
         src = generate_adjoint_source(rt_jit_signature(dtable, n_surf), beam_mode, rows, n_surf, rt::kBlock, 0, batch,
                                       ltab);
     } else {
-        src = generate_source(rt_jit_signature(dtable, n_surf), beam_mode, false, rt::kBlock, 0, batch);
+        src = generate_source(rt_jit_signature(dtable, n_surf), beam_mode,
+                              record ? (RT_JIT_VAR_LOSS | RT_JIT_VAR_EXT) : 0, rt::kBlock, 0, batch);
     }
     std::vector<char> code;
     std::string log;

@@ -29,7 +29,10 @@ bool rt_jit_ltab_enabled();
 // compiled on first use and cached for the life of the process; nullptr
 // when the runtime compiler is unavailable or fails (the caller then
 // launches the generic kernel, which reads the structure from the records).
-hipFunction_t rt_jit_kernel(const std::string& signature, int beam_mode, bool dump, int block, int waves,
+// kernel variants of rt_jit_kernel (bit mask)
+enum { RT_JIT_VAR_DUMP = 1, RT_JIT_VAR_LOSS = 2, RT_JIT_VAR_EXT = 4 };
+
+hipFunction_t rt_jit_kernel(const std::string& signature, int beam_mode, int variant, int block, int waves,
                             int device, bool batch = false);
 
 // The adjoint kernel generated for (signature, beam type, gradient rows,

@@ -479,3 +479,107 @@ extern "C" int rt_hip_spline_eval(const double* spline, size_t n, const double* 
     if (e != hipSuccess) return rt_set_error(RT_ERR_FAILED, "spline kernel: %s", hipGetErrorString(e));
     return RT_OK;
 }
+
+// ------------------------------------------------------------ ccd_frame flux
+
+struct FluxArgs {
+    uint64_t n;
+    const double *X, *Y, *L;
+    const double* qe;          // device spline record [K, x0, inv, x, y, y2], or null (qe_c)
+    const double* sp;          // the same for the incident spectrum (sp_c)
+    double qe_c, sp_c;
+    double half_x, half_y, xpix, ypix, area;
+    uint32_t xdim, ydim;
+    double* image;
+    unsigned long long* counts;   // [0] rays on the frame, [1] off the frame, [2] lambda outside a table
+};
+
+// ccd_frame's per-file work on the device (ccd/ccd_frame.cpp:190-195): the
+// pixel of every ray (ccd_coordinates_kernel, rt_cuda_engine_ccd.cu:30-45),
+// its photon weight spec(lambda) lambda / hc QE(lambda) pixel area (ccd.cu:
+// 286-287,360) and the atomic add into the flux image (compute_ccd_image,
+// ccd.cu:219-232, with a bounds check, D10)
+__global__ __launch_bounds__(kThreads) void rt_ccd_flux_kernel(FluxArgs a) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned on_n = 0, off_n = 0, oob_n = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+        const uint32_t px = rt::trunc_u32((a.X[i] + a.half_x) / a.xpix);
+        const uint32_t py = rt::trunc_u32((a.Y[i] + a.half_y) / a.ypix);
+        if (!(px < a.xdim && py < a.ydim)) {
+            ++off_n;
+            continue;
+        }
+        const double lam = a.L[i];
+        int eq = 0, es = 0;
+        const double q = a.qe ? rt::spline_eval(a.qe, lam, eq) : a.qe_c;
+        const double s = (a.sp ? rt::spline_eval(a.sp, lam, es) : a.sp_c) * lam;
+        if (eq | es) {
+            ++oob_n;
+            continue;
+        }
+        ++on_n;
+        unsafeAtomicAdd(&a.image[(uint64_t)py * a.xdim + px], s / rt::kHC * q * a.area);
+    }
+    if (a.counts) {
+        if (on_n) atomicAdd(&a.counts[0], (unsigned long long)on_n);
+        if (off_n) atomicAdd(&a.counts[1], (unsigned long long)off_n);
+        if (oob_n) atomicAdd(&a.counts[2], (unsigned long long)oob_n);
+    }
+}
+
+namespace {
+// device record of a host spline [K, x, y, y2] (the layout spline_eval reads)
+std::vector<double> spline_record(const double* spline) {
+    const size_t K = (size_t)spline[0];
+    std::vector<double> rec(3 + 3 * K);
+    rec[0] = (double)K;
+    rec[1] = spline[1];
+    rec[2] = (double)(K - 1) / (spline[K] - spline[1]);
+    memcpy(rec.data() + 3, spline + 1, 3 * K * sizeof(double));
+    return rec;
+}
+}  // namespace
+
+extern "C" int rt_hip_ccd_flux(size_t n, const double* X, const double* Y, const double* lambda,
+                               const double* qe_spline, double qe_const, const double* spec_spline,
+                               double spec_const, double xsize, double ysize, double xpix, double ypix, size_t xdim,
+                               size_t ydim, double* image, unsigned long long* counts, void* stream) {
+    if (!image || (n > 0 && (!X || !Y || !lambda))) return rt_set_error(RT_ERR_BAD_VALUE, "NULL argument");
+    if (xdim == 0 || ydim == 0 || xdim > 0xffffffffu || ydim > 0xffffffffu ||
+        (unsigned long long)xdim * ydim > 4294967296ull)
+        return rt_set_error(RT_ERR_BAD_VALUE, "bad CCD dimensions");
+    for (const double* sp : {qe_spline, spec_spline})
+        if (sp && ((size_t)sp[0] < 2 || !(sp[(size_t)sp[0]] > sp[1])))
+            return rt_set_error(RT_ERR_BAD_VALUE, "bad spline table");
+    if (n == 0) return RT_OK;
+    int rc = rt_ensure_device();
+    if (rc != RT_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<double> rq = qe_spline ? spline_record(qe_spline) : std::vector<double>();
+    std::vector<double> rs = spec_spline ? spline_record(spec_spline) : std::vector<double>();
+    std::vector<double> both(rq);
+    both.insert(both.end(), rs.begin(), rs.end());
+    double* dev = nullptr;
+    if (!both.empty()) {
+        HIP_TRY(hipMallocAsync(&dev, both.size() * sizeof(double), st), RT_ERR_BAD_ALLOC);
+        HIP_TRY(hipMemcpyAsync(dev, both.data(), both.size() * sizeof(double), hipMemcpyHostToDevice, st),
+                RT_ERR_FAILED);
+    }
+    FluxArgs a;
+    memset(&a, 0, sizeof a);
+    a.n = n; a.X = X; a.Y = Y; a.L = lambda;
+    a.qe = qe_spline ? dev : nullptr;
+    a.sp = spec_spline ? dev + rq.size() : nullptr;
+    a.qe_c = qe_const; a.sp_c = spec_const;
+    a.half_x = xsize / 2.0; a.half_y = ysize / 2.0; a.xpix = xpix; a.ypix = ypix;
+    a.area = xpix * ypix * 1.0E-2;
+    a.xdim = (uint32_t)xdim; a.ydim = (uint32_t)ydim;
+    a.image = image; a.counts = counts;
+    hipLaunchKernelGGL(rt_ccd_flux_kernel, dim3(grid_for(n)), dim3(kThreads), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (dev) (void)hipFreeAsync(dev, st);
+    // the host records must outlive the asynchronous copy
+    HIP_TRY(hipStreamSynchronize(st), RT_ERR_FAILED);
+    if (e != hipSuccess) return rt_set_error(RT_ERR_FAILED, "ccd flux kernel: %s", hipGetErrorString(e));
+    return RT_OK;
+}

@@ -95,14 +95,29 @@ using rt::TraceArgs;
 constexpr int kLoad = -1;
 
 // Generic kernel: the structure of every surface is read from its record.
-template <int BEAM>
+// MODE: the recording modes (rt::kRecLoss | rt::kRecExt) or 0.
+template <int BEAM, int MODE = 0>
 __global__ __launch_bounds__(kBlock) void rt_trace_kernel(TraceArgs a) {
-    rt::trace_body<BEAM>(a, [&](Ray& r, bool& alive, rt::TraceCtx& c, uint64_t i) {
+    rt::trace_body<BEAM, MODE>(a, [&](Ray& r, bool& alive, rt::TraceCtx& c, uint64_t i) {
         for (int s = 0; s < a.n_surf; ++s) {
-            rt::surface_step(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, s, r, alive, c);
+            rt::surface_step<rt::DynSurf, MODE>(rt::DynSurf{c.table + (size_t)s * rt::DEV_STRIDE}, s, r, alive, c);
             if (a.dump) rt::dump_state(a, s, r, alive, i);
         }
     });
+}
+
+// extents keys before a recording trace: min keys all ones, max keys zero
+__global__ void rt_ext_init_kernel(unsigned long long* keys, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) keys[i] = (i & 1) ? 0ull : ~0ull;
+}
+
+// extents as doubles (NaN where no ray was alive)
+__global__ void rt_ext_decode_kernel(const unsigned long long* keys, int n, double* out) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const unsigned long long k = keys[i];
+        const bool empty = (i & 1) ? k == 0ull : k == ~0ull;
+        out[i] = empty ? __builtin_nan("") : rt::key_value(k);
+    }
 }
 
 // Generic batched kernel (structure read from the records of each instance)
@@ -764,12 +779,14 @@ static void beam_uniforms(TraceArgs& a) {
     a.beam.u.gauss_tg = tan(stddev);
 }
 
-extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, uint64_t seed,
-                            size_t ray_begin, size_t n_rays, double* X, double* Y, double* Z,
-                            double* cX, double* cY, double* cZ, double* lambda, signed char* flag,
-                            int flags, unsigned int* ccd_hist, double* ccd_flux,
-                            unsigned long long* stats, double* dump, signed char* dump_flag,
-                            void* stream) {
+namespace {
+// rt_hip_trace and rt_hip_trace_record: rec is the recording mode (rt::kRecLoss
+// | rt::kRecExt or 0), loss / ext_keys its outputs
+int trace_launch(const rt_hip_scheme* sch, const double* beam_desc, uint64_t seed, size_t ray_begin, size_t n_rays,
+                 double* X, double* Y, double* Z, double* cX, double* cY, double* cZ, double* lambda,
+                 signed char* flag, int flags, unsigned int* ccd_hist, double* ccd_flux, unsigned long long* stats,
+                 double* dump, signed char* dump_flag, int rec, unsigned char* loss, unsigned long long* ext_keys,
+                 void* stream) {
     if (!sch) return set_error(RT_ERR_BAD_VALUE, "scheme is NULL");
     const bool gen = (flags & RT_TRACE_GENERATE) != 0;
     if (gen) {
@@ -803,6 +820,8 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
     a.X = X; a.Y = Y; a.Z = Z; a.cX = cX; a.cY = cY; a.cZ = cZ; a.L = lambda; a.F = flag;
     a.hist = ccd_hist; a.flux = ccd_flux; a.dump = dump; a.dump_flag = dump_flag;
     a.stats = stats ? sch->stat_copies : nullptr;
+    a.loss = loss;
+    a.ext = ext_keys;
 
     const int cus = rt_device_cus();
     size_t lds = sch->pool_len * sizeof(double);
@@ -812,8 +831,10 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
     const uint64_t nk = gen && sch->unroll ? ltab_nk(sch->lt, beam_desc) : 0;
     hipFunction_t fn = nullptr;
     if (rt_jit_enabled() && sch->unroll)
-        fn = rt_jit_kernel(sch->signature, beam_mode | (nk ? RT_JIT_LTAB : 0), dump != nullptr, sch->block,
-                           sch->waves, sch->device);
+        fn = rt_jit_kernel(sch->signature, beam_mode | (nk ? RT_JIT_LTAB : 0),
+                           (dump ? RT_JIT_VAR_DUMP : 0) | ((rec & rt::kRecLoss) ? RT_JIT_VAR_LOSS : 0) |
+                               ((rec & rt::kRecExt) ? RT_JIT_VAR_EXT : 0),
+                           sch->block, sch->waves, sch->device);
     if (fn) {
         if (nk) {
             rc = ltab_fill(sch->lt, a, sch->pool_dev, nk, st);
@@ -838,15 +859,64 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
         uint64_t want = (n_rays + kBlock - 1) / kBlock;
         uint64_t cap = (uint64_t)cus * 8;      // persistent grid: 8 blocks (32 waves) per CU
         unsigned grid = (unsigned)(want < cap ? want : cap);
-        auto kern = beam_mode == RT_BEAM_PARALLEL ? rt_trace_kernel<RT_BEAM_PARALLEL>
-                  : beam_mode == RT_BEAM_CONIC    ? rt_trace_kernel<RT_BEAM_CONIC>
-                  : beam_mode == RT_BEAM_GAUSS    ? rt_trace_kernel<RT_BEAM_GAUSS>
-                                                  : rt_trace_kernel<kLoad>;
+        constexpr int R = rt::kRecLoss | rt::kRecExt;
+        auto kern = rec == 0 ? (beam_mode == RT_BEAM_PARALLEL ? rt_trace_kernel<RT_BEAM_PARALLEL>
+                                : beam_mode == RT_BEAM_CONIC  ? rt_trace_kernel<RT_BEAM_CONIC>
+                                : beam_mode == RT_BEAM_GAUSS  ? rt_trace_kernel<RT_BEAM_GAUSS>
+                                                              : rt_trace_kernel<kLoad>)
+                             : (beam_mode == RT_BEAM_PARALLEL ? rt_trace_kernel<RT_BEAM_PARALLEL, R>
+                                : beam_mode == RT_BEAM_CONIC  ? rt_trace_kernel<RT_BEAM_CONIC, R>
+                                : beam_mode == RT_BEAM_GAUSS  ? rt_trace_kernel<RT_BEAM_GAUSS, R>
+                                                              : rt_trace_kernel<kLoad, R>);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, a);
         g_last_kernel = "generic";
     }
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     if (stats) return stats_fold(sch->stat_copies, stats, st);
+    return RT_OK;
+}
+}  // namespace
+
+extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, uint64_t seed,
+                            size_t ray_begin, size_t n_rays, double* X, double* Y, double* Z,
+                            double* cX, double* cY, double* cZ, double* lambda, signed char* flag,
+                            int flags, unsigned int* ccd_hist, double* ccd_flux,
+                            unsigned long long* stats, double* dump, signed char* dump_flag,
+                            void* stream) {
+    return trace_launch(sch, beam_desc, seed, ray_begin, n_rays, X, Y, Z, cX, cY, cZ, lambda, flag, flags, ccd_hist,
+                        ccd_flux, stats, dump, dump_flag, 0, nullptr, nullptr, stream);
+}
+
+extern "C" int rt_hip_trace_record(const rt_hip_scheme* sch, const double* beam_desc, uint64_t seed, size_t n_rays,
+                                   double* X, double* Y, double* Z, double* cX, double* cY, double* cZ,
+                                   double* lambda, signed char* flag, int flags, unsigned long long* stats,
+                                   unsigned char* loss, double* extents, void* stream) {
+    if (!sch) return set_error(RT_ERR_BAD_VALUE, "scheme is NULL");
+    if (n_rays > 0 && !loss) return set_error(RT_ERR_BAD_VALUE, "loss is NULL");
+    if (flags & (RT_TRACE_HIST | RT_TRACE_FLUX))
+        return set_error(RT_ERR_BAD_VALUE, "a recording trace accumulates no CCD frame");
+    int rc = ensure_device();
+    if (rc != RT_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int n_ext = sch->n_surf * rt::kExtPerSurf;
+    unsigned long long* keys = nullptr;
+    if (extents) {
+        rc = sch->work.ensure(WorkBufs::EXT_KEYS, (size_t)rt::kExtPerSurf * RT_MAX_SURF * sizeof(unsigned long long));
+        if (rc != RT_OK) return rc;
+        keys = sch->work.get<unsigned long long>(WorkBufs::EXT_KEYS);
+        hipLaunchKernelGGL(rt_ext_init_kernel, dim3(1), dim3(256), 0, st, keys, n_ext);
+        HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    }
+    rc = trace_launch(sch, beam_desc, seed, 0, n_rays, X, Y, Z, cX, cY, cZ, lambda, flag, flags, nullptr, nullptr,
+                      stats, nullptr, nullptr, rt::kRecLoss | (extents ? rt::kRecExt : 0), loss, keys, stream);
+    if (rc != RT_OK || !extents) return rc;
+    if (n_rays == 0) {
+        hipLaunchKernelGGL(rt_ext_init_kernel, dim3(1), dim3(256), 0, st, keys, n_ext);
+        HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    }
+    hipLaunchKernelGGL(rt_ext_decode_kernel, dim3(1), dim3(256), 0, st, (const unsigned long long*)keys, n_ext,
+                       extents);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     return RT_OK;
 }
 

@@ -58,19 +58,22 @@ def derivs_cost(rec) -> int:
     return (9 + 2 * n) if int(rec[S.S_CLASS]) == S.RT_CONIC else (20 + 4 * n)
 
 
-def fn_cost(kind) -> int:
-    return SPLINE if kind == S.RT_FN_SPLINE else 0
+def fn_cost(kind, table: bool = False) -> int:
+    """a spline-valued function: SPLINE per evaluation, 0 when the kernel
+    loads it from the per-wavelength table"""
+    return SPLINE if kind == S.RT_FN_SPLINE and not table else 0
 
 
-def action_cost(rec, n1_reused: bool = False) -> int:
+def action_cost(rec, n1_reused: bool = False, table: bool = False) -> int:
     """reflection / refraction / diffraction with the surface normal; n1(lambda)
-    is not counted when the previous action already evaluated the same table"""
+    is not counted when the previous action already evaluated the same table,
+    neither index when the per-wavelength table supplies it"""
     t, cls, n = int(rec[S.S_TYPE]), int(rec[S.S_CLASS]), int(rec[S.S_NPOLY])
     if t == S.RT_MIRROR:
         if cls == S.RT_PLANE:
             return 0
         return (26 + 2 * n) if cls == S.RT_CONIC else (37 + 4 * n)
-    idx = (0 if n1_reused else fn_cost(rec[S.S_N1KIND])) + fn_cost(rec[S.S_N2KIND]) + 1
+    idx = (0 if n1_reused else fn_cost(rec[S.S_N1KIND], table)) + fn_cost(rec[S.S_N2KIND], table) + 1
     if t == S.RT_LENS:
         if cls == S.RT_PLANE:
             return idx + 11
@@ -106,10 +109,12 @@ def mask_cost(rec) -> int:
 
 
 def trace_flops(scheme: "S.Scheme", stats: np.ndarray, generated: bool = True,
-                generated_kernel: bool = True) -> float:
+                generated_kernel: bool = True, table: bool = False) -> float:
     """Algorithmic FP64 operations of the trace(s) that produced `stats`
     (`generated`: the beam was generated in the kernel; `generated_kernel`:
-    the per-scheme kernel ran, which reuses repeated refractive indices)."""
+    the per-scheme kernel ran, which reuses repeated refractive indices;
+    `table`: that kernel read the spline-valued functions from the
+    per-wavelength table, so their evaluations count 0 per ray)."""
     t = np.asarray(scheme.table).reshape(scheme.n_surf, S.SURF_STRIDE)
     st = np.asarray(stats).astype(np.float64)
     total = 0.0
@@ -129,9 +134,10 @@ def trace_flops(scheme: "S.Scheme", stats: np.ndarray, generated: bool = True,
         if is_newton(rec):
             total += iters * newton_iter_cost(rec)
         total += after_int * mask_cost(rec)
-        total += after_act * (action_cost(rec, reuse[s]) + NORMALIZE)
+        total += after_act * (action_cost(rec, reuse[s], table) + NORMALIZE)
         if int(rec[S.S_TYPE]) == S.RT_DETECTOR and rec[S.S_CCD_XDIM] > 0:
-            total += after_act * CCD_MAP + st[S.ST_CCD_HITS] * FLUX
+            flux = FLUX - (fn_cost(rec[S.S_QEKIND], False) if table else 0)
+            total += after_act * CCD_MAP + st[S.ST_CCD_HITS] * flux
     return total
 
 

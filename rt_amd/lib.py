@@ -62,6 +62,10 @@ def lib():
         L.rt_hip_adjoint.argtypes = [_vp, _dp, C.c_uint64, C.c_size_t, _vp, C.c_size_t,
                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, _vp, _vp, _vp]
         L.rt_hip_group_spots.argtypes = [_vp, _vp, _vp, C.c_size_t, C.c_size_t, _vp, _vp]
+        L.rt_hip_trace_record.argtypes = [_vp, _dp, C.c_uint64, C.c_size_t] + [_vp] * 8 + [C.c_int, _vp, _vp,
+                                                                                            _vp, _vp]
+        L.rt_hip_rearrange.argtypes = [_vp, C.c_size_t, _vp, C.POINTER(C.c_uint64), _vp]
+        L.rt_hip_beam_order.argtypes = [C.c_size_t, _vp, _vp, C.POINTER(C.c_uint8)] + [_vp] * 8 + [_vp]
         L.rt_hip_spot_loss.argtypes = [_vp, C.c_size_t, _vp, C.c_double, _vp, _vp, _vp]
         L.rt_hip_batch_group_spots.argtypes = [_vp, _vp, _vp, _vp, C.c_size_t, _vp, _vp]
         L.rt_hip_batch_descend.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, _vp, _vp, _vp]
@@ -170,6 +174,27 @@ class DeviceScheme:
                                 _tp(hist), _tp(flux), _tp(stats), _tp(dump), _tp(dump_flag), stream)
         check(rc, "rt_hip_trace")
 
+    def trace_record(self, rays, flag, loss, *, seed=0, n=None, extents=None, stats=None, beam=None,
+                     stream=None):
+        """rt_hip_trace_record: the generated beam of `seed` traced into rays
+        (7 x n), flag, loss (n uint8 CUDA tensor) and, when given, extents
+        (n_surf x RT_EXT_PER_SURF float64 CUDA tensor)."""
+        import torch
+        b = np.ascontiguousarray((beam if beam is not None else self.scheme.beam).pack())
+        if n is None:
+            n = rays.shape[1]
+        assert rays.dtype == torch.float64 and rays.is_cuda and rays.is_contiguous() and rays.shape[1] >= n
+        assert flag.dtype == torch.int8 and loss.dtype == torch.uint8 and loss.numel() >= n
+        if extents is not None:
+            assert extents.dtype == torch.float64 and extents.is_contiguous()
+            assert extents.numel() >= self.scheme.n_surf * EXT_PER_SURF
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        rc = lib().rt_hip_trace_record(self.handle, _dptr(b), seed, n, *[rays[k].data_ptr() for k in range(7)],
+                                       _tp(flag), S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, _tp(stats), _tp(loss),
+                                       _tp(extents), stream)
+        check(rc, "rt_hip_trace_record")
+
     def adjoint(self, group_seeds, param_surface, param_field, *, seed=0, n=None, grad=None, stats=None,
                 beam=None, stream=None):
         """rt_hip_adjoint: dLoss/dparameters for a loss seeded per ray group by
@@ -260,6 +285,35 @@ def init(device: int = 0):
 
 
 SPOT_STRIDE = 6     # RT_SPOT_STRIDE: n, centroid x, y, variance x, y, RMS radius
+EXT_PER_SURF = 12   # RT_EXT_PER_SURF
+
+
+def rearrange(loss, stream=None):
+    """rt_hip_rearrange: (perm, counts): the ray at each position of the
+    saved beam (uint64 CUDA tensor, int64 dtype) and the rays of every loss
+    code (numpy uint64[256])."""
+    import torch
+    n = loss.numel()
+    perm = torch.empty(n, dtype=torch.int64, device=loss.device)
+    counts = np.zeros(256, dtype=np.uint64)
+    st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    check(lib().rt_hip_rearrange(_tp(loss), n, _tp(perm), counts.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                 C.c_void_p(st)), "rt_hip_rearrange")
+    return perm, counts
+
+
+def beam_order(perm, loss, renorm, rays, stream=None):
+    """rt_hip_beam_order: the 7 x n arrays of Beam::Save in file order (CUDA
+    float64 tensor); renorm: numpy uint8[256]."""
+    import torch
+    n = perm.numel()
+    out = torch.empty((7, n), dtype=torch.float64, device=rays.device)
+    ren = np.ascontiguousarray(renorm, dtype=np.uint8)
+    st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    check(lib().rt_hip_beam_order(n, _tp(perm), _tp(loss), ren.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                  *[rays[k].data_ptr() for k in range(7)], _tp(out), C.c_void_p(st)),
+          "rt_hip_beam_order")
+    return out
 
 
 def group_spots(rays, flag, n_groups: int, stream=None, out=None):
@@ -450,3 +504,71 @@ class Graph:
             self.close()
         except Exception:
             pass
+
+
+# ---- librt_scheme_hip.so: the Scheme path (XML loader, Scheme::run,
+# ComputeQE, ccd_frame)
+
+SCHEME_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librt_scheme_hip.so")
+_SCHEME_LIB = None
+
+
+def scheme_lib():
+    """Load librt_scheme_hip.so; raise if it is not built."""
+    global _SCHEME_LIB
+    if _SCHEME_LIB is None:
+        if not os.path.exists(SCHEME_LIB_PATH):
+            raise EngineError(f"{SCHEME_LIB_PATH} is not built; run __graft_entry__.build()")
+        lib()                                     # librt_hip.so first (the scheme library links it)
+        L = C.CDLL(SCHEME_LIB_PATH)
+        cp = C.c_char_p
+        L.rt_scheme_run_file.argtypes = [cp, cp, cp]
+        L.rt_scheme_compute_qe.argtypes = [cp, C.c_size_t, cp]
+        L.rt_scheme_pack_file.argtypes = [cp, _dp, C.c_int, C.POINTER(C.c_int), _dp, C.c_size_t,
+                                          C.POINTER(C.c_size_t), _dp, C.POINTER(C.c_uint64)]
+        L.rt_ccd_frame.argtypes = [cp, cp, cp, cp, cp]
+        L.rt_write_fits_image.argtypes = [cp, C.c_size_t, C.c_size_t, _dp, C.c_int, C.POINTER(cp),
+                                          C.POINTER(cp)]
+        _SCHEME_LIB = L
+    return _SCHEME_LIB
+
+
+def _b(s):
+    return None if s is None else os.fsencode(s)
+
+
+def scheme_pack_file(path: str, max_surf: int = 32, max_pool: int = 1 << 20):
+    """rt_scheme_pack_file: (status, table, pool, beam_desc, n_rays) of an XML
+    scheme as Scheme::run would hand it to the engine (no device used)."""
+    table = np.zeros(max_surf * S.SURF_STRIDE)
+    pool = np.zeros(max_pool)
+    beam = np.zeros(S.BEAM_STRIDE)
+    ns, pl, nr = C.c_int(0), C.c_size_t(0), C.c_uint64(0)
+    rc = scheme_lib().rt_scheme_pack_file(_b(path), _dptr(table), max_surf, C.byref(ns), _dptr(pool), max_pool,
+                                          C.byref(pl), _dptr(beam), C.byref(nr))
+    if rc:
+        return rc, None, None, None, 0
+    return 0, table[:ns.value * S.SURF_STRIDE].copy(), pool[:pl.value].copy(), beam, nr.value
+
+
+def scheme_run_file(path: str, list_file: str, log_file: str = None) -> int:
+    """rt_scheme_run_file: ray-tracing/main.cpp's flow on the engine."""
+    return scheme_lib().rt_scheme_run_file(_b(path), _b(list_file), _b(log_file))
+
+
+def scheme_compute_qe(path: str, nlambda: int, qe_file: str) -> int:
+    """rt_scheme_compute_qe: the qe_comp flow (Scheme::ComputeQE)."""
+    return scheme_lib().rt_scheme_compute_qe(_b(path), nlambda, _b(qe_file))
+
+
+def ccd_frame(rt_list: str, qe_file: str, spec_file: str, ccd_spec: str, fits_file: str) -> int:
+    """rt_ccd_frame: ccd_frame's program on the engine (exit code)."""
+    return scheme_lib().rt_ccd_frame(_b(rt_list), _b(qe_file), _b(spec_file), _b(ccd_spec), _b(fits_file))
+
+
+def write_fits_image(path: str, image: np.ndarray, keys=()):
+    """rt_write_fits_image of a (ydim, xdim) float64 image"""
+    img = np.ascontiguousarray(image, dtype=np.float64)
+    ks = (C.c_char_p * max(1, len(keys)))(*[k.encode() for k, _ in keys])
+    vs = (C.c_char_p * max(1, len(keys)))(*[v.encode() for _, v in keys])
+    return scheme_lib().rt_write_fits_image(_b(path), img.shape[1], img.shape[0], _dptr(img), len(keys), ks, vs)

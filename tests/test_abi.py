@@ -240,3 +240,44 @@ def test_batched_kernel_compiles_and_batches_are_validated():
                                np.ascontiguousarray(a.pool).ctypes.data_as(C.POINTER(C.c_double)), len(a.pool),
                                counts.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(h))
     assert rc == 4 and "structure" in lib.last_error()
+
+
+SCHEME_LIB = os.path.join(ROOT, "rt_amd", "librt_scheme_hip.so")
+
+# The public interface of common-lib/scheme.h:21-66 and the reader of
+# beam.h:18-19, as callers such as ray-tracing/main.cpp, ccd/ccd_frame.cpp and
+# efficiency/losses_comp.cpp link them (C++ linkage).
+SCHEME_SIGNATURES = [
+    "Scheme::Scheme()",
+    "Scheme::Scheme(std::__cxx11::basic_string<char, std::char_traits<char>, std::allocator<char> > const&)",
+    "Scheme::~Scheme()",
+    "Scheme::load_from_file(std::__cxx11::basic_string<char, std::char_traits<char>, std::allocator<char> > "
+    "const&, bool)",
+    "Scheme::destroy()",
+    "Scheme::run(std::__cxx11::list<std::__cxx11::basic_string<char, std::char_traits<char>, std::allocator<char> >, "
+    "std::allocator<std::__cxx11::basic_string<char, std::char_traits<char>, std::allocator<char> > > >&, "
+    "std::ostream&, unsigned long)",
+    "Scheme::GetNumberOfSurfaces() const",
+    "Scheme::ComputeQE(unsigned long, char*)",
+    "Scheme::bad_scheme::bad_scheme(Scheme::SchemeError)",
+    "Scheme::bad_scheme::scheme_error() const",
+    "Beam::bad_beam::bad_beam(Beam::BeamError)",
+    "Surface::bad_surface::bad_surface(Surface::SurfaceError)",
+    "load_beam_data(char const*, unsigned long*, unsigned long*, unsigned long*, double**, double**, double**, "
+    "double**, double**, double**, double**, bool)",
+]
+
+
+def test_scheme_library_exports_the_scheme_api():
+    exported = _demangled_exports(SCHEME_LIB)
+    missing = [s for s in SCHEME_SIGNATURES if s not in exported]
+    assert not missing, f"not exported with the reference signature: {missing}"
+    for header in ("scheme.h", "ccd_frame.h"):
+        text = open(os.path.join(ROOT, "include", "rt", header)).read()
+        names = re.findall(r"^int (rt_\w+)\(", text, re.M)
+        assert names
+        for n in names:
+            assert n in exported, n
+    # the reference's enumerator values (scheme.h:26-28, beam.h:35, surface.h:29-32)
+    text = open(os.path.join(ROOT, "include", "rt", "scheme.h")).read()
+    assert "FileOpenFailure, ParseFailure, NoClassAttr, BadAttrValue, NanAttrValue, MemAllocFailure" in text
