@@ -1242,7 +1242,13 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
     const bool store = (a.flags & RT_TRACE_STORE) != 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+    // Recording kernels walk the rays a wave at a time: every lane stays in
+    // the loop while the wave's first lane has a ray, a lane past the end
+    // carrying a dead ray, so the extents' wave reductions read live lanes
+    // only.  Other kernels let each lane leave after its own last ray.
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+         MODE == 0 ? i < a.n : i - __lane_id() < a.n; i += stride) {
+        const bool valid = MODE == 0 || i < a.n;
         // Surface parameters stay per-iteration scalar loads: the table
         // pointer is passed through an empty asm so the loads cannot be
         // hoisted out of this loop (hoisted, they outnumber the SGPRs and
@@ -1263,19 +1269,21 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
             uint64_t j;
             gen_ray<BEAM>(bm, *sd, a.ray_begin + i, r, j);
             c.lk = (uint32_t)j;
-            alive = true;
+            alive = valid;
         } else {
             c.lk = 0;
             const cargs ka = kernel_args();
-            r.X = ka->X[i]; r.Y = ka->Y[i]; r.Z = ka->Z[i];
-            r.cX = ka->cX[i]; r.cY = ka->cY[i]; r.cZ = ka->cZ[i];
-            r.lam = ka->L[i];
-            alive = ka->F[i] != 0;
+            const uint64_t k = valid ? i : 0;
+            r.X = ka->X[k]; r.Y = ka->Y[k]; r.Z = ka->Z[k];
+            r.cX = ka->cX[k]; r.cY = ka->cY[k]; r.cZ = ka->cZ[k];
+            r.lam = ka->L[k];
+            alive = valid && ka->F[k] != 0;
         }
         if constexpr ((MODE & kRecLoss) != 0) c.loss = alive ? 0 : kLossDead;
         surfaces(r, alive, c, i);
-        if constexpr ((MODE & kRecLoss) != 0) kernel_args()->loss[i] = c.loss;
-        if (store || BEAM >= 0) {
+        if constexpr ((MODE & kRecLoss) != 0)
+            if (valid) kernel_args()->loss[i] = c.loss;
+        if ((store || BEAM >= 0) && valid) {
             // the ray pointers are read from the argument segment where they are
             // used, not held in SGPRs across the surface chain
             const cargs ka = kernel_args();
@@ -1286,7 +1294,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
             }
             if (ka->F) ka->F[i] = (signed char)alive;
         }
-        unsigned long long good = __ballot(alive), all = __ballot(true);
+        unsigned long long good = __ballot(alive), all = __ballot(valid);
         if (__lane_id() == 0) {
             atomicAdd(&s_stats[RT_ST_RAYS], (unsigned)__popcll(all));
             if (good) atomicAdd(&s_stats[RT_ST_GOOD], (unsigned)__popcll(good));

@@ -855,6 +855,10 @@ int trace_launch(const rt_hip_scheme* sch, const double* beam_desc, uint64_t see
         HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, sch->block, 1, 1, (unsigned)lds, st, params, nullptr),
                 RT_ERR_FAILED);
         g_last_kernel = "generated";
+        if (getenv("RT_HIP_VERBOSE"))
+            fprintf(stderr, "rt_hip: trace generated kernel %p, %zu rays, block %d, waves %d, grid %u, table %llu, "
+                            "record %d\n", (void*)fn, n_rays, sch->block, sch->waves, grid, (unsigned long long)nk,
+                    rec);
     } else {
         uint64_t want = (n_rays + kBlock - 1) / kBlock;
         uint64_t cap = (uint64_t)cus * 8;      // persistent grid: 8 blocks (32 waves) per CU
@@ -870,6 +874,8 @@ int trace_launch(const rt_hip_scheme* sch, const double* beam_desc, uint64_t see
                                                               : rt_trace_kernel<kLoad, R>);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, a);
         g_last_kernel = "generic";
+        if (getenv("RT_HIP_VERBOSE"))
+            fprintf(stderr, "rt_hip: trace generic kernel, %zu rays, grid %u, record %d\n", n_rays, grid, rec);
     }
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     if (stats) return stats_fold(sch->stat_copies, stats, st);

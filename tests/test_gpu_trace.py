@@ -52,7 +52,7 @@ def gpu_trace(hip, sch, seed=0, dump=True, flux=False, rays=None, flag=None, exp
     n = sch.beam.n_rays if rays is None else rays.shape[1]
     ds = hip.DeviceScheme(sch)
     det = sch.detector()
-    npix = det.ccd["ydim"] * det.ccd["xdim"]
+    npix = det.ccd["ydim"] * det.ccd["xdim"] if det is not None else 1      # no CCD frame: nothing is added
     hist = torch.zeros(npix, dtype=torch.int32, device="cuda")
     fl = torch.zeros(npix, dtype=torch.float64, device="cuda") if flux else None
     stats = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
