@@ -462,6 +462,19 @@ int rt_hip_adjoint(const rt_hip_scheme *scheme, const double *beam_desc, uint64_
 int rt_hip_group_spots(const double *X, const double *Y, const signed char *flag, size_t n, size_t n_groups,
                        double *spots, void *stream);
 
+/* rt_hip_group_spots without any allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  The partial moments go to a static 8 MiB device array per GPU
+   (2^20 doubles, 5 per (chunk of 8 rays of a group, group)): inputs needing
+   more, roughly n > 1.6e6 rays, are rejected with RT_ERR_BAD_VALUE.  Same
+   kernels and summation order as rt_hip_group_spots, so the spots are equal
+   bit for bit.  Calls on one GPU share the array: calls that overlap on one
+   GPU (different streams) race.  rt_hip_init looks the array up for its
+   device; otherwise the first call on a device does. */
+int rt_hip_group_spots_noalloc(const double *X, const double *Y, const signed char *flag, size_t n,
+                               size_t n_groups, double *spots, void *stream);
+
 /* The spot loss over groups with live rays, sum of rms + weight x squared
    centroid offset from target (device, n_groups x 2, or NULL for 0), into
    *loss (device), and the adjoint seeds of rt_hip_adjoint (device, n_groups

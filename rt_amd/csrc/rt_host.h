@@ -48,6 +48,9 @@ struct WorkBufs {
 int rt_ensure_device();
 // Compute units of the current device (after rt_ensure_device).
 int rt_device_cus();
+// Looks up, once per device, the device address of the scratch array of
+// rt_hip_group_spots_noalloc (dev must be the current device).
+int rt_scratch_setup(int dev);
 
 // Device record (rt::DEV_STRIDE doubles) of one public packed record: the
 // public fields plus the host-derived constants; spline references are

@@ -31,6 +31,7 @@
 
 #include <string>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "rt_device.h"
@@ -475,7 +476,9 @@ extern "C" int rt_hip_init(int device) {
         return set_error(RT_ERR_INIT_FAILED, "device %d out of range (%d devices)", device, count);
     HIP_TRY(hipSetDevice(device), RT_ERR_INIT_FAILED);
     g_dev.valid = false;
-    return ensure_device();
+    int rc = ensure_device();
+    // the device address of the no-allocation entry point's scratch array
+    return rc == RT_OK ? rt_scratch_setup(device) : rc;
 }
 
 extern "C" int rt_hip_info(char* buf, size_t len) {
@@ -1668,6 +1671,69 @@ extern "C" int rt_hip_group_spots(const double* X, const double* Y, const signed
     if (rc != RT_OK) return rc;
     return group_spots_launch(X, Y, flag, n, G, n_chunks, w->get<double>(WorkBufs::SPOT_PART), spots,
                               (hipStream_t)stream);
+}
+
+// ------------------------------------ group spots without allocation
+//
+// Partial moments of rt_hip_group_spots_noalloc live in this file-scope
+// array, one copy per GPU, held until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in):
+// its device address is looked up once per device with that device current
+// and kept in g_scratch_ptr.  Every call on one GPU shares the array; the
+// moments kernel writes every partial it reads, so nothing has to start at
+// zero, and calls that overlap on one GPU (two streams) race.
+constexpr size_t kScratchDoubles = size_t(1) << 20;        // 8 MiB per GPU
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+// the device address of `scratch` on the current device (dev)
+int scratch_ptr(int dev, double** out) {
+    if (dev < 0 || dev >= kScratchDevices) return set_error(RT_ERR_FAILED, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return set_error(RT_ERR_INIT_FAILED, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                             hipGetErrorString(e));
+        }
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return RT_OK;
+}
+}  // namespace
+
+int rt_scratch_setup(int dev) {
+    double* p = nullptr;
+    return scratch_ptr(dev, &p);
+}
+
+extern "C" int rt_hip_group_spots_noalloc(const double* X, const double* Y, const signed char* flag, size_t n,
+                                          size_t n_groups, double* spots, void* stream) {
+    if (!spots || n_groups == 0) return set_error(RT_ERR_BAD_VALUE, "spots NULL or no groups");
+    if (n > 0 && (!X || !Y || !flag)) return set_error(RT_ERR_BAD_VALUE, "NULL ray array");
+    const uint64_t G = n_groups;
+    const uint64_t cap = kScratchDoubles / 5;                 // partial moments that fit
+    const uint64_t n_chunks = G <= cap ? spot_chunks(n, G) : 0;
+    if (G > cap || n_chunks > cap / G)
+        return set_error(RT_ERR_BAD_VALUE,
+                         "%zu rays in %zu groups need %llu partial moments; the no-allocation entry point "
+                         "holds %llu (use rt_hip_group_spots)",
+                         n, n_groups, (unsigned long long)(G > cap ? G : n_chunks * G), (unsigned long long)cap);
+    int rc = ensure_device();
+    if (rc != RT_OK) return rc;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), RT_ERR_FAILED);
+    double* part = nullptr;
+    rc = scratch_ptr(dev, &part);
+    if (rc != RT_OK) return rc;
+    return group_spots_launch(X, Y, flag, n, G, n_chunks, part, spots, (hipStream_t)stream);
 }
 
 extern "C" int rt_hip_spot_loss(const double* spots, size_t n_groups, const double* target, double weight,

@@ -62,6 +62,7 @@ def lib():
         L.rt_hip_adjoint.argtypes = [_vp, _dp, C.c_uint64, C.c_size_t, _vp, C.c_size_t,
                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, _vp, _vp, _vp]
         L.rt_hip_group_spots.argtypes = [_vp, _vp, _vp, C.c_size_t, C.c_size_t, _vp, _vp]
+        L.rt_hip_group_spots_noalloc.argtypes = [_vp, _vp, _vp, C.c_size_t, C.c_size_t, _vp, _vp]
         L.rt_hip_trace_record.argtypes = [_vp, _dp, C.c_uint64, C.c_size_t] + [_vp] * 8 + [C.c_int, _vp, _vp,
                                                                                             _vp, _vp]
         L.rt_hip_rearrange.argtypes = [_vp, C.c_size_t, _vp, C.POINTER(C.c_uint64), _vp]
@@ -316,10 +317,11 @@ def beam_order(perm, loss, renorm, rays, stream=None):
     return out
 
 
-def group_spots(rays, flag, n_groups: int, stream=None, out=None):
+def group_spots(rays, flag, n_groups: int, stream=None, out=None, noalloc=False):
     """rt_hip_group_spots: (n_groups, SPOT_STRIDE) float64 CUDA tensor of the
     spots of the rays with flag 1, ray r in group r mod n_groups (rays: the
-    7 x n state on the device, flag: n int8)."""
+    7 x n state on the device, flag: n int8).  noalloc: through
+    rt_hip_group_spots_noalloc (static scratch, bounded size)."""
     import torch
     n = rays.shape[1]
     if rays.dtype != torch.float64 or not rays.is_contiguous() or flag.dtype != torch.int8:
@@ -327,9 +329,10 @@ def group_spots(rays, flag, n_groups: int, stream=None, out=None):
     if out is None:
         out = torch.empty((n_groups, SPOT_STRIDE), dtype=torch.float64, device=rays.device)
     st = torch.cuda.current_stream().cuda_stream if stream is None else stream
-    check(lib().rt_hip_group_spots(C.c_void_p(rays[0].data_ptr()), C.c_void_p(rays[1].data_ptr()),
-                                   C.c_void_p(flag.data_ptr()), n, n_groups, C.c_void_p(out.data_ptr()),
-                                   C.c_void_p(st)), "rt_hip_group_spots")
+    name = "rt_hip_group_spots_noalloc" if noalloc else "rt_hip_group_spots"
+    check(getattr(lib(), name)(C.c_void_p(rays[0].data_ptr()), C.c_void_p(rays[1].data_ptr()),
+                               C.c_void_p(flag.data_ptr()), n, n_groups, C.c_void_p(out.data_ptr()),
+                               C.c_void_p(st)), name)
     return out
 
 
