@@ -44,6 +44,8 @@ enum DevField {
     D_HALF_X = RT_SURF_STRIDE + 2,     // CCD xsize/2
     D_HALF_Y = RT_SURF_STRIDE + 3,     // CCD ysize/2
     D_K1C2 = RT_SURF_STRIDE + 4,       // (k+1) c c, the product the reference forms per ray
+    D_INV_XPIX = RT_SURF_STRIDE + 5,   // 1/CCD x pixel size
+    D_INV_YPIX = RT_SURF_STRIDE + 6,   // 1/CCD y pixel size
     D_IA = RT_SURF_STRIDE + 8,         // i a_i, i = 1..RT_MAX_POLY (aspheric derivative terms)
     DEV_STRIDE = RT_SURF_STRIDE + 24
 };
@@ -112,28 +114,36 @@ struct StatSurf {
 // The hardware reciprocal / reciprocal square root estimates refined by two
 // Newton-Raphson steps each (quadratic convergence: any estimate better than
 // 2^-14 reaches the FP64 rounding floor), about 1 ulp from the IEEE result
-// at a third of the instructions of the IEEE division sequence.  Zero,
-// infinity and NaN inputs return the IEEE special values; denormal
-// arguments are outside the domain of every caller.
+// at a third of the instructions of the IEEE division sequence.
+// Reciprocals and quotients take their special values from v_div_fixup_f64,
+// the instruction that ends the IEEE division sequence: for a zero, infinite
+// or NaN operand it returns the IEEE quotient (x/0 = inf, x/inf = 0, 0/0 and
+// inf/inf NaN), otherwise the refined quotient with the operands' sign.
+// Quotients beyond the FP64 range and denormal arguments are outside the
+// domain of every caller.
 
 __device__ __forceinline__ bool finite_nonzero(double r) {
     return __builtin_isfinite(r) && r != 0.0;
 }
 
-__device__ __forceinline__ double rcp(double x) {
+// 1/x refined, for x finite and away from zero (a zero or infinite x gives NaN)
+__device__ __forceinline__ double rcp_fin(double x) {
     double r = __builtin_amdgcn_rcp(x);
     double e = fma(-x, r, 1.0);
     double q = fma(r, e, r);
     e = fma(-x, q, 1.0);
-    q = fma(q, e, q);
-    return finite_nonzero(r) ? q : r;
+    return fma(q, e, q);
 }
 
+__device__ __forceinline__ double rcp(double x) {
+    return __builtin_amdgcn_div_fixup(rcp_fin(x), x, 1.0);
+}
+
+// a / b: the refined reciprocal, the product and one residual correction
 __device__ __forceinline__ double fdiv(double a, double b) {
-    double r = rcp(b);
-    double q = a * r;
-    double c = fma(fma(-b, q, a), r, q);
-    return __builtin_isfinite(q) ? c : q;
+    const double r = rcp_fin(b);
+    const double q = a * r;
+    return __builtin_amdgcn_div_fixup(fma(fma(-b, q, a), r, q), b, a);
 }
 
 __device__ __forceinline__ double rsqrt(double x) {
@@ -146,34 +156,20 @@ __device__ __forceinline__ double rsqrt(double x) {
     return finite_nonzero(y) ? z : y;
 }
 
-// sqrt(x) from x * rsqrt(x) with one residual correction
+// sqrt(x) from x * rsqrt(x) with one residual correction; x = +-0 and +inf
+// return x (the IEEE results), negative and NaN x give NaN through ry
 __device__ __forceinline__ double fsqrt_with(double x, double ry) {
     double s = x * ry;
     double t = fma(-s, s, x);
     s = fma(t, 0.5 * ry, s);
-    return (x == 0.0 || !__builtin_isfinite(x)) ? x + x * 0.0 : s;
+    constexpr int kZeroOrPosInf = (1 << 5) | (1 << 6) | (1 << 9);    // v_cmp_class: -0, +0, +inf
+    return __builtin_amdgcn_class(x, kZeroOrPosInf) ? x : s;
 }
 
 __device__ __forceinline__ double fsqrt(double x) { return fsqrt_with(x, rsqrt(x)); }
 
-// Variants for operands that the caller has shown to be finite and away from
-// zero (or whose special values end the caller's iteration the same way):
-// the same refinement without the special-value selects.
-__device__ __forceinline__ double rcp_fin(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    double q = fma(r, e, r);
-    e = fma(-x, q, 1.0);
-    return fma(q, e, q);
-}
-
-// a / b for b finite, nonzero; a may be any value (inf and NaN propagate)
-__device__ __forceinline__ double fdiv_fin(double a, double b) {
-    double r = rcp_fin(b);
-    double q = a * r;
-    double c = fma(fma(-b, q, a), r, q);
-    return __builtin_isfinite(q) ? c : q;
-}
+// a / b for b finite, nonzero (the same sequence as fdiv)
+__device__ __forceinline__ double fdiv_fin(double a, double b) { return fdiv(a, b); }
 
 // 1/sqrt(x) for x > 0 finite; x <= 0 gives NaN (the Newton loops end on it)
 __device__ __forceinline__ double rsqrt_pos(double x) {
@@ -618,6 +614,10 @@ __device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& ite
         // Special values inside the loop: t <= 0 (outside the conic) gives NaN
         // through rsqrt_pos, and NaN or infinite F, dF end the loop at this
         // iterate exactly as the IEEE forms do (s2 is not used after the loop).
+        // The step F/dF is F times a refined reciprocal, within 2 ulp of the
+        // quotient: the iterates agree with the IEEE ones to rounding, and the
+        // exit test |s2 - s1| >= eps only differs for a step within rounding
+        // of eps.
         const double K = p[D_K1C2];
         const double X0 = r.X, Y0 = r.Y, cX = r.cX, cY = r.cY, cZ = r.cZ;
         double s0 = fdiv(-r.Z, cZ);
@@ -633,14 +633,18 @@ __device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& ite
             double r2 = Xi * Xi + Yi * Yi;
             double t = 1.0 - K * r2;
             double irt = rsqrt_pos(t);
-            double F = Zi - (c * r2) * rcp_fin(1.0 + fsqrt_pos(t, irt));
+            const double u = 1.0 + fsqrt_pos(t, irt);
             double P, Q;
             asphere_pq(d, r2, P, Q);
-            F -= P;
             double E = c * irt + 2.0 * Q;
             double dF = -E * Xi * cX - E * Yi * cY + cZ;
             if (fabs(dF) < kNewtonEps) { alive = false; break; }
-            s2 = s1 - fdiv_fin(F, dF);
+            // one reciprocal for both quotients: w = 1/(u dF), 1/u = dF w,
+            // 1/dF = u w (each within 2 ulp)
+            const double w = rcp_fin(u * dF);
+            double F = Zi - (c * r2) * (dF * w);
+            F -= P;
+            s2 = fma(-F, u * w, s1);
             ++it;
         } while ((fabs(s2 - s1) >= kNewtonEps) && (it <= kNewtonMaxIter));
         r.X = Xi; r.Y = Yi; r.Z = Zi;
@@ -673,7 +677,7 @@ __device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& ite
             double dFy = c * Yi * irt + 2.0 * Q * Yi;
             double dF = -cR * Xi * cX + (cR * Fy - 1.0) * dFy * cY + (1.0 - cR * Zi) * cZ;
             if (fabs(dF) < kNewtonEps) { alive = false; break; }
-            s2 = s1 - fdiv_fin(F, dF);
+            s2 = fma(-F, rcp_fin(dF), s1);
             ++it;
         } while ((fabs(s2 - s1) >= kNewtonEps) && (it <= kNewtonMaxIter));
         r.X = Xi; r.Y = Yi; r.Z = Zi;
@@ -833,6 +837,13 @@ __device__ __forceinline__ bool diffract_ray(const SD& d, Ray& r, double mu, dou
     return true;
 }
 
+// A derived-table value whose wavelength lies outside a function's table: a
+// NaN with its own payload (compared bit for bit, so it differs from any NaN
+// the arithmetic makes).
+constexpr uint64_t kOutOfTableBits = 0x7ff8000000010001ull;
+__device__ __forceinline__ double out_of_table_value() { return __builtin_bit_cast(double, kOutOfTableBits); }
+__device__ __forceinline__ bool out_of_table(double v) { return __builtin_bit_cast(uint64_t, v) == kOutOfTableBits; }
+
 // grating order of a ray: Order[0] (surface.cpp:570) or the echelle blaze rule
 template <class SD>
 __device__ __forceinline__ double grating_order(const SD& d, double lam) {
@@ -889,9 +900,10 @@ struct TraceArgs {
     const uint64_t* ray_start;
     double* tile_moments;         // per tile: n, mean x, mean y, M2 x, M2 y of the rays alive at the end
     uint64_t n_inst, n_tiles, inst_stride;
-    // grid-wavelength beams: per-wavelength values of the scheme's spline
-    // tables, ltab[q * lt_nk + j] for column q at grid index j (generated
-    // kernels built for it only)
+    // grid-wavelength beams: per-wavelength values, ltab[q * lt_nk + j] for
+    // column q at grid index j (generated kernels built for it only): the
+    // derived table (dtab_body) for a single scheme, the spline tables
+    // (ltab_body) for a batch
     const double* ltab;
     uint64_t lt_nk;
     const uint32_t* tile_inst;    // batched traces: the instance of each tile
@@ -1014,11 +1026,13 @@ __device__ __forceinline__ void ext_add(unsigned long long* e, int q, bool on, d
 // a detector.  Counters are wave-aggregated into LDS.
 //
 // lv, col: the ray's per-wavelength table values (generated kernels of
-// grid-wavelength beams): this surface's spline-valued functions read
-// lv[col], lv[col + 1], ... in the order n1 (unless reused), n2, QE,
-// spectrum, a NaN meaning a wavelength outside the table; col < 0 evaluates
-// the splines here.
-template <class SD, int MODE = 0>
+// grid-wavelength beams).  DERIV false: this surface's spline-valued
+// functions read lv[col], lv[col + 1], ... in the order n1 (unless reused),
+// n2, QE, spectrum, a NaN meaning a wavelength outside the table; col < 0
+// evaluates the splines here.  DERIV true: the derived table's columns
+// (dtab_body) give mu, then the grating term, or the detector's flux weight,
+// kOutOfTable where a function's wavelength is outside its table.
+template <class SD, int MODE = 0, bool DERIV = false>
 __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& alive, TraceCtx& c,
                                              const double* lv = nullptr, int col = -1) {
     // the record's loads are issued at this surface, not hoisted ahead of
@@ -1051,6 +1065,16 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
         if (alive) {                                             // :1035
             if (d.type() == RT_MIRROR) {
                 reflect_ray(d, r);
+            } else if (DERIV && (d.type() == RT_LENS || d.type() == RT_GRATING)) {
+                const double mu = lv[col];
+                if (out_of_table(mu)) {
+                    alive = false;
+                    ++c.n_oob;
+                } else if (d.type() == RT_LENS) {
+                    alive = refract_ray(d, r, mu);
+                } else {
+                    alive = diffract_ray(d, r, mu, lv[col + 1]);
+                }
             } else if (d.type() == RT_LENS || d.type() == RT_GRATING) {
                 int e1 = 0, e2 = 0;
                 double n1, n2;
@@ -1109,20 +1133,28 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
     }
     if (d.ccd() && (c.do_hist || c.do_flux)) {
         // ccd_coordinates_kernel (rt_cuda_engine_ccd.cu:30-45) with a bounds check (D10);
-        // xdim * ydim <= 2^32 (validated), so a pixel index fits 32 bits
+        // xdim * ydim <= 2^32 (validated), so a pixel index fits 32 bits.  The
+        // coordinate divided by the pixel size is taken as a product with the
+        // reciprocal (within 1 ulp of the quotient: a ray can change pixel
+        // only within about 1e-12 px of an edge, and rays within 1e-9 px of
+        // one are counted in RT_ST_CCD_EDGE)
         const uint32_t xdim = (uint32_t)p[RT_S_CCD_XDIM], ydim = (uint32_t)p[RT_S_CCD_YDIM];
         bool on = false;
         uint32_t pix = 0;
         if (alive) {
-            double vx = (r.X + p[D_HALF_X]) / p[RT_S_CCD_XPIX];
-            double vy = (r.Y + p[D_HALF_Y]) / p[RT_S_CCD_YPIX];
+            double vx = (r.X + p[D_HALF_X]) * p[D_INV_XPIX];
+            double vy = (r.Y + p[D_HALF_Y]) * p[D_INV_YPIX];
             uint32_t px = trunc_u32(vx), py = trunc_u32(vy);
             on = px < xdim && py < ydim;
             if (on) {
                 pix = py * xdim + px;
                 ++c.n_hit;
                 if (near_edge(vx) || near_edge(vy)) ++c.n_edge;
-                if (c.do_flux) {
+                if (DERIV && c.do_flux) {
+                    const double w = lv[col];
+                    if (out_of_table(w)) ++c.n_oob;
+                    else unsafeAtomicAdd(&c.flux[pix], w);
+                } else if (c.do_flux) {
                     // compute_ccd_image_kernel (ccd.cu:219-232), weights of ccd.cu:286-287,360
                     int eq = 0, es = 0;
                     double qe, sp;
@@ -1210,6 +1242,74 @@ __device__ __forceinline__ void ltab_body(const LtabArgs& a) {
     int err = 0;
     const double v = spline_eval(a.pool + (size_t)a.col_off[q], lam, err);
     a.out[idx] = err ? __builtin_nan("") : v;
+}
+
+// Arguments of the derived-table kernel; the beam first (offset 0 of the
+// argument segment, read through the constant address space).
+struct DtabArgs {
+    BeamArg beam;
+    const double* table;          // n_surf device records
+    const double* pool;           // device spline pool
+    double* out;                  // n_cols x nk
+    uint64_t nk;
+    int n_surf, pad;
+};
+
+// out[q * nk + j] at the wavelength of grid index j for the surface of
+// workgroup row blockIdx.y, in the column order of rt_jit_dtab_layout
+// (surfaces in order): for a lens or a grating mu = n1 / n2, and for a
+// grating L = m lambda / (n2 d), as the per-ray action forms them
+// (surface.cu:1095-1096,1568); for a CCD detector the flux weight of
+// compute_ccd_image_kernel (ccd.cu:286-287,360) as the per-ray flux forms
+// it.  Functions evaluated by the per-ray spline code on the same
+// wavelength, so every value is the one the per-ray path computes;
+// kOutOfTable where the per-ray path finds a wavelength outside a table.
+__device__ __forceinline__ int dtab_cols(const DynSurf& d) {
+    const int type = d.type();
+    return type == RT_GRATING ? 2 : (type == RT_LENS || d.ccd()) ? 1 : 0;
+}
+
+__device__ __forceinline__ void dtab_body(const DtabArgs& a) {
+    const int s = blockIdx.y;
+    const DynSurf d{(cptr)(a.table + (size_t)s * DEV_STRIDE)};
+    const int nc = dtab_cols(d);
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nc == 0 || j >= a.nk) return;
+    uint64_t q = 0;                                   // the surface's first column
+    for (int k = 0; k < s; ++k) q += dtab_cols(DynSurf{(cptr)(a.table + (size_t)k * DEV_STRIDE)});
+    cptr B = (cptr)__builtin_amdgcn_kernarg_segment_ptr();
+    const double lam = (int)B[RT_B_RANGE_DISTR] == RT_RANGE_MONO
+                           ? mono_lambda(B)
+                           : grid_lambda(B, (uint64_t)B[RT_B_NLAMBDA], j);
+    const cptr p = d.p;
+    const int type = d.type();
+    if (type == RT_LENS || type == RT_GRATING) {
+        int e1 = 0, e2 = 0;
+        const double n1 = fn_eval(a.pool, d.n1spline(), p[RT_S_N1VAL], lam, e1);
+        const double n2 = fn_eval(a.pool, d.n2spline(), p[RT_S_N2VAL], lam, e2);
+        double mu = out_of_table_value(), Lg = out_of_table_value();
+        if (!(e1 | e2)) {
+            const double in2 = rcp(n2);
+            mu = n1 * in2;
+            if (type == RT_GRATING) {
+                const double m = grating_order(d, lam);
+                Lg = m * lam * in2 * p[D_INV_D];
+            }
+        }
+        a.out[q * a.nk + j] = mu;
+        if (type == RT_GRATING) a.out[(q + 1) * a.nk + j] = Lg;
+    } else {
+        int eq = 0, es = 0;
+        const double qe = fn_eval(a.pool, d.qespline(), p[RT_S_QEVAL], lam, eq);
+        const double sp = fn_eval(a.pool, d.spspline(), p[RT_S_SPVAL], lam, es);
+        double w = out_of_table_value();
+        if (!(eq | es)) {
+            double spec = sp * lam;
+            double pix_area = p[RT_S_CCD_XPIX] * p[RT_S_CCD_YPIX] * 1.0E-2;
+            w = spec * (1.0 / kHC) * qe * pix_area;
+        }
+        a.out[q * a.nk + j] = w;
+    }
 }
 
 // Kernel body: the spline pool goes to LDS, every lane of a persistent grid

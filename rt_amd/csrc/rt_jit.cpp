@@ -116,6 +116,45 @@ void rt_jit_ltab_layout(const std::string& sig, std::vector<int>& col_base, std:
     }
 }
 
+// The columns of the derived per-wavelength table (single-scheme traces):
+// per surface, in order, mu = n1/n2 for a lens or a grating, then the
+// diffraction term L = m lambda / (n2 d) for a grating, and the flux weight
+// for a CCD detector (StatSurf's TYPE and CCD, its 2nd and 11th template
+// arguments).
+void rt_jit_dtab_layout(const std::string& sig, std::vector<int>& col_base, std::vector<int>& col_surf,
+                        std::vector<int>& col_role) {
+    col_base.clear();
+    col_surf.clear();
+    col_role.clear();
+    size_t pos = 0;
+    int s = 0;
+    while (pos < sig.size()) {
+        const size_t end = sig.find(';', pos);
+        const std::string t = sig.substr(pos, end - pos);
+        int type = -1;
+        bool ccd = false;
+        size_t q = t.find('<');
+        for (int k = 0; q != std::string::npos && k <= 10; ++k) {
+            ++q;
+            if (k == 1) type = atoi(t.c_str() + q);
+            if (k == 10) ccd = t.compare(q, 4, "true") == 0;
+            q = t.find(k < 10 ? ',' : '>', q);
+        }
+        col_base.push_back(-1);
+        int roles[2], nr = 0;
+        if (type == RT_LENS || type == RT_GRATING) roles[nr++] = RT_DTAB_MU;
+        if (type == RT_GRATING) roles[nr++] = RT_DTAB_L;
+        if (type == RT_DETECTOR && ccd) roles[nr++] = RT_DTAB_W;
+        for (int r = 0; r < nr; ++r) {
+            if (col_base.back() < 0) col_base.back() = (int)col_surf.size();
+            col_surf.push_back(s);
+            col_role.push_back(roles[r]);
+        }
+        pos = end + 1;
+        ++s;
+    }
+}
+
 bool rt_jit_enabled() {
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_mode < 0) {
@@ -154,12 +193,16 @@ static std::string generate_source(const std::string& sig, int beam_mode, int va
                                    bool batch = false) {
     const bool dump = (variant & RT_JIT_VAR_DUMP) != 0;
     const int rec = ((variant & RT_JIT_VAR_LOSS) ? rt::kRecLoss : 0) | ((variant & RT_JIT_VAR_EXT) ? rt::kRecExt : 0);
-    // RT_JIT_LTAB: the spline-valued functions come from the per-wavelength
-    // table (TraceArgs::ltab), loaded at the start of each ray
+    // RT_JIT_LTAB: per-wavelength values come from a table (TraceArgs::ltab),
+    // each column loaded at the surface that reads it: for a single scheme the
+    // derived table (mu, the grating term, the flux weight), for a batch the
+    // spline-valued functions themselves
     const bool ltab = beam_mode >= 0 && (beam_mode & RT_JIT_LTAB) != 0;
     if (beam_mode >= 0) beam_mode &= ~RT_JIT_LTAB;
+    const bool derived = ltab && !batch;
     std::vector<int> col_base, col_surf, col_role;
-    rt_jit_ltab_layout(sig, col_base, col_surf, col_role);
+    if (derived) rt_jit_dtab_layout(sig, col_base, col_surf, col_role);
+    else rt_jit_ltab_layout(sig, col_base, col_surf, col_role);
     const bool use_tab = ltab && !col_surf.empty();
     std::ostringstream os;
     os << "#include \"rt_device.h\"\n"
@@ -179,7 +222,7 @@ static std::string generate_source(const std::string& sig, int beam_mode, int va
             for (size_t q = 0; q < col_surf.size(); ++q)
                 if (col_surf[q] == s) os << "    rt::load_lval(lv, c, " << q << ");\n";
         os << "    rt::surface_step";
-        if (rec) os << "<" << type << ", " << rec << ">";
+        if (rec || (derived && use_tab)) os << "<" << type << ", " << rec << (derived && use_tab ? ", true" : "") << ">";
         os << "(" << type << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s << ", r, alive, c";
         if (use_tab && col_base[(size_t)s] >= 0) os << ", lv, " << col_base[(size_t)s];
         os << ");\n";

@@ -16,6 +16,15 @@ std::string rt_jit_signature(const double* dtable, int n_surf);
 void rt_jit_ltab_layout(const std::string& signature, std::vector<int>& col_base, std::vector<int>& col_surf,
                         std::vector<int>& col_role);
 
+// The columns of the derived per-wavelength table that generated
+// single-scheme traces read (RT_JIT_LTAB without batch): column q holds role
+// col_role[q] (RT_DTAB_MU n1/n2, RT_DTAB_L the grating's m lambda / (n2 d),
+// RT_DTAB_W the detector's flux weight) of surface col_surf[q]; col_base[s]
+// is surface s's first column or -1.  rt::dtab_body writes the same order.
+enum { RT_DTAB_MU = 0, RT_DTAB_L = 1, RT_DTAB_W = 2 };
+void rt_jit_dtab_layout(const std::string& signature, std::vector<int>& col_base, std::vector<int>& col_surf,
+                        std::vector<int>& col_role);
+
 // Whether generated kernels are used (environment RT_HIP_JIT=0 disables;
 // rt_hip_set_jit overrides).
 bool rt_jit_enabled();

@@ -404,12 +404,35 @@ struct Ltab {
 };
 constexpr uint64_t kLtabMax = 16384;   // wavelengths of the largest grid served by the table
 
+// The derived per-wavelength table of a scheme's generated traces
+// (rt_jit_dtab_layout): n_cols x kLtabMax doubles, filled on the launch
+// stream before a trace whose wavelength grid or stream differs from those of
+// the last fill (a trace on another stream is not ordered after that fill).
+// Once a launch is captured into a graph, every launch fills it (a replay may
+// run at any later time, with the grid it was captured with).  Traces of one
+// scheme with different wavelength grids on two streams at once race on it.
+struct Dtab {
+    int n_cols = 0;
+    double* tab_dev = nullptr;
+    bool valid = false;              // tab_dev holds the grid of key
+    bool always = false;             // a launch was captured: fill every time
+    double key[5] = {0, 0, 0, 0, 0};
+    hipStream_t stream = nullptr;    // stream of the last fill
+    void release() {
+        (void)hipFree(tab_dev);
+        tab_dev = nullptr;
+        n_cols = 0;
+        valid = false;
+    }
+};
+
 struct rt_hip_scheme {
     int device;
     std::string signature;           // structural key of the generated kernels
     int n_surf;
     size_t pool_len;                 // device pool (doubles)
     double* table_dev;               // n_surf records of rt::DEV_STRIDE
+    mutable Dtab dt;                 // derived per-wavelength table of the generated traces
     double* pool_dev;
     std::vector<double> table;       // public packed table, as given
     std::vector<double> pool;        // public spline pool, as given
@@ -559,6 +582,8 @@ void rt_device_record(const double* p, double* d) {
     d[rt::D_INV_D] = p[RT_S_GRCONST] != 0.0 ? 1.0 / p[RT_S_GRCONST] : 0.0;
     d[rt::D_HALF_X] = p[RT_S_CCD_XSIZE] / 2.0;
     d[rt::D_HALF_Y] = p[RT_S_CCD_YSIZE] / 2.0;
+    d[rt::D_INV_XPIX] = p[RT_S_CCD_XPIX] != 0.0 ? 1.0 / p[RT_S_CCD_XPIX] : 0.0;
+    d[rt::D_INV_YPIX] = p[RT_S_CCD_YPIX] != 0.0 ? 1.0 / p[RT_S_CCD_YPIX] : 0.0;
     d[rt::D_K1C2] = (p[RT_S_K] + 1) * p[RT_S_C] * p[RT_S_C];
     for (int i = 1; i <= RT_MAX_POLY; ++i) d[rt::D_IA + i - 1] = (double)i * p[RT_S_A + i - 1];
 }
@@ -606,6 +631,8 @@ static void build_device_tables(rt_hip_scheme* s) {
 
 __global__ __launch_bounds__(256) void rt_ltab_kernel(rt::LtabArgs a) { rt::ltab_body(a); }
 
+__global__ __launch_bounds__(256) void rt_dtab_kernel(rt::DtabArgs a) { rt::dtab_body(a); }
+
 __global__ __launch_bounds__(256) void rt_stats_fold_kernel(unsigned long long* copies, unsigned long long* stats) {
     rt::stats_fold_body(copies, stats);
 }
@@ -651,16 +678,59 @@ int stats_fold(unsigned long long* copies, unsigned long long* stats, hipStream_
     return RT_OK;
 }
 
-// wavelengths of the beam's grid when the table serves it (one wavelength, or
-// a composite uniform grid of at most kLtabMax), else 0
-uint64_t ltab_nk(const Ltab& t, const double* beam) {
-    if (!t.ok || !rt_jit_ltab_enabled() || !rt_jit_enabled()) return 0;
+// wavelengths of the beam's grid when per-wavelength tables serve it (one
+// wavelength, or a composite uniform grid of at most kLtabMax), else 0
+uint64_t grid_nk(const double* beam) {
+    if (!rt_jit_ltab_enabled() || !rt_jit_enabled()) return 0;
     const int distr = (int)beam[RT_B_RANGE_DISTR];
     if (distr == RT_RANGE_MONO) return 1;
     if (distr != RT_RANGE_UNIFORM || beam[RT_B_COMPOSITE] == 0.0) return 0;
     const double nl = beam[RT_B_NLAMBDA];
     if (!(nl >= 1.0) || nl > (double)kLtabMax) return 0;
     return (uint64_t)nl;
+}
+
+uint64_t ltab_nk(const Ltab& t, const double* beam) { return t.ok ? grid_nk(beam) : 0; }
+
+// the derived table's columns for the signature; its buffer on first use
+int dtab_setup(Dtab& t, const std::string& sig) {
+    std::vector<int> base, surf, role;
+    rt_jit_dtab_layout(sig, base, surf, role);
+    t.n_cols = (int)surf.size();
+    if (t.n_cols > 0 && !t.tab_dev)
+        HIP_TRY(hipMalloc(&t.tab_dev, (size_t)t.n_cols * kLtabMax * sizeof(double)), RT_ERR_BAD_ALLOC);
+    return RT_OK;
+}
+
+uint64_t dtab_nk(const Dtab& t, const double* beam) { return t.n_cols > 0 ? grid_nk(beam) : 0; }
+
+// the derived table for nk grid wavelengths of a's beam, filled on `st`
+// unless it holds that grid already
+int dtab_fill(Dtab& t, const TraceArgs& a, const double* table_dev, const double* pool_dev, int n_surf,
+              uint64_t nk, hipStream_t st) {
+    const double key[5] = {a.beam.v[RT_B_RANGE_DISTR], a.beam.v[RT_B_RANGE + 0], a.beam.v[RT_B_RANGE + 1],
+                           a.beam.v[RT_B_NLAMBDA], (double)nk};
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(st, &cs), RT_ERR_FAILED);
+    if (cs != hipStreamCaptureStatusNone) t.always = true;
+    if (!t.always && t.valid && st == t.stream && memcmp(key, t.key, sizeof key) == 0) return RT_OK;
+    t.valid = false;
+    rt::DtabArgs da;
+    memset(&da, 0, sizeof da);
+    da.beam = a.beam;
+    da.table = table_dev;
+    da.pool = pool_dev;
+    da.out = t.tab_dev;
+    da.nk = nk;
+    da.n_surf = n_surf;
+    hipLaunchKernelGGL(rt_dtab_kernel, dim3((unsigned)((nk + 255) / 256), (unsigned)n_surf), dim3(256), 0, st, da);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    if (!t.always) {
+        memcpy(t.key, key, sizeof key);
+        t.stream = st;
+        t.valid = true;
+    }
+    return RT_OK;
 }
 
 // fill the table for nk grid wavelengths of a's beam on `st`
@@ -715,10 +785,12 @@ extern "C" int rt_hip_scheme_create(const double* surf_table, int n_surf, const 
         return set_error(RT_ERR_FAILED, "upload of scheme tables failed");
     }
     rc = ltab_setup(s->lt, s->signature, s->dtable.data(), 1, n_surf);
+    if (rc == RT_OK) rc = dtab_setup(s->dt, s->signature);
     if (rc == RT_OK) rc = stat_copies_alloc(&s->stat_copies);
     if (rc != RT_OK) {
         (void)hipFree(s->stat_copies);
         s->lt.release();
+        s->dt.release();
         (void)hipFree(s->table_dev);
         (void)hipFree(s->pool_dev);
         delete s;
@@ -749,6 +821,7 @@ extern "C" int rt_hip_scheme_destroy(rt_hip_scheme* s) {
     s->work.release();
     (void)hipFree(s->stat_copies);
     s->lt.release();
+    s->dt.release();
     (void)hipFree(s->table_dev);
     (void)hipFree(s->pool_dev);
     delete s;
@@ -831,7 +904,7 @@ int trace_launch(const rt_hip_scheme* sch, const double* beam_desc, uint64_t see
     hipStream_t st = (hipStream_t)stream;
     const int beam_mode = gen ? (int)a.beam.v[RT_B_TYPE] : kLoad;
 
-    const uint64_t nk = gen && sch->unroll ? ltab_nk(sch->lt, beam_desc) : 0;
+    const uint64_t nk = gen && sch->unroll ? dtab_nk(sch->dt, beam_desc) : 0;
     hipFunction_t fn = nullptr;
     if (rt_jit_enabled() && sch->unroll)
         fn = rt_jit_kernel(sch->signature, beam_mode | (nk ? RT_JIT_LTAB : 0),
@@ -840,9 +913,9 @@ int trace_launch(const rt_hip_scheme* sch, const double* beam_desc, uint64_t see
                            sch->block, sch->waves, sch->device);
     if (fn) {
         if (nk) {
-            rc = ltab_fill(sch->lt, a, sch->pool_dev, nk, st);
+            rc = dtab_fill(sch->dt, a, sch->table_dev, sch->pool_dev, sch->n_surf, nk, st);
             if (rc != RT_OK) return rc;
-            a.ltab = sch->lt.tab_dev;
+            a.ltab = sch->dt.tab_dev;
             a.lt_nk = nk;
         }
         // persistent grid: the workgroups that are resident at once on every CU
@@ -1056,7 +1129,7 @@ extern "C" int rt_hip_autotune(rt_hip_scheme* sch, const double* beam_desc, uint
                 if ((rc = run()) != RT_OK) break;
                 if (hipStreamSynchronize(st) != hipSuccess) { rc = set_error(RT_ERR_FAILED, "autotune"); break; }
                 int local = 0;
-                const int mode = beam_mode | (ltab_nk(sch->lt, beam_desc) ? RT_JIT_LTAB : 0);
+                const int mode = beam_mode | (dtab_nk(sch->dt, beam_desc) ? RT_JIT_LTAB : 0);
                 hipFunction_t fn =
                     rt_jit_enabled() ? rt_jit_kernel(sch->signature, mode, false, b, w, sch->device) : nullptr;
                 if (fn && hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fn) != hipSuccess)

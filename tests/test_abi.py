@@ -95,17 +95,28 @@ def test_generated_kernel_compiles(make, mode):
 
 
 def test_generated_table_kernel_compiles():
-    """The trace generated for a grid-wavelength beam reading the
-    per-wavelength table (0x400 = RT_JIT_LTAB) compiles, with one column per
-    spline-valued function that is not reused from the previous surface: the
-    echelle's three glasses (n2 of each first lens surface) and the detector
-    QE; each column is loaded just before the surface that uses it."""
+    """The trace generated for a grid-wavelength beam reading a per-wavelength
+    table (0x400 = RT_JIT_LTAB) compiles.  A single scheme's trace reads the
+    derived table: mu for each lens and grating, the grating term for each
+    grating, the flux weight for the CCD detector -- on the echelle, 2 + 2
+    columns for its two gratings, 6 for its six lens surfaces and 1 for the
+    detector, each loaded just before the surface that uses it.  The batched
+    trace (0x100 = RT_JIT_BATCH) reads one column per spline-valued function
+    that is not reused from the previous surface: the three glasses (n2 of
+    each first lens surface) and the detector QE."""
     sch = schemes.echelle(1, 4)
     rc, size, text = lib.jit_check(sch, S.RT_BEAM_GAUSS | 0x400)
+    assert rc == 0, text[-2000:]
+    assert "double lv[11];" in text
+    assert text.count("rt::load_lval(lv, c, ") == 11
+    assert text.count(", lv, ") == 9
+    assert text.count(", 0, true>(") == sch.n_surf
+    rc, size, text = lib.jit_check(sch, S.RT_BEAM_GAUSS | 0x400 | 0x100)
     assert rc == 0, text[-2000:]
     assert "double lv[4];" in text
     assert text.count("rt::load_lval(lv, c, ") == 4
     assert text.count(", lv, ") == 4
+    assert "true>(" not in text
     # without the flag: no table
     rc, size, text = lib.jit_check(sch, S.RT_BEAM_GAUSS)
     assert rc == 0 and "lv" not in text

@@ -303,3 +303,67 @@ def test_wavelength_table_matches_per_ray_splines(hip, make):
     assert int(on["stats"][S.ST_RAYS]) == sch.beam.n_rays
     if make is _echelle_oob:
         assert int(on["stats"][S.ST_SPLINE_OOB]) > 0
+
+
+def test_derived_table_follows_beam_stream_and_graph(hip):
+    """The derived per-wavelength table of a scheme is refilled when the
+    wavelength grid or the stream changes, and on every launch once a launch
+    has been captured in a graph: traces of two beams alternated on one
+    scheme, on two streams, and a captured trace replayed after an eager trace
+    of the other beam each equal the trace of a fresh scheme that has seen
+    only that beam (bit for bit: same kernel, same table values)."""
+    import dataclasses
+    sch = schemes.echelle(300, 64)
+    beam_a = sch.beam
+    beam_b = dataclasses.replace(beam_a, range=(beam_a.range[0] + 0.05, beam_a.range[1] + 0.02))
+    n = beam_a.n_rays
+    flags = S.RT_TRACE_GENERATE | S.RT_TRACE_STORE
+
+    def run(ds, beam, stream=None):
+        R = torch.zeros((7, n), dtype=torch.float64, device="cuda")
+        F = torch.zeros(n, dtype=torch.int8, device="cuda")
+        ds.trace(R, F, seed=5, flags=flags, beam=beam, stream=stream)
+        return R, F
+
+    def fresh(beam):
+        ds = hip.DeviceScheme(sch)
+        R, F = run(ds, beam)
+        torch.cuda.synchronize()
+        assert hip.last_kernel() == "generated"
+        out = R.cpu().numpy().view(np.uint64).copy(), F.cpu().numpy().copy()
+        ds.close()
+        return out
+
+    ref = {"a": fresh(beam_a), "b": fresh(beam_b)}
+    assert not np.array_equal(ref["a"][0][6], ref["b"][0][6])
+
+    def same(R, F, key):
+        torch.cuda.synchronize()
+        assert np.array_equal(R.cpu().numpy().view(np.uint64), ref[key][0]), key
+        assert np.array_equal(F.cpu().numpy(), ref[key][1]), key
+
+    ds = hip.DeviceScheme(sch)
+    print("[derived table] alternating beams", flush=True)
+    for key in ("a", "b", "a", "a", "b"):
+        same(*run(ds, beam_a if key == "a" else beam_b), key)
+    print("[derived table] second stream", flush=True)
+    s2 = torch.cuda.Stream()
+    with torch.cuda.stream(s2):
+        R, F = run(ds, beam_b, stream=s2.cuda_stream)
+    s2.synchronize()
+    same(R, F, "b")
+    print("[derived table] graph replays between eager traces of the other beam", flush=True)
+    Rg = torch.zeros((7, n), dtype=torch.float64, device="cuda")
+    Fg = torch.zeros(n, dtype=torch.int8, device="cuda")
+    g = torch.cuda.CUDAGraph()
+    cs = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=cs):
+        ds.trace(Rg, Fg, seed=5, flags=flags, beam=beam_a, stream=cs.cuda_stream)
+    for _ in range(3):
+        same(*run(ds, beam_b), "b")
+        Rg.zero_()
+        g.replay()
+        same(Rg, Fg, "a")
+    same(*run(ds, beam_b), "b")
+    ds.close()
