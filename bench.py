@@ -254,8 +254,9 @@ def adjoint_ratio(warmup, steps):
     echelle at 15625 rays x 64 wavelengths (1e6 rays), loss = sum over the
     wavelengths of the RMS spot radius, gradient over 48 design parameters.
     Times: the plain forward trace (beam generated, nothing stored), the loss
-    forward (rays stored; spots, loss and adjoint seeds reduced on the
-    device), the adjoint launch (kernel + fixed-order gradient reduction);
+    forward (the trace forming the group spots itself, rt_hip_trace_spots;
+    loss and adjoint seeds reduced on the device), the adjoint launch (kernel
+    + fixed-order gradient reduction);
     ratio = (loss forward + adjoint) / plain forward, on the same rays and
     surfaces."""
     import torch
@@ -266,15 +267,13 @@ def adjoint_ratio(warmup, steps):
     v0 = spec.values(base)
     n = base.beam.n_rays
     ds = lib.DeviceScheme(base)
-    rays = torch.zeros((7, n), dtype=torch.float64, device="cuda")
-    flag = torch.zeros(n, dtype=torch.int8, device="cuda")
+    spots = torch.zeros((64, lib.SPOT_STRIDE), dtype=torch.float64, device="cuda")
     ps = np.array([p.surface for p in spec.params], dtype=np.int32)
     pf = np.array([p.field for p in spec.params], dtype=np.int32)
     seeds = torch.zeros((64, 3), dtype=torch.float64, device="cuda")
 
     def loss_forward():
-        ds.trace(rays, flag, seed=5, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE)
-        sp = lib.group_spots(rays, flag, 64)
+        sp = ds.trace_spots(64, seed=5, out=spots)
         loss, sd = lib.spot_loss(sp)
         seeds.copy_(sd)
         return loss
@@ -363,10 +362,11 @@ def batch_adjoint_ratio(n_inst, warmup, steps, max_rays):
     (64 wavelengths, ray counts log-uniform in [1e4, --max-rays]), the spot
     loss per instance (sum over wavelengths of the RMS spot radius), gradient of
     every instance over the 48 design parameters.  Times per step: the plain
-    batched forward (beams generated, nothing stored), the loss forward (rays
-    stored; spots per instance and wavelength, loss and adjoint seeds on the
-    device), the batched adjoint (kernel + per-instance reduction).  Ratio =
-    (loss forward + adjoint) / plain forward on the same rays and surfaces; the
+    batched forward (beams generated, nothing stored), the loss forward (the
+    batched trace forming the spots per instance and wavelength itself,
+    rt_hip_batch_trace_group_spots; loss and adjoint seeds on the device), the
+    batched adjoint (kernel + per-instance reduction).  Ratio = (loss forward
+    + adjoint) / plain forward on the same rays and surfaces; the
     per ray*surface ratio against the single-scheme forward rate of the
     default bench is in detail."""
     import torch
@@ -377,20 +377,18 @@ def batch_adjoint_ratio(n_inst, warmup, steps, max_rays):
     inst, counts = schemes.tolerance_instances(base, n_inst, seed=1, min_rays=10_000, max_rays=max_rays)
     db = lib.DeviceBatch(inst, counts)
     total = int(sum(counts))
-    rays = torch.zeros((7, total), dtype=torch.float64, device="cuda")
-    flag = torch.zeros(total, dtype=torch.int8, device="cuda")
     ps = np.array([p.surface for p in spec.params], dtype=np.int32)
     pf = np.array([p.field for p in spec.params], dtype=np.int32)
     seeds = torch.zeros((len(inst), 64, 3), dtype=torch.float64, device="cuda")
     grad = torch.zeros((len(inst), len(ps)), dtype=torch.float64, device="cuda")
     stats = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
+    spots = torch.zeros((len(inst), 64, lib.SPOT_STRIDE), dtype=torch.float64, device="cuda")
 
     def plain():
         db.trace(seed=5, flags=S.RT_TRACE_GENERATE, stats=stats)
 
     def loss_forward():
-        db.trace(seed=5, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, rays=rays, flag=flag)
-        sp = db.group_spots(rays, flag, 64)
+        sp = db.trace_group_spots(64, seed=5, out=spots)
         loss, sd = lib.spot_loss(sp.reshape(len(inst) * 64, lib.SPOT_STRIDE))
         seeds.copy_(sd.reshape(len(inst), 64, 3))
         return loss
@@ -458,17 +456,14 @@ def bench_design(args):
     comp = [j for j, (s, f) in enumerate(zip(ps, pf)) if s == ccd and f in (S.S_DIST + 2, S.S_ANG, S.S_ANG + 1)]
     n, total = len(inst), int(sum(counts))
     db = lib.DeviceBatch(inst, counts)
-    b = dict(rays=torch.zeros((7, total), dtype=torch.float64, device="cuda"),
-             flag=torch.zeros(total, dtype=torch.int8, device="cuda"),
-             spots=torch.zeros((n, G, lib.SPOT_STRIDE), dtype=torch.float64, device="cuda"),
+    b = dict(spots=torch.zeros((n, G, lib.SPOT_STRIDE), dtype=torch.float64, device="cuda"),
              seeds=torch.zeros((n * G, 3), dtype=torch.float64, device="cuda"),
              loss=torch.zeros(1, dtype=torch.float64, device="cuda"),
              grad=torch.zeros((n, len(ps)), dtype=torch.float64, device="cuda"))
     lr = torch.zeros(len(ps), dtype=torch.float64, device="cuda")
 
     def step():
-        db.trace(seed=9, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, rays=b["rays"], flag=b["flag"])
-        db.group_spots(b["rays"], b["flag"], G, out=b["spots"])
+        db.trace_group_spots(G, seed=9, out=b["spots"])
         lib.spot_loss(b["spots"].view(n * G, lib.SPOT_STRIDE), seeds=b["seeds"], loss=b["loss"])
         db.adjoint(b["seeds"].view(n, G, 3), ps, pf, seed=9, grad=b["grad"])
         db.descend(ps, pf, b["grad"], lr)

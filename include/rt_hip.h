@@ -462,6 +462,18 @@ int rt_hip_adjoint(const rt_hip_scheme *scheme, const double *beam_desc, uint64_
 int rt_hip_group_spots(const double *X, const double *Y, const signed char *flag, size_t n, size_t n_groups,
                        double *spots, void *stream);
 
+/* The generated beam of `seed` (rays 0 .. n_rays - 1 of beam_desc) traced
+   through the scheme with the group spots of the rays alive at the end (ray
+   r in group r mod n_groups) formed in the trace itself: each workgroup tile
+   of rays adds its per-group moments, merged in a fixed order afterwards.
+   spots (device, n_groups x RT_SPOT_STRIDE) receives what
+   rt_hip_trace(RT_TRACE_GENERATE | RT_TRACE_STORE) followed by
+   rt_hip_group_spots gives, to rounding (another summation order), without
+   storing the rays; stats as for rt_hip_trace.  No counterpart in the
+   reference; it is the loss forward of the adjoint. */
+int rt_hip_trace_spots(const rt_hip_scheme *scheme, const double *beam_desc, uint64_t seed, size_t n_rays,
+                       size_t n_groups, double *spots, unsigned long long *stats, void *stream);
+
 /* rt_hip_group_spots without any allocation.  No counterpart in the
    reference: it is for callers that cannot allocate device memory (code
    under HIP graph capture, tight serving loops, harnesses that forbid
@@ -502,6 +514,15 @@ int rt_hip_batch_adjoint(const rt_hip_batch *batch, const double *beam_desc, uin
    instance's rays alone. */
 int rt_hip_batch_group_spots(const rt_hip_batch *batch, const double *X, const double *Y,
                              const signed char *flag, size_t n_groups, double *spots, void *stream);
+
+/* The batched trace of generated beams (rt_hip_batch_trace with
+   RT_TRACE_GENERATE) with the spots per (instance, group) of the rays alive
+   at the end formed in the trace itself (instance i's ray r in group
+   r mod n_groups): spots (device, n_inst x n_groups x RT_SPOT_STRIDE)
+   receives what RT_TRACE_STORE followed by rt_hip_batch_group_spots gives,
+   to rounding, without storing the rays.  No counterpart in the reference. */
+int rt_hip_batch_trace_group_spots(const rt_hip_batch *batch, const double *beam_desc, uint64_t seed,
+                                   size_t n_groups, double *spots, unsigned long long *stats, void *stream);
 
 /* One gradient-descent step on the device tables of every instance of a
    batch: field -= lr[j] * grad[i][j] for the parameters (param_surface,

@@ -65,6 +65,21 @@ RT_HD double dsqrt(double x) {
 #endif
 }
 
+// 1/sqrt(s) for the squared norm s of a direction: IEEE on the host; on the
+// device the form of the trace's normalization (rt::normalize_ray): for
+// s = 1 + d with |d| < 1e-8 on every lane of the wave 1 - d/2 (within 4e-17
+// of 1/sqrt(s)), else the refined reciprocal square root
+RT_HD double dinvnorm(double s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double d = s - 1.0;
+    const bool near = fabs(d) < 1e-8;
+    if (__all(near)) return fma(-0.5, d, 1.0);
+    return near ? fma(-0.5, d, 1.0) : rt::rsqrt(s);
+#else
+    return 1.0 / sqrt(s);
+#endif
+}
+
 // parameter slots of one surface
 enum ParamSlot {
     PG_C = 0,                          // curvature c
@@ -248,13 +263,13 @@ RT_HD void adj_transform(const Rec& d, St& r, St& a, double* gp) {
 
 // --------------------------------------------------------------- normalize
 RT_HD void fwd_normalize(St& r) {
-    const double in = drsqrt(r.cX * r.cX + r.cY * r.cY + r.cZ * r.cZ);
+    const double in = dinvnorm(r.cX * r.cX + r.cY * r.cY + r.cZ * r.cZ);
     r.cX *= in; r.cY *= in; r.cZ *= in;
 }
 
 // c: direction before normalizing; a: adjoint of the normalized direction (in/out)
 RT_HD void adj_normalize(double cX, double cY, double cZ, double& aX, double& aY, double& aZ) {
-    const double in = drsqrt(cX * cX + cY * cY + cZ * cZ);
+    const double in = dinvnorm(cX * cX + cY * cY + cZ * cZ);
     const double uX = cX * in, uY = cY * in, uZ = cZ * in;
     const double d = uX * aX + uY * aY + uZ * aZ;
     aX = (aX - uX * d) * in;
@@ -272,24 +287,40 @@ struct NewtonSeq {
     double s[kMaxNewtonAdj];        // s^(0) .. s^(n-1); the point is at s^(n-1)
 };
 
+// The point at parameter s along the ray, the profile there, and F and
+// dF/ds of the iterative intersections
+struct NewtonAt {
+    double X, Y, Z, F, G;
+    Conic k;
+};
+
+template <class Rec>
+RT_HD NewtonAt newton_at(const Rec& d, int kind, const St& in, double s0, double s) {
+    const auto p = d.p;
+    NewtonAt o;
+    const double u = s + s0;
+    o.X = in.X + u * in.cX; o.Y = in.Y + u * in.cY; o.Z = s * in.cZ;
+    if (kind == AK_CONIC_NEWTON) {
+        o.k = conic_at(d, o.X * o.X + o.Y * o.Y);
+        o.F = o.Z - o.k.S - o.k.pl.P;
+        o.G = -o.k.E * o.X * in.cX - o.k.E * o.Y * in.cY + in.cZ;
+    } else {
+        const double cR = p[RT_S_CR];
+        o.k = conic_at(d, o.Y * o.Y);
+        const double Fy = o.k.S + o.k.pl.P;
+        const double dFy = o.k.E * o.Y;
+        o.F = o.Z - Fy - 0.5 * cR * (o.X * o.X + o.Z * o.Z - Fy * Fy);
+        o.G = -cR * o.X * in.cX + (cR * Fy - 1.0) * dFy * in.cY + (1.0 - cR * o.Z) * in.cZ;
+    }
+    return o;
+}
+
 // F and dF/ds of the iterative intersections at parameter s along the ray
 template <class Rec>
 RT_HD void newton_FG(const Rec& d, int kind, const St& in, double s0, double s, double& F, double& G) {
-    const auto p = d.p;
-    const double u = s + s0;
-    const double X = in.X + u * in.cX, Y = in.Y + u * in.cY, Z = s * in.cZ;
-    if (kind == AK_CONIC_NEWTON) {
-        const Conic k = conic_at(d, X * X + Y * Y);
-        F = Z - k.S - k.pl.P;
-        G = -k.E * X * in.cX - k.E * Y * in.cY + in.cZ;
-    } else {
-        const double cR = p[RT_S_CR];
-        const Conic k = conic_at(d, Y * Y);
-        const double Fy = k.S + k.pl.P;
-        const double dFy = k.E * Y;
-        F = Z - Fy - 0.5 * cR * (X * X + Z * Z - Fy * Fy);
-        G = -cR * X * in.cX + (cR * Fy - 1.0) * dFy * in.cY + (1.0 - cR * Z) * in.cZ;
-    }
+    const NewtonAt o = newton_at(d, kind, in, s0, s);
+    F = o.F;
+    G = o.G;
 }
 
 // forward intersection from `in`; n_fwd > 0 replays that many Newton
@@ -364,18 +395,17 @@ RT_HD void point_adj(const St& in, double s0, double s, double aX, double aY, do
     ain.cZ += aZ * s;
 }
 
-// adjoint of F, G (newton_FG) at parameter s: point adjoint out (aX, aY, aZ),
-// direct direction adjoints into ain, parameters into gp
+// adjoint of F, G (newton_at, evaluated as `o`): point adjoint out
+// (aX, aY, aZ), direct direction adjoints into ain, parameters into gp
 template <class Rec>
-RT_HD void newton_FG_adj(const Rec& d, int kind, const St& in, double s0, double s, double aF, double aG,
+RT_HD void newton_FG_adj(const Rec& d, int kind, const St& in, const NewtonAt& o, double aF, double aG,
                          double& aX, double& aY, double& aZ, St& ain, double* gp) {
     const auto p = d.p;
-    const double u = s + s0;
-    const double X = in.X + u * in.cX, Y = in.Y + u * in.cY, Z = s * in.cZ;
+    const double X = o.X, Y = o.Y, Z = o.Z;
+    const Conic& k = o.k;
     aX = aY = aZ = 0.0;
     if (kind == AK_CONIC_NEWTON) {
         const double r2 = X * X + Y * Y;
-        const Conic k = conic_at(d, r2);
         const double W = X * in.cX + Y * in.cY;
         // G = -E W + cZ
         const double aE = -aG * W, aW = -aG * k.E;
@@ -390,7 +420,6 @@ RT_HD void newton_FG_adj(const Rec& d, int kind, const St& in, double s0, double
     } else {
         const double cR = p[RT_S_CR];
         const double Y2 = Y * Y;
-        const Conic k = conic_at(d, Y2);
         const double Fy = k.S + k.pl.P, dFy = k.E * Y;
         // G = -cR X cX + (cR Fy - 1) dFy cY + (1 - cR Z) cZ
         aX += -aG * cR * in.cX; ain.cX += -aG * cR * X;
@@ -447,12 +476,11 @@ RT_UNROLL
                 point_adj(in, q.s0, q.s[k], a.X, a.Y, a.Z, ain, as, as0);
             } else if (k < n - 1) {
                 // s^(k+1) = s^(k) - F/G at s^(k)
-                double F, G;
-                newton_FG(d, kind, in, q.s0, q.s[k], F, G);
-                const double iG = drcp(G);
-                const double aF = -as * iG, aG = as * F * iG * iG;
+                const NewtonAt o = newton_at(d, kind, in, q.s0, q.s[k]);
+                const double iG = drcp(o.G);
+                const double aF = -as * iG, aG = as * o.F * iG * iG;
                 double pX, pY, pZ;
-                newton_FG_adj(d, kind, in, q.s0, q.s[k], aF, aG, pX, pY, pZ, ain, gp);
+                newton_FG_adj(d, kind, in, o, aF, aG, pX, pY, pZ, ain, gp);
                 point_adj(in, q.s0, q.s[k], pX, pY, pZ, ain, as, as0);   // as keeps the direct term
             }
         }
@@ -473,30 +501,39 @@ struct Normal {
     double x, y, z;
 };
 
+// The normal at a point with the profile it was formed from (evaluated once
+// for the action, its adjoint and the normal's adjoint)
+struct Geom {
+    Normal N;
+    Conic k;
+};
+
 template <class Rec>
-RT_HD Normal normal_at(const Rec& d, double X, double Y, double Z) {
+RT_HD Geom geom_at(const Rec& d, double X, double Y, double Z) {
     const auto p = d.p;
     const int cls = d.cls();
+    Geom g;
     if (cls == RT_CONIC) {
-        const Conic k = conic_at(d, X * X + Y * Y);
-        return Normal{-k.E * X, -k.E * Y, 1.0};
-    }
-    if (cls == RT_TORIC) {
+        g.k = conic_at(d, X * X + Y * Y);
+        g.N = Normal{-g.k.E * X, -g.k.E * Y, 1.0};
+    } else if (cls == RT_TORIC) {
         const double cR = p[RT_S_CR];
-        const Conic k = conic_at(d, Y * Y);
-        const double Fy = k.S + k.pl.P, dFy = k.E * Y;
-        return Normal{-cR * X, (cR * Fy - 1.0) * dFy, 1.0 - cR * Z};
+        g.k = conic_at(d, Y * Y);
+        const double Fy = g.k.S + g.k.pl.P, dFy = g.k.E * Y;
+        g.N = Normal{-cR * X, (cR * Fy - 1.0) * dFy, 1.0 - cR * Z};
+    } else {
+        g.N = Normal{0.0, 0.0, 1.0};
     }
-    return Normal{0.0, 0.0, 1.0};
+    return g;
 }
 
 template <class Rec>
-RT_HD void normal_adj(const Rec& d, double X, double Y, double Z, const Normal& aN, St& apt, double* gp) {
+RT_HD void normal_adj(const Rec& d, double X, double Y, double Z, const Conic& k, const Normal& aN, St& apt,
+                      double* gp) {
     const auto p = d.p;
     const int cls = d.cls();
     if (cls == RT_CONIC) {
         const double r2 = X * X + Y * Y;
-        const Conic k = conic_at(d, r2);
         const double aE = -(aN.x * X + aN.y * Y);
         apt.X += -k.E * aN.x;
         apt.Y += -k.E * aN.y;
@@ -506,7 +543,6 @@ RT_HD void normal_adj(const Rec& d, double X, double Y, double Z, const Normal& 
     } else if (cls == RT_TORIC) {
         const double cR = p[RT_S_CR];
         const double Y2 = Y * Y;
-        const Conic k = conic_at(d, Y2);
         const double Fy = k.S + k.pl.P, dFy = k.E * Y;
         apt.X += -cR * aN.x;
         apt.Z += -cR * aN.z;
@@ -520,13 +556,15 @@ RT_HD void normal_adj(const Rec& d, double X, double Y, double Z, const Normal& 
 // ------------------------------------------------------------------ actions
 // The action's output direction (before the normalisation that follows);
 // false for total internal reflection or an evanescent order.
+// (g: geom_at of the point, read for curved mirrors and lenses)
 template <class Rec>
-RT_HD bool fwd_action(const Rec& d, const St& r, double mu, double L, double& oX, double& oY, double& oZ) {
+RT_HD bool fwd_action(const Rec& d, const St& r, const Geom& g, double mu, double L, double& oX, double& oY,
+                      double& oZ) {
     const int type = d.type(), cls = d.cls();
     oX = r.cX; oY = r.cY; oZ = r.cZ;
     if (type == RT_MIRROR) {
         if (cls == RT_PLANE) { oZ = -r.cZ; return true; }
-        const Normal N = normal_at(d, r.X, r.Y, r.Z);
+        const Normal N = g.N;
         const double a2 = 2.0 * (r.cX * N.x + r.cY * N.y + r.cZ * N.z) * drcp(N.x * N.x + N.y * N.y + N.z * N.z);
         oX = r.cX - a2 * N.x; oY = r.cY - a2 * N.y; oZ = r.cZ - a2 * N.z;
         return true;
@@ -539,7 +577,7 @@ RT_HD bool fwd_action(const Rec& d, const St& r, double mu, double L, double& oX
             oX = mu * r.cX; oY = mu * r.cY; oZ = mu * r.cZ + G;
             return true;
         }
-        const Normal N = normal_at(d, r.X, r.Y, r.Z);
+        const Normal N = g.N;
         const double iM = drcp(N.x * N.x + N.y * N.y + N.z * N.z);
         const double a = mu * (r.cX * N.x + r.cY * N.y + r.cZ * N.z) * iM, b = (mu * mu - 1.0) * iM;
         if (b > a * a) return false;
@@ -569,13 +607,13 @@ RT_HD bool fwd_action(const Rec& d, const St& r, double mu, double L, double& oX
 // action's output direction; a: receives the adjoint of r (its point part is
 // added to, its direction part set); aL: adjoint of L
 template <class Rec>
-RT_HD bool adj_action(const Rec& d, const St& r, double mu, double L, double aX, double aY, double aZ, St& a,
-                      double& aL, double* gp) {
+RT_HD bool adj_action(const Rec& d, const St& r, const Geom& g, double mu, double L, double aX, double aY,
+                      double aZ, St& a, double& aL, double* gp) {
     const int type = d.type(), cls = d.cls();
     aL = 0.0;
     if (type == RT_MIRROR) {
         if (cls == RT_PLANE) { a.cX = aX; a.cY = aY; a.cZ = -aZ; return true; }
-        const Normal N = normal_at(d, r.X, r.Y, r.Z);
+        const Normal N = g.N;
         const double D = r.cX * N.x + r.cY * N.y + r.cZ * N.z, M = N.x * N.x + N.y * N.y + N.z * N.z;
         const double aoN = aX * N.x + aY * N.y + aZ * N.z;
         const double iM = drcp(M), q2 = -2.0 * D * iM;
@@ -583,7 +621,7 @@ RT_HD bool adj_action(const Rec& d, const St& r, double mu, double L, double aX,
         Normal aN = {q2 * aX + aD * r.cX + 2.0 * aM * N.x, q2 * aY + aD * r.cY + 2.0 * aM * N.y,
                      q2 * aZ + aD * r.cZ + 2.0 * aM * N.z};
         a.cX = aX + aD * N.x; a.cY = aY + aD * N.y; a.cZ = aZ + aD * N.z;
-        normal_adj(d, r.X, r.Y, r.Z, aN, a, gp);
+        normal_adj(d, r.X, r.Y, r.Z, g.k, aN, a, gp);
         return true;
     }
     if (type == RT_LENS) {
@@ -593,7 +631,7 @@ RT_HD bool adj_action(const Rec& d, const St& r, double mu, double L, double aX,
             a.cX = mu * aX; a.cY = mu * aY; a.cZ = mu * aZ + aa * mu;
             return true;
         }
-        const Normal N = normal_at(d, r.X, r.Y, r.Z);
+        const Normal N = g.N;
         const double M = N.x * N.x + N.y * N.y + N.z * N.z;
         const double D = r.cX * N.x + r.cY * N.y + r.cZ * N.z;
         const double iM = drcp(M);
@@ -605,7 +643,7 @@ RT_HD bool adj_action(const Rec& d, const St& r, double mu, double L, double aX,
         Normal aN = {G * aX + aD * r.cX + 2.0 * aM * N.x, G * aY + aD * r.cY + 2.0 * aM * N.y,
                      G * aZ + aD * r.cZ + 2.0 * aM * N.z};
         a.cX = mu * aX + aD * N.x; a.cY = mu * aY + aD * N.y; a.cZ = mu * aZ + aD * N.z;
-        normal_adj(d, r.X, r.Y, r.Z, aN, a, gp);
+        normal_adj(d, r.X, r.Y, r.Z, g.k, aN, a, gp);
         return true;
     }
     if (type == RT_GRATING) {
@@ -654,14 +692,17 @@ RT_HD bool surface_adjoint(const Rec& d, const St& entry, int n_fwd, double mu, 
     St sI;
     NewtonSeq q;
     if (!fwd_intersect(d, sN, n_fwd, sI, q)) return false;
+    // the profile at the intersection, for curved mirrors and lenses
+    Geom g{};
+    if ((type == RT_MIRROR || type == RT_LENS) && d.cls() != RT_PLANE) g = geom_at(d, sI.X, sI.Y, sI.Z);
     double oX, oY, oZ;
-    if (!fwd_action(d, sI, mu, L, oX, oY, oZ)) return false;
+    if (!fwd_action(d, sI, g, mu, L, oX, oY, oZ)) return false;
     // normalisation after the action
     adj_normalize(oX, oY, oZ, a.cX, a.cY, a.cZ);
     // action: point adjoint accumulates, direction adjoint replaced
     St aI = {a.X, a.Y, a.Z, 0.0, 0.0, 0.0};
     double aL = 0.0;
-    if (!adj_action(d, sI, mu, L, a.cX, a.cY, a.cZ, aI, aL, gp)) return false;
+    if (!adj_action(d, sI, g, mu, L, a.cX, a.cY, a.cZ, aI, aL, gp)) return false;
     if (type == RT_GRATING && L != 0.0) gp[PG_D] += aL * (-L * drcp(p[RT_S_GRCONST]));
     // intersection, first normalisation, transform
     adj_intersect(d, sN, sI, q, aI, gp);
@@ -785,14 +826,15 @@ __device__ __forceinline__ LaneRows lane_rows(double* base, uint64_t rows, uint6
 // forward through one surface: tape of the entry state (rows k0 .. k0+6),
 // the trace's surface step, the Newton count
 // (lv, col: the ray's per-wavelength table values, as surface_step takes them)
-template <class SD>
+// (DERIV: lv holds the derived table's columns, as surface_step takes them)
+template <bool DERIV = false, class SD>
 __device__ __forceinline__ void fwd_step(const SD& d, int s, Ray& r, bool& alive, TraceCtx& c, const LaneRows& t,
                                          int k0, const double* lv = nullptr, int col = -1) {
     if (alive) {
         t.st(k0 + 0, r.X); t.st(k0 + 1, r.Y); t.st(k0 + 2, r.Z);
         t.st(k0 + 3, r.cX); t.st(k0 + 4, r.cY); t.st(k0 + 5, r.cZ);
     }
-    surface_step(d, s, r, alive, c, lv, col);
+    surface_step<SD, 0, DERIV>(d, s, r, alive, c, lv, col);
     // the Newton count (row 6) only where the intersection iterates
     if ((d.ikind() == IK_CONIC_NEWTON || d.ikind() == IK_TORIC_NEWTON) && alive) t.st(k0 + 6, (double)c.last_it);
 }
@@ -827,6 +869,23 @@ RT_UNROLL
         const double in2 = rcp(n2);
         mu = n1 * in2;
         if (d.type() == RT_GRATING) L = grating_order(d, r.lam) * r.lam * in2 * p[D_INV_D];
+    }
+    return surface_adjoint(d, entry, nit, mu, L, ad, gp);
+}
+
+// bwd_step for kernels reading the derived table: mu = lv[col] and, for a
+// grating, L = lv[col + 1], the values the forward step read
+template <class SD>
+__device__ __forceinline__ bool bwd_step_d(const SD& d, const Ray& r, St& ad, const TraceCtx& c, const LaneRows& t,
+                                           int k0, double* gp, const double* lv, int col) {
+RT_UNROLL
+    for (int k = 0; k < PG_N; ++k) gp[k] = 0.0;
+    const St entry = {t.ld(k0 + 0), t.ld(k0 + 1), t.ld(k0 + 2), t.ld(k0 + 3), t.ld(k0 + 4), t.ld(k0 + 5)};
+    const int nit = (d.ikind() == IK_CONIC_NEWTON || d.ikind() == IK_TORIC_NEWTON) ? (int)t.ld(k0 + 6) : 0;
+    double mu = 0.0, L = 0.0;
+    if (d.type() == RT_LENS || d.type() == RT_GRATING) {
+        mu = lv[col];
+        if (d.type() == RT_GRATING) L = lv[col + 1];
     }
     return surface_adjoint(d, entry, nit, mu, L, ad, gp);
 }

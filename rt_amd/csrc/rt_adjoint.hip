@@ -324,7 +324,7 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
 
     // the kernel generated for this structure and parameter list, else the generic one
     // (grid-wavelength beams: the generated kernel reads the per-wavelength table)
-    const uint64_t nk = rt_scheme_ltab_nk(sch, beam_desc);
+    const uint64_t nk = rt_scheme_dtab_nk(sch, beam_desc);
     hipFunction_t fn = nullptr;
     if (rt_jit_enabled())
         fn = rt_jit_adjoint_kernel(rt_scheme_signature(sch), beam_type, host, n_surf, kAdjBlock, adjoint_waves(),
@@ -357,7 +357,7 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     a.tape = B.tape;
     a.gacc = B.gacc;
     if (fn && nk) {
-        rc = rt_scheme_ltab_fill(sch, beam_desc, nk, st, &a.ltab);
+        rc = rt_scheme_dtab_fill(sch, beam_desc, nk, st, &a.ltab);
         if (rc != RT_OK) return rc;
         a.lt_nk = nk;
     }

@@ -916,6 +916,12 @@ struct TraceArgs {
     // encoded, combined with atomic min / max)
     unsigned char* loss;
     unsigned long long* ext;
+    // group spot partials (kernels built with kRecSpots, and batched traces
+    // when set): per tile of blockDim.x rays and group (ray r in group
+    // r mod n_groups; r counted from the instance's first ray in a batch),
+    // group_part[(tile * n_groups + g) * 5 ..] = n, mean x, mean y, M2 x, M2 y
+    double* group_part;
+    uint64_t n_groups;
 };
 constexpr int kBeamRec = (int)(sizeof(BeamArg) / sizeof(double));
 // copies of the statistics counters the trace kernels add to (TraceArgs::stats
@@ -948,6 +954,9 @@ __device__ __forceinline__ void stats_fold_body(unsigned long long* copies, unsi
 // the mask, of cZ after the action's normalisation, over the rays alive there.
 constexpr int kRecLoss = 1;
 constexpr int kRecExt = 2;
+// kRecSpots: the trace walks its rays a workgroup tile at a time and writes
+// each tile's group spot partials (TraceArgs::group_part)
+constexpr int kRecSpots = 4;
 constexpr unsigned char kLossDead = 255;
 constexpr int kExtPerSurf = 12;
 
@@ -1312,6 +1321,39 @@ __device__ __forceinline__ void dtab_body(const DtabArgs& a) {
     }
 }
 
+// Group spot partials of one tile of blockDim.x consecutive rays, the first
+// of index r0 (ray r in group r mod G): per group, the moments (n, mean x,
+// mean y, M2 x, M2 y) of the tile's rays alive at the end, in ray order
+// (Welford's update, as rt_group_moments_kernel forms them), written to
+// part[(tile * G + g) * 5 ..].  Called by every thread of the workgroup.
+__device__ __forceinline__ void tile_group_moments(double* part, uint64_t tile, uint64_t r0, uint64_t G, bool alive,
+                                                   double x, double y) {
+    __shared__ double s_gx[kBlock], s_gy[kBlock];
+    __shared__ unsigned char s_ga[kBlock];
+    const int B = blockDim.x;                         // <= kBlock
+    __syncthreads();                                  // the previous tile's readers are done
+    s_gx[threadIdx.x] = x;
+    s_gy[threadIdx.x] = y;
+    s_ga[threadIdx.x] = alive ? 1 : 0;
+    __syncthreads();
+    const uint64_t shift = r0 % G;
+    for (uint64_t g = threadIdx.x; g < G; g += (uint64_t)B) {
+        double n = 0.0, mx = 0.0, my = 0.0, qx = 0.0, qy = 0.0;
+        for (uint64_t q = (g + G - shift) % G; q < (uint64_t)B; q += G) {
+            if (!s_ga[q]) continue;
+            const double vx = s_gx[q], vy = s_gy[q];
+            n += 1.0;
+            const double dx = vx - mx, dy = vy - my;
+            mx += dx / n;
+            my += dy / n;
+            qx += dx * (vx - mx);
+            qy += dy * (vy - my);
+        }
+        double* o = part + (tile * G + g) * 5;
+        o[0] = n; o[1] = mx; o[2] = my; o[3] = qx; o[4] = qy;
+    }
+}
+
 // Kernel body: the spline pool goes to LDS, every lane of a persistent grid
 // walks rays in a grid-stride loop (generated in-kernel when BEAM >= 0,
 // loaded otherwise), `surfaces` applies the scheme, counters are flushed per
@@ -1345,9 +1387,12 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
     // Recording kernels walk the rays a wave at a time: every lane stays in
     // the loop while the wave's first lane has a ray, a lane past the end
     // carrying a dead ray, so the extents' wave reductions read live lanes
-    // only.  Other kernels let each lane leave after its own last ray.
+    // only; spot kernels likewise a workgroup at a time (the tile's partials
+    // are formed by the whole workgroup).  Other kernels let each lane leave
+    // after its own last ray.
+    constexpr bool kTiles = (MODE & kRecSpots) != 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-         MODE == 0 ? i < a.n : i - __lane_id() < a.n; i += stride) {
+         MODE == 0 ? i < a.n : (kTiles ? i - threadIdx.x : i - __lane_id()) < a.n; i += stride) {
         const bool valid = MODE == 0 || i < a.n;
         // Surface parameters stay per-iteration scalar loads: the table
         // pointer is passed through an empty asm so the loads cannot be
@@ -1399,6 +1444,9 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
             atomicAdd(&s_stats[RT_ST_RAYS], (unsigned)__popcll(all));
             if (good) atomicAdd(&s_stats[RT_ST_GOOD], (unsigned)__popcll(good));
         }
+        if constexpr (kTiles)
+            tile_group_moments(kernel_args()->group_part, (i - threadIdx.x) / blockDim.x,
+                               a.ray_begin + (i - threadIdx.x), a.n_groups, alive, r.X, r.Y);
     }
     lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
     lds_add(&s_stats[RT_ST_CCD_HITS], c.n_hit);
@@ -1562,17 +1610,23 @@ __device__ __forceinline__ void trace_batch_body(const TraceArgs& a, Surfaces su
             if (all) atomicAdd(&s_stats[RT_ST_RAYS], (unsigned)__popcll(all));
             if (good) atomicAdd(&s_stats[RT_ST_GOOD], (unsigned)__popcll(good));
         }
-        // spot moments of the tile: count and means, then squared deviations
-        double m1[3] = {alive ? 1.0 : 0.0, alive ? ray.X : 0.0, alive ? ray.Y : 0.0};
-        block_sum<3>(m1, s_red);
-        const double cnt = m1[0];
-        const double mx = cnt > 0.0 ? m1[1] / cnt : 0.0, my = cnt > 0.0 ? m1[2] / cnt : 0.0;
-        double m2[2] = {alive ? (ray.X - mx) * (ray.X - mx) : 0.0, alive ? (ray.Y - my) * (ray.Y - my) : 0.0};
-        block_sum<2>(m2, s_red);
-        if (threadIdx.x == 0 && a.tile_moments) {
-            double* o = a.tile_moments + t * 5;
-            o[0] = cnt; o[1] = mx; o[2] = my; o[3] = m2[0]; o[4] = m2[1];
+        // spot moments of the tile (asked for): count and means, then
+        // squared deviations
+        if (a.tile_moments) {
+            double m1[3] = {alive ? 1.0 : 0.0, alive ? ray.X : 0.0, alive ? ray.Y : 0.0};
+            block_sum<3>(m1, s_red);
+            const double cnt = m1[0];
+            const double mx = cnt > 0.0 ? m1[1] / cnt : 0.0, my = cnt > 0.0 ? m1[2] / cnt : 0.0;
+            double m2[2] = {alive ? (ray.X - mx) * (ray.X - mx) : 0.0, alive ? (ray.Y - my) * (ray.Y - my) : 0.0};
+            block_sum<2>(m2, s_red);
+            if (threadIdx.x == 0) {
+                double* o = a.tile_moments + t * 5;
+                o[0] = cnt; o[1] = mx; o[2] = my; o[3] = m2[0]; o[4] = m2[1];
+            }
         }
+        // group spot partials of the tile (asked for)
+        if (a.group_part)
+            tile_group_moments(a.group_part, t, (t - ts[inst]) * blockDim.x, a.n_groups, alive, ray.X, ray.Y);
     }
     lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
     __syncthreads();

@@ -27,7 +27,7 @@ int rt_set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2
 // order of one stream.
 struct WorkBufs {
     enum Slot { TAPE, GACC, PART, PARAMS, SPOT_PART, SPOT_CS, DESCEND, EXT_KEYS, SAVE_HIST, SAVE_LPOS, SAVE_TILES,
-                SAVE_GPOS, N_SLOTS };
+                SAVE_GPOS, TRACE_PART, N_SLOTS };
     void* p[N_SLOTS] = {};
     size_t have[N_SLOTS] = {};
     std::vector<void*> owned;
@@ -91,6 +91,12 @@ WorkBufs* rt_batch_work(const rt_hip_batch* b);
 // (*tab: the device table, column q at q * nk).
 uint64_t rt_scheme_ltab_nk(const rt_hip_scheme* s, const double* beam_desc);
 int rt_scheme_ltab_fill(const rt_hip_scheme* s, const double* beam_desc, uint64_t nk, hipStream_t stream,
+                        const double** tab);
+// The derived per-wavelength table of a scheme (mu, grating term, flux
+// weight; the layout of rt_jit_dtab_layout), likewise; kept while the grid
+// and the stream stay those of its last fill.
+uint64_t rt_scheme_dtab_nk(const rt_hip_scheme* s, const double* beam_desc);
+int rt_scheme_dtab_fill(const rt_hip_scheme* s, const double* beam_desc, uint64_t nk, hipStream_t stream,
                         const double** tab);
 uint64_t rt_batch_ltab_nk(const rt_hip_batch* b, const double* beam_desc);
 int rt_batch_ltab_fill(const rt_hip_batch* b, const double* beam_desc, uint64_t nk, hipStream_t stream,

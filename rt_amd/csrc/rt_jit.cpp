@@ -188,11 +188,13 @@ extern "C" int rt_hip_set_jit(int mode) {
 }
 
 // variant: RT_JIT_VAR_DUMP (per-surface state dump), RT_JIT_VAR_LOSS and
-// RT_JIT_VAR_EXT (the recording modes rt::kRecLoss, rt::kRecExt)
+// RT_JIT_VAR_EXT (the recording modes rt::kRecLoss, rt::kRecExt),
+// RT_JIT_VAR_SPOTS (group spot partials per tile, rt::kRecSpots)
 static std::string generate_source(const std::string& sig, int beam_mode, int variant, int block, int waves,
                                    bool batch = false) {
     const bool dump = (variant & RT_JIT_VAR_DUMP) != 0;
-    const int rec = ((variant & RT_JIT_VAR_LOSS) ? rt::kRecLoss : 0) | ((variant & RT_JIT_VAR_EXT) ? rt::kRecExt : 0);
+    const int rec = ((variant & RT_JIT_VAR_LOSS) ? rt::kRecLoss : 0) | ((variant & RT_JIT_VAR_EXT) ? rt::kRecExt : 0) |
+                    ((variant & RT_JIT_VAR_SPOTS) ? rt::kRecSpots : 0);
     // RT_JIT_LTAB: per-wavelength values come from a table (TraceArgs::ltab),
     // each column loaded at the surface that reads it: for a single scheme the
     // derived table (mu, the grating term, the flux weight), for a batch the
@@ -260,15 +262,20 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
         types.push_back(sig.substr(pos, end - pos));
         pos = end + 1;
     }
-    // per-wavelength table (RT_JIT_LTAB): the forward steps read their columns
-    // as the trace does; the backward steps read n1 and n2 from the columns
-    // the forward read them from (a reused n1 is the previous non-auxiliary
-    // surface's n2 column)
+    // per-wavelength table (RT_JIT_LTAB).  One scheme: the derived table, whose
+    // mu and grating columns the forward steps read as the trace does and the
+    // backward steps read again (the detector's flux weight is not read: no
+    // flux in the adjoint).  A batch: the spline-valued functions' columns,
+    // read by the forward steps as the trace does; the backward steps read n1
+    // and n2 from the columns the forward read them from (a reused n1 is the
+    // previous non-auxiliary surface's n2 column)
+    const bool derived = ltab && !batch;
     std::vector<int> col_base, col_surf, col_role;
-    rt_jit_ltab_layout(sig, col_base, col_surf, col_role);
+    if (derived) rt_jit_dtab_layout(sig, col_base, col_surf, col_role);
+    else rt_jit_ltab_layout(sig, col_base, col_surf, col_role);
     const bool use_tab = ltab && !col_surf.empty();
     std::vector<int> n1col(n_surf, -1), n2col(n_surf, -1);
-    {
+    if (!derived) {
         int prev_n2 = -1;
         for (int s = 0; s < n_surf; ++s) {
             const int type = sig_arg(types[s], 1), fnmask = sig_arg(types[s], 7);
@@ -328,12 +335,12 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
         bool tab_s = false;
         if (use_tab)
             for (size_t q = 0; q < nc; ++q)
-                if (col_surf[q] == s && col_role[q] <= 1) {
+                if (col_surf[q] == s && (derived ? col_role[q] != RT_DTAB_W : col_role[q] <= 1)) {
                     os << "      rt::adj::load_lval_adj(lv, c, " << q << ");\n";
                     tab_s = true;
                 }
-        os << "      rt::adj::fwd_step(" << types[s] << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s
-           << ", r, alive, c, t, " << s * rt::adj::kTape;
+        os << "      rt::adj::fwd_step" << (derived && use_tab ? "<true>" : "") << "(" << types[s] << "{c.table + "
+           << s << " * rt::DEV_STRIDE}, " << s << ", r, alive, c, t, " << s * rt::adj::kTape;
         if (tab_s) os << ", lv, " << col_base[(size_t)s];
         os << ");\n";
     }
@@ -342,6 +349,31 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
           "auto& g, bool ok) -> bool {\n";
     if (use_tab) os << "      double lb[" << nc << "];\n";
     for (int s = n_surf - 1; s >= 0; --s) {
+        if (derived && use_tab) {
+            // the surface's mu (and grating) columns, loaded again for its backward step
+            bool tab_s = false;
+            os << "      {";
+            for (size_t q = 0; q < nc; ++q)
+                if (col_surf[q] == s && col_role[q] != RT_DTAB_W) {
+                    os << " rt::adj::load_lval_adj(lb, c, " << q << ");";
+                    tab_s = true;
+                }
+            os << " double gp[rt::adj::PG_N]; const bool use = rt::adj::bwd_step_d(" << types[s] << "{c.table + " << s
+               << " * rt::DEV_STRIDE}, r, ad, c, t, " << s * rt::adj::kTape << ", gp, lb, "
+               << (tab_s ? col_base[(size_t)s] : -1) << ") && ok;";
+            for (size_t gi = 0; gi < groups.size(); ++gi) {
+                if (groups[gi].surf != s) continue;
+                os << " g.template add4<" << gi << ">(";
+                for (int k = 0; k < 4; ++k) {
+                    if (k) os << ", ";
+                    if (groups[gi].sl[k] >= 0) os << "use ? gp[" << groups[gi].sl[k] << "] : 0.0";
+                    else os << "0.0";
+                }
+                os << ");";
+            }
+            os << " ok = use; }\n";
+            continue;
+        }
         const bool tab_s = use_tab && (n1col[(size_t)s] >= 0 || n2col[(size_t)s] >= 0);
         os << "      {";
         if (tab_s) {

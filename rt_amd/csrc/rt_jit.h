@@ -39,7 +39,7 @@ bool rt_jit_ltab_enabled();
 // when the runtime compiler is unavailable or fails (the caller then
 // launches the generic kernel, which reads the structure from the records).
 // kernel variants of rt_jit_kernel (bit mask)
-enum { RT_JIT_VAR_DUMP = 1, RT_JIT_VAR_LOSS = 2, RT_JIT_VAR_EXT = 4 };
+enum { RT_JIT_VAR_DUMP = 1, RT_JIT_VAR_LOSS = 2, RT_JIT_VAR_EXT = 4, RT_JIT_VAR_SPOTS = 8 };
 
 hipFunction_t rt_jit_kernel(const std::string& signature, int beam_mode, int variant, int block, int waves,
                             int device, bool batch = false);

@@ -862,7 +862,7 @@ int trace_launch(const rt_hip_scheme* sch, const double* beam_desc, uint64_t see
                  double* X, double* Y, double* Z, double* cX, double* cY, double* cZ, double* lambda,
                  signed char* flag, int flags, unsigned int* ccd_hist, double* ccd_flux, unsigned long long* stats,
                  double* dump, signed char* dump_flag, int rec, unsigned char* loss, unsigned long long* ext_keys,
-                 void* stream) {
+                 void* stream, double* group_part = nullptr, uint64_t n_groups = 0, int* block_used = nullptr) {
     if (!sch) return set_error(RT_ERR_BAD_VALUE, "scheme is NULL");
     const bool gen = (flags & RT_TRACE_GENERATE) != 0;
     if (gen) {
@@ -898,6 +898,8 @@ int trace_launch(const rt_hip_scheme* sch, const double* beam_desc, uint64_t see
     a.stats = stats ? sch->stat_copies : nullptr;
     a.loss = loss;
     a.ext = ext_keys;
+    a.group_part = group_part;
+    a.n_groups = n_groups;
 
     const int cus = rt_device_cus();
     size_t lds = sch->pool_len * sizeof(double);
@@ -909,7 +911,8 @@ int trace_launch(const rt_hip_scheme* sch, const double* beam_desc, uint64_t see
     if (rt_jit_enabled() && sch->unroll)
         fn = rt_jit_kernel(sch->signature, beam_mode | (nk ? RT_JIT_LTAB : 0),
                            (dump ? RT_JIT_VAR_DUMP : 0) | ((rec & rt::kRecLoss) ? RT_JIT_VAR_LOSS : 0) |
-                               ((rec & rt::kRecExt) ? RT_JIT_VAR_EXT : 0),
+                               ((rec & rt::kRecExt) ? RT_JIT_VAR_EXT : 0) |
+                               ((rec & rt::kRecSpots) ? RT_JIT_VAR_SPOTS : 0),
                            sch->block, sch->waves, sch->device);
     if (fn) {
         if (nk) {
@@ -931,6 +934,7 @@ int trace_launch(const rt_hip_scheme* sch, const double* beam_desc, uint64_t see
         HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, sch->block, 1, 1, (unsigned)lds, st, params, nullptr),
                 RT_ERR_FAILED);
         g_last_kernel = "generated";
+        if (block_used) *block_used = sch->block;
         if (getenv("RT_HIP_VERBOSE"))
             fprintf(stderr, "rt_hip: trace generated kernel %p, %zu rays, block %d, waves %d, grid %u, table %llu, "
                             "record %d\n", (void*)fn, n_rays, sch->block, sch->waves, grid, (unsigned long long)nk,
@@ -939,17 +943,21 @@ int trace_launch(const rt_hip_scheme* sch, const double* beam_desc, uint64_t see
         uint64_t want = (n_rays + kBlock - 1) / kBlock;
         uint64_t cap = (uint64_t)cus * 8;      // persistent grid: 8 blocks (32 waves) per CU
         unsigned grid = (unsigned)(want < cap ? want : cap);
-        constexpr int R = rt::kRecLoss | rt::kRecExt;
+        constexpr int R = rt::kRecLoss | rt::kRecExt, SP = rt::kRecSpots;
         auto kern = rec == 0 ? (beam_mode == RT_BEAM_PARALLEL ? rt_trace_kernel<RT_BEAM_PARALLEL>
                                 : beam_mode == RT_BEAM_CONIC  ? rt_trace_kernel<RT_BEAM_CONIC>
                                 : beam_mode == RT_BEAM_GAUSS  ? rt_trace_kernel<RT_BEAM_GAUSS>
                                                               : rt_trace_kernel<kLoad>)
+                  : rec == SP ? (beam_mode == RT_BEAM_PARALLEL ? rt_trace_kernel<RT_BEAM_PARALLEL, SP>
+                                 : beam_mode == RT_BEAM_CONIC  ? rt_trace_kernel<RT_BEAM_CONIC, SP>
+                                                               : rt_trace_kernel<RT_BEAM_GAUSS, SP>)
                              : (beam_mode == RT_BEAM_PARALLEL ? rt_trace_kernel<RT_BEAM_PARALLEL, R>
                                 : beam_mode == RT_BEAM_CONIC  ? rt_trace_kernel<RT_BEAM_CONIC, R>
                                 : beam_mode == RT_BEAM_GAUSS  ? rt_trace_kernel<RT_BEAM_GAUSS, R>
                                                               : rt_trace_kernel<kLoad, R>);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, a);
         g_last_kernel = "generic";
+        if (block_used) *block_used = kBlock;
         if (getenv("RT_HIP_VERBOSE"))
             fprintf(stderr, "rt_hip: trace generic kernel, %zu rays, grid %u, record %d\n", n_rays, grid, rec);
     }
@@ -967,6 +975,38 @@ extern "C" int rt_hip_trace(const rt_hip_scheme* sch, const double* beam_desc, u
                             void* stream) {
     return trace_launch(sch, beam_desc, seed, ray_begin, n_rays, X, Y, Z, cX, cY, cZ, lambda, flag, flags, ccd_hist,
                         ccd_flux, stats, dump, dump_flag, 0, nullptr, nullptr, stream);
+}
+
+extern "C" int rt_hip_trace_spots(const rt_hip_scheme* sch, const double* beam_desc, uint64_t seed, size_t n_rays,
+                                  size_t n_groups, double* spots, unsigned long long* stats, void* stream) {
+    if (!sch) return set_error(RT_ERR_BAD_VALUE, "scheme is NULL");
+    if (!spots || n_groups == 0) return set_error(RT_ERR_BAD_VALUE, "spots NULL or no groups");
+    if (!beam_desc) return set_error(RT_ERR_BAD_VALUE, "beam_desc is NULL");
+    int rc = ensure_device();
+    if (rc != RT_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const uint64_t G = n_groups;
+    // one partial row per tile of the launched kernel's workgroup size (the
+    // generated kernel's or the generic one's): room for the smaller
+    const uint64_t b_min = (uint64_t)std::min(sch->block, kBlock);
+    const uint64_t tiles_max = (n_rays + b_min - 1) / b_min;
+    if (tiles_max > (~0ull / 40) / G) return set_error(RT_ERR_BAD_VALUE, "too many rays x groups");
+    rc = sch->work.ensure(WorkBufs::TRACE_PART, std::max<uint64_t>(tiles_max, 1) * G * 5 * sizeof(double));
+    if (rc != RT_OK) return rc;
+    double* part = sch->work.get<double>(WorkBufs::TRACE_PART);
+    uint64_t rows = 0;
+    if (n_rays > 0) {
+        int block = 0;
+        rc = trace_launch(sch, beam_desc, seed, 0, n_rays, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                          nullptr, nullptr, RT_TRACE_GENERATE, nullptr, nullptr, stats, nullptr, nullptr,
+                          rt::kRecSpots, nullptr, nullptr, stream, part, G, &block);
+        if (rc != RT_OK) return rc;
+        rows = (n_rays + (uint64_t)block - 1) / (uint64_t)block;
+    }
+    hipLaunchKernelGGL(rt_group_spots_kernel, dim3((unsigned)G), dim3(256), 0, st, (const double*)part, rows, G,
+                       spots);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
+    return RT_OK;
 }
 
 extern "C" int rt_hip_trace_record(const rt_hip_scheme* sch, const double* beam_desc, uint64_t seed, size_t n_rays,
@@ -1462,10 +1502,12 @@ extern "C" int rt_hip_batch_set_launch(rt_hip_batch* b, int block, int waves, in
     return prepare_tiles(b);
 }
 
-extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_desc, uint64_t seed, int flags,
-                                  double* X, double* Y, double* Z, double* cX, double* cY, double* cZ,
-                                  double* lambda, signed char* flag, double* spots,
-                                  unsigned long long* stats, void* stream) {
+namespace {
+// rt_hip_batch_trace; group_part (when set): the group spot partials of every
+// tile, n_groups groups per tile (TraceArgs::group_part)
+int batch_trace(const rt_hip_batch* bc, const double* beam_desc, uint64_t seed, int flags, double* X, double* Y,
+                double* Z, double* cX, double* cY, double* cZ, double* lambda, signed char* flag, double* spots,
+                unsigned long long* stats, void* stream, double* group_part, uint64_t n_groups) {
     rt_hip_batch* b = const_cast<rt_hip_batch*>(bc);
     if (!b) return set_error(RT_ERR_BAD_VALUE, "batch is NULL");
     int rc = validate_beam(beam_desc);
@@ -1498,7 +1540,9 @@ extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_des
     a.tile_start = b->tile_start_dev;
     a.tile_inst = b->tile_inst_dev;
     a.ray_start = b->ray_start_dev;
-    a.tile_moments = b->moments_dev;
+    a.tile_moments = spots ? b->moments_dev : nullptr;
+    a.group_part = group_part;
+    a.n_groups = n_groups;
     a.n_inst = (uint64_t)b->n_inst;
     a.n_tiles = b->n_tiles;
     a.inst_stride = (uint64_t)b->n_surf * rt::DEV_STRIDE;
@@ -1543,7 +1587,7 @@ extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_des
             if (rc != RT_OK) return rc;
             a.tile_start = b->tile_start_dev;
             a.tile_inst = b->tile_inst_dev;
-            a.tile_moments = b->moments_dev;
+            a.tile_moments = spots ? b->moments_dev : nullptr;
             a.n_tiles = b->n_tiles;
         }
         uint64_t cap = (uint64_t)cus * 8;
@@ -1565,6 +1609,43 @@ extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_des
                            (const uint64_t*)b->tile_start_dev, (uint64_t)b->n_inst, spots);
         HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     }
+    return RT_OK;
+}
+}  // namespace
+
+extern "C" int rt_hip_batch_trace(const rt_hip_batch* bc, const double* beam_desc, uint64_t seed, int flags,
+                                  double* X, double* Y, double* Z, double* cX, double* cY, double* cZ,
+                                  double* lambda, signed char* flag, double* spots,
+                                  unsigned long long* stats, void* stream) {
+    return batch_trace(bc, beam_desc, seed, flags, X, Y, Z, cX, cY, cZ, lambda, flag, spots, stats, stream, nullptr,
+                       0);
+}
+
+extern "C" int rt_hip_batch_trace_group_spots(const rt_hip_batch* bc, const double* beam_desc, uint64_t seed,
+                                              size_t n_groups, double* spots, unsigned long long* stats,
+                                              void* stream) {
+    rt_hip_batch* b = const_cast<rt_hip_batch*>(bc);
+    if (!b) return set_error(RT_ERR_BAD_VALUE, "batch is NULL");
+    if (!spots || n_groups == 0) return set_error(RT_ERR_BAD_VALUE, "spots NULL or no groups");
+    int rc = ensure_device();
+    if (rc != RT_OK) return rc;
+    const uint64_t G = n_groups;
+    // partial rows for the tiling of the smaller workgroup the launch may use
+    // (the generic kernel retiles to its own)
+    const uint64_t b_min = (uint64_t)std::min(b->block, kBlock);
+    uint64_t tiles_max = 0;
+    for (int i = 0; i < b->n_inst; ++i) tiles_max += (b->ray_start[i + 1] - b->ray_start[i] + b_min - 1) / b_min;
+    if (tiles_max > (~0ull / 40) / G) return set_error(RT_ERR_BAD_VALUE, "too many rays x groups");
+    rc = b->work.ensure(WorkBufs::TRACE_PART, std::max<uint64_t>(tiles_max, 1) * G * 5 * sizeof(double));
+    if (rc != RT_OK) return rc;
+    double* part = b->work.get<double>(WorkBufs::TRACE_PART);
+    rc = batch_trace(b, beam_desc, seed, RT_TRACE_GENERATE, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, nullptr, nullptr, stats, stream, part, G);
+    if (rc != RT_OK) return rc;
+    // tiles of an instance with no rays hold no rows: its spots stay empty
+    hipLaunchKernelGGL(rt_batch_spots_groups_kernel, dim3((unsigned)(b->n_inst * G)), dim3(256), 0,
+                       (hipStream_t)stream, (const double*)part, (const uint64_t*)b->tile_start_dev, G, spots);
+    HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     return RT_OK;
 }
 
@@ -1657,6 +1738,17 @@ int rt_scheme_ltab_fill(const rt_hip_scheme* s, const double* beam_desc, uint64_
     memcpy(a.beam.v, beam_desc, sizeof a.beam.v);
     *tab = s->lt.tab_dev;
     return ltab_fill(s->lt, a, s->pool_dev, nk, stream);
+}
+
+uint64_t rt_scheme_dtab_nk(const rt_hip_scheme* s, const double* beam_desc) { return dtab_nk(s->dt, beam_desc); }
+
+int rt_scheme_dtab_fill(const rt_hip_scheme* s, const double* beam_desc, uint64_t nk, hipStream_t stream,
+                        const double** tab) {
+    TraceArgs a;
+    memset(&a, 0, sizeof a);
+    memcpy(a.beam.v, beam_desc, sizeof a.beam.v);
+    *tab = s->dt.tab_dev;
+    return dtab_fill(s->dt, a, s->table_dev, s->pool_dev, s->n_surf, nk, stream);
 }
 
 uint64_t rt_batch_ltab_nk(const rt_hip_batch* b, const double* beam_desc) {

@@ -137,14 +137,10 @@ class SpotLoss:
         self.pf = np.array([p.field for p in spec.params], dtype=np.int32)
 
     def _forward(self, values):
-        import torch
         sch = self.spec.apply(self.base, values)
         ds = lib.DeviceScheme(sch)
-        n = sch.beam.n_rays
-        rays = torch.zeros((7, n), dtype=torch.float64, device="cuda")
-        flag = torch.zeros(n, dtype=torch.int8, device="cuda")
-        ds.trace(rays, flag, seed=self.seed, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE)
-        sp = lib.group_spots(rays, flag, self.n_groups)
+        # the trace forms the group spots itself (rt_hip_trace_spots): no rays stored
+        sp = ds.trace_spots(self.n_groups, seed=self.seed)
         return sch, ds, sp
 
     def spots(self, values):

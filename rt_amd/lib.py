@@ -69,6 +69,8 @@ def lib():
         L.rt_hip_beam_order.argtypes = [C.c_size_t, _vp, _vp, C.POINTER(C.c_uint8)] + [_vp] * 8 + [_vp]
         L.rt_hip_spot_loss.argtypes = [_vp, C.c_size_t, _vp, C.c_double, _vp, _vp, _vp]
         L.rt_hip_batch_group_spots.argtypes = [_vp, _vp, _vp, _vp, C.c_size_t, _vp, _vp]
+        L.rt_hip_trace_spots.argtypes = [_vp, _dp, C.c_uint64, C.c_size_t, C.c_size_t, _vp, _vp, _vp]
+        L.rt_hip_batch_trace_group_spots.argtypes = [_vp, _dp, C.c_uint64, C.c_size_t, _vp, _vp, _vp]
         L.rt_hip_batch_descend.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, _vp, _vp, _vp]
         L.rt_hip_batch_adjoint.argtypes = [_vp, _dp, C.c_uint64, _vp, C.c_size_t, C.POINTER(C.c_int),
                                            C.POINTER(C.c_int), C.c_int, _vp, _vp, _vp]
@@ -195,6 +197,22 @@ class DeviceScheme:
                                        _tp(flag), S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, _tp(stats), _tp(loss),
                                        _tp(extents), stream)
         check(rc, "rt_hip_trace_record")
+
+    def trace_spots(self, n_groups: int, *, seed=0, n=None, stats=None, beam=None, stream=None, out=None):
+        """rt_hip_trace_spots: the generated beam of `seed` traced with the
+        (n_groups, SPOT_STRIDE) float64 CUDA spots of the rays alive at the end
+        formed in the trace (no rays stored)."""
+        import torch
+        b = np.ascontiguousarray((beam if beam is not None else self.scheme.beam).pack())
+        if n is None:
+            n = self.scheme.beam.n_rays
+        if out is None:
+            out = torch.empty((n_groups, SPOT_STRIDE), dtype=torch.float64, device="cuda")
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        check(lib().rt_hip_trace_spots(self.handle, _dptr(b), seed, n, n_groups, _tp(out), _tp(stats), stream),
+              "rt_hip_trace_spots")
+        return out
 
     def adjoint(self, group_seeds, param_surface, param_field, *, seed=0, n=None, grad=None, stats=None,
                 beam=None, stream=None):
@@ -431,6 +449,21 @@ class DeviceBatch:
         self._beam = b
         check(lib().rt_hip_batch_trace(self.handle, _dptr(b), seed, flags, *ptrs, _tp(spots), _tp(stats), stream),
               "rt_hip_batch_trace")
+
+    def trace_group_spots(self, n_groups: int, *, seed=0, stats=None, beam=None, stream=None, out=None):
+        """rt_hip_batch_trace_group_spots: the batched trace of generated beams
+        with the (n_inst, n_groups, SPOT_STRIDE) float64 CUDA spots formed in
+        the trace (no rays stored)."""
+        import torch
+        b, _ = self._args(beam, None, None)
+        if out is None:
+            out = torch.empty((self.n_inst, n_groups, SPOT_STRIDE), dtype=torch.float64, device="cuda")
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self._beam = b
+        check(lib().rt_hip_batch_trace_group_spots(self.handle, _dptr(b), seed, n_groups, _tp(out), _tp(stats),
+                                                   stream), "rt_hip_batch_trace_group_spots")
+        return out
 
     def group_spots(self, rays, flag, n_groups: int, stream=None, out=None):
         """rt_hip_batch_group_spots: (n_inst, n_groups, SPOT_STRIDE) float64 CUDA

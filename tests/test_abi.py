@@ -133,11 +133,20 @@ def test_generated_adjoint_kernel_compiles():
     assert "rt_adjoint_jit" in text
     assert text.count("rt::adj::fwd_step(") == sch.n_surf
     assert text.count("rt::adj::bwd_step(") == sch.n_surf
-    # reading the per-wavelength table (0x400): the three glass columns in the
-    # forward (not the detector QE: no flux in the adjoint) and n1/n2 of the
-    # six lens surfaces in the backward, where a back surface's n1 is its
-    # front surface's n2 column
+    # reading a per-wavelength table (0x400).  One scheme: the derived
+    # table's mu and grating columns (2 + 2 for the gratings, 6 for the lens
+    # surfaces; not the detector's flux weight: no flux in the adjoint), read
+    # by the forward steps and again by the backward steps
     rc, size, text = lib.jit_check(sch, S.RT_BEAM_GAUSS | 0x200 | 0x400)
+    assert rc == 0, text[-2000:]
+    assert text.count("rt::adj::load_lval_adj(lv, c, ") == 10
+    assert text.count("rt::adj::load_lval_adj(lb, c, ") == 10
+    assert text.count("rt::adj::fwd_step<true>(") == sch.n_surf
+    assert text.count("rt::adj::bwd_step_d(") == sch.n_surf
+    # a batch (0x100): the three glass columns in the forward (not the
+    # detector QE) and n1/n2 of the six lens surfaces in the backward, where a
+    # back surface's n1 is its front surface's n2 column
+    rc, size, text = lib.jit_check(sch, S.RT_BEAM_GAUSS | 0x200 | 0x400 | 0x100)
     assert rc == 0, text[-2000:]
     assert text.count("rt::adj::load_lval_adj(lv, c, ") == 3
     assert text.count("rt::adj::load_lval_adj(lb, c, ") == 6

@@ -236,6 +236,49 @@ def test_group_spots_match_two_pass_moments(hip, n, n_groups):
         assert np.all(np.abs(sp[g, 1:] - want) <= 1e-12 * np.maximum(np.abs(want), 1.0)), (g, sp[g], want)
 
 
+def spots_close(a, b, what):
+    """group spots of the same rays summed in two orders: counts exact,
+    centroids within 1e-12 of their size (floor 1 mm), variances and RMS
+    radii within 1e-10 relative (floor 1e-12 mm^2 / mm)"""
+    assert np.array_equal(a[:, 0], b[:, 0]), what
+    c = np.abs(a[:, 1:3] - b[:, 1:3]) <= 1e-12 * np.maximum(np.abs(b[:, 1:3]), 1.0)
+    v = np.abs(a[:, 3:6] - b[:, 3:6]) <= 1e-10 * np.abs(b[:, 3:6]) + 1e-12
+    assert np.all(c) and np.all(v), (what, np.argwhere(~c), np.argwhere(~v))
+
+
+@pytest.mark.parametrize("n_groups,launch", [(64, None), (7, None), (300, None), (64, (128, 0, 2)), (5, "generic")])
+def test_trace_spots_equal_stored_rays(hip, n_groups, launch):
+    """rt_hip_trace_spots (spots formed per workgroup tile inside the trace)
+    equals rt_hip_trace with the rays stored followed by rt_hip_group_spots,
+    on groups that divide the workgroup, that do not, that outnumber it, on a
+    128-lane launch and on the generic kernel; the trace's counters are the
+    same."""
+    sch = schemes.echelle(n_per_lambda=157, nlambda=64)        # 10,048 rays: a partial last tile
+    n = sch.beam.n_rays
+    print(f"[trace spots] groups {n_groups} launch {launch}", flush=True)
+    ds = hip.DeviceScheme(sch)
+    prev = hip.set_jit(launch != "generic")
+    try:
+        if isinstance(launch, tuple):
+            ds.set_launch(*launch)
+        R = torch.zeros((7, n), dtype=torch.float64, device="cuda")
+        F = torch.zeros(n, dtype=torch.int8, device="cuda")
+        st_a = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
+        st_b = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
+        ds.trace(R, F, seed=11, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, stats=st_a)
+        want = hip.group_spots(R, F, n_groups).cpu().numpy()
+        got = ds.trace_spots(n_groups, seed=11, stats=st_b).cpu().numpy()
+        torch.cuda.synchronize()
+        kernel = hip.last_kernel()
+    finally:
+        hip.set_jit(prev)
+    assert kernel == ("generic" if launch == "generic" else "generated"), kernel
+    assert np.array_equal(st_a.cpu().numpy(), st_b.cpu().numpy())
+    assert want[:, 0].sum() == int(st_a.cpu().numpy()[S.ST_GOOD]) > 0
+    spots_close(got, want, f"groups {n_groups}")
+    ds.close()
+
+
 def test_spot_loss_and_seeds(hip):
     """rt_hip_spot_loss against the loss and seeds written out in numpy, with
     and without a target, including a group without live rays."""

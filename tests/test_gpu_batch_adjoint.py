@@ -48,6 +48,32 @@ def test_batch_group_spots_equal_per_instance(hip):
     db.close()
 
 
+def test_batch_trace_group_spots_equal_stored_rays(hip):
+    """rt_hip_batch_trace_group_spots (spots per instance and group formed per
+    tile inside the batched trace) equals the batched trace with the rays
+    stored followed by rt_hip_batch_group_spots: counts exact, centroids within
+    1e-12 of their size, variances and RMS radii within 1e-10; the counters
+    are the same."""
+    print("[batch trace spots]", flush=True)
+    base, inst, counts, ps, pf, db = setup_batch(hip)
+    total = int(sum(counts))
+    rays = torch.zeros((7, total), dtype=torch.float64, device="cuda")
+    flag = torch.zeros(total, dtype=torch.int8, device="cuda")
+    st_a = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
+    st_b = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
+    db.trace(seed=7, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, rays=rays, flag=flag, stats=st_a)
+    want = db.group_spots(rays, flag, G).cpu().numpy()
+    got = db.trace_group_spots(G, seed=7, stats=st_b).cpu().numpy()
+    torch.cuda.synchronize()
+    assert np.array_equal(st_a.cpu().numpy(), st_b.cpu().numpy())
+    for i in range(len(inst)):
+        a, b = got[i], want[i]
+        assert np.array_equal(a[:, 0], b[:, 0]), i
+        assert np.all(np.abs(a[:, 1:3] - b[:, 1:3]) <= 1e-12 * np.maximum(np.abs(b[:, 1:3]), 1.0)), i
+        assert np.all(np.abs(a[:, 3:6] - b[:, 3:6]) <= 1e-10 * np.abs(b[:, 3:6]) + 1e-12), i
+    db.close()
+
+
 def test_batch_adjoint_matches_single_scheme_adjoints(hip):
     print("[batch adjoint]", flush=True)
     base, inst, counts, ps, pf, db = setup_batch(hip)
@@ -139,12 +165,14 @@ def test_batch_descend_matches_host_update(hip):
     dh.close()
 
 
-def test_design_step_graph_replays_match_eager(hip):
-    """One design-loop step (batched trace with stored rays, spots, loss and
-    seeds, batched adjoint, descent on the compensators) captured in a graph:
-    three replays give bit for bit the losses and tables of three eager steps
-    on an identical batch."""
-    print("[design graph]", flush=True)
+@pytest.mark.parametrize("fused", [False, True], ids=["stored-rays", "spots-in-trace"])
+def test_design_step_graph_replays_match_eager(hip, fused):
+    """One design-loop step (batched trace with stored rays and spots from
+    them, or the trace forming the spots itself; loss and seeds, batched
+    adjoint, descent on the compensators) captured in a graph: three replays
+    give bit for bit the losses and tables of three eager steps on an
+    identical batch."""
+    print("[design graph]", "spots in the trace" if fused else "stored rays", flush=True)
 
     def make():
         base, inst, counts, _, _, db = setup_batch(hip, n_inst=3, seed=13)
@@ -169,8 +197,11 @@ def test_design_step_graph_replays_match_eager(hip):
                     grad=torch.zeros((n, len(ps)), dtype=torch.float64, device="cuda"))
 
     def step(db, b):
-        db.trace(seed=11, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, rays=b["rays"], flag=b["flag"])
-        db.group_spots(b["rays"], b["flag"], G, out=b["spots"])
+        if fused:
+            db.trace_group_spots(G, seed=11, out=b["spots"])
+        else:
+            db.trace(seed=11, flags=S.RT_TRACE_GENERATE | S.RT_TRACE_STORE, rays=b["rays"], flag=b["flag"])
+            db.group_spots(b["rays"], b["flag"], G, out=b["spots"])
         hip.spot_loss(b["spots"].view(n * G, hip.SPOT_STRIDE), seeds=b["seeds"], loss=b["loss"])
         db.adjoint(b["seeds"].view(n, G, 3), ps, pf, seed=11, grad=b["grad"])
         db.descend(ps, pf, b["grad"], lr)
@@ -202,10 +233,15 @@ def test_design_step_graph_replays_match_eager(hip):
 
 
 def test_adjoint_table_matches_per_ray_splines(hip):
-    """The generated adjoints of a grid-wavelength beam read n1 and n2 from the
-    per-wavelength table in the forward and the backward steps; the gradients
-    equal those of the per-ray spline evaluation bit for bit (single scheme
-    and batch)."""
+    """The generated adjoints of a grid-wavelength beam read their per-
+    wavelength values from a table in the forward and the backward steps (one
+    scheme: mu and the grating term from the derived table; a batch: n1 and n2
+    from the spline-value table).  The batch's gradients equal those of the
+    per-ray spline evaluation bit for bit.  The single scheme's agree per
+    component within 1e-8 relative (plus 1e-13 of the largest component):
+    every per-ray value is the same, but the table kernel's register count
+    sets its occupancy, hence its grid, and the gradient sums run in a fixed
+    order per grid (bit-identity is reported)."""
     print("[adjoint table]", flush=True)
     base, inst, counts, ps, pf, db = setup_batch(hip, n_inst=3, seed=21)
     gen = torch.Generator().manual_seed(4)
@@ -223,7 +259,12 @@ def test_adjoint_table_matches_per_ray_splines(hip):
                 hip.set_ltab(prev)
             out[tab] = (g1, gb)
         assert np.all(np.isfinite(out[True][0])) and np.abs(out[True][0]).max() > 0
-        assert np.array_equal(out[True][0], out[False][0])
+        a, b = out[True][0], out[False][0]
+        scale = np.abs(b).max()
+        rel = float(np.max(np.abs(a - b) / (np.abs(b) + 1e-13 * scale)))
+        print(f"[adjoint table] single scheme: worst relative difference {rel:.3e}, bit-identical "
+              f"{np.array_equal(a, b)}", flush=True)
+        assert np.all(np.abs(a - b) <= 1e-8 * np.abs(b) + 1e-13 * scale)
         assert np.array_equal(out[True][1], out[False][1])
         ds.close()
     finally:
