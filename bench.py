@@ -274,8 +274,7 @@ def adjoint_ratio(warmup, steps):
 
     def loss_forward():
         sp = ds.trace_spots(64, seed=5, out=spots)
-        loss, sd = lib.spot_loss(sp)
-        seeds.copy_(sd)
+        loss, _ = lib.spot_loss(sp, seeds=seeds)
         return loss
 
     def timed(fn, k):
@@ -389,8 +388,7 @@ def batch_adjoint_ratio(n_inst, warmup, steps, max_rays):
 
     def loss_forward():
         sp = db.trace_group_spots(64, seed=5, out=spots)
-        loss, sd = lib.spot_loss(sp.reshape(len(inst) * 64, lib.SPOT_STRIDE))
-        seeds.copy_(sd.reshape(len(inst), 64, 3))
+        loss, _ = lib.spot_loss(sp.view(len(inst) * 64, lib.SPOT_STRIDE), seeds=seeds.view(len(inst) * 64, 3))
         return loss
 
     def adjoint():

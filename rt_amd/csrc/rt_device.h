@@ -917,9 +917,10 @@ struct TraceArgs {
     unsigned char* loss;
     unsigned long long* ext;
     // group spot partials (kernels built with kRecSpots, and batched traces
-    // when set): per tile of blockDim.x rays and group (ray r in group
+    // when set): per group and tile of blockDim.x rays (ray r in group
     // r mod n_groups; r counted from the instance's first ray in a batch),
-    // group_part[(tile * n_groups + g) * 5 ..] = n, mean x, mean y, M2 x, M2 y
+    // group-major, group_part[(g * tiles + tile) * 5 ..] = n, mean x, mean y,
+    // M2 x, M2 y (tiles: of the launch, or the batch's n_tiles)
     double* group_part;
     uint64_t n_groups;
 };
@@ -1324,10 +1325,12 @@ __device__ __forceinline__ void dtab_body(const DtabArgs& a) {
 // Group spot partials of one tile of blockDim.x consecutive rays, the first
 // of index r0 (ray r in group r mod G): per group, the moments (n, mean x,
 // mean y, M2 x, M2 y) of the tile's rays alive at the end, in ray order
-// (Welford's update, as rt_group_moments_kernel forms them), written to
-// part[(tile * G + g) * 5 ..].  Called by every thread of the workgroup.
-__device__ __forceinline__ void tile_group_moments(double* part, uint64_t tile, uint64_t r0, uint64_t G, bool alive,
-                                                   double x, double y) {
+// (Welford's update, as rt_group_moments_kernel forms them), written
+// group-major to part[(g * tiles + tile) * 5 ..] (a group's rows are
+// consecutive for the merge that reads them).  Called by every thread of the
+// workgroup.
+__device__ __forceinline__ void tile_group_moments(double* part, uint64_t tile, uint64_t tiles, uint64_t r0,
+                                                   uint64_t G, bool alive, double x, double y) {
     __shared__ double s_gx[kBlock], s_gy[kBlock];
     __shared__ unsigned char s_ga[kBlock];
     const int B = blockDim.x;                         // <= kBlock
@@ -1349,7 +1352,7 @@ __device__ __forceinline__ void tile_group_moments(double* part, uint64_t tile, 
             qx += dx * (vx - mx);
             qy += dy * (vy - my);
         }
-        double* o = part + (tile * G + g) * 5;
+        double* o = part + (g * tiles + tile) * 5;
         o[0] = n; o[1] = mx; o[2] = my; o[3] = qx; o[4] = qy;
     }
 }
@@ -1446,7 +1449,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
         }
         if constexpr (kTiles)
             tile_group_moments(kernel_args()->group_part, (i - threadIdx.x) / blockDim.x,
-                               a.ray_begin + (i - threadIdx.x), a.n_groups, alive, r.X, r.Y);
+                               (a.n + blockDim.x - 1) / blockDim.x, a.ray_begin + (i - threadIdx.x), a.n_groups,
+                               alive, r.X, r.Y);
     }
     lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
     lds_add(&s_stats[RT_ST_CCD_HITS], c.n_hit);
@@ -1626,7 +1630,8 @@ __device__ __forceinline__ void trace_batch_body(const TraceArgs& a, Surfaces su
         }
         // group spot partials of the tile (asked for)
         if (a.group_part)
-            tile_group_moments(a.group_part, t, (t - ts[inst]) * blockDim.x, a.n_groups, alive, ray.X, ray.Y);
+            tile_group_moments(a.group_part, t, a.n_tiles, (t - ts[inst]) * blockDim.x, a.n_groups, alive, ray.X,
+                               ray.Y);
     }
     lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
     __syncthreads();
