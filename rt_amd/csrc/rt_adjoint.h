@@ -324,9 +324,13 @@ RT_HD void newton_FG(const Rec& d, int kind, const St& in, double s0, double s, 
 }
 
 // forward intersection from `in`; n_fwd > 0 replays that many Newton
-// iterations, 0 iterates to the reference's convergence test.  false: lost
+// iterations, 0 iterates to the reference's convergence test; `given` (the
+// forward pass's own iterates s^(0) .. s^(n_fwd - 1), as its tape holds them)
+// takes their place without replaying them.  false: lost, or more iterations
+// than are differentiated here
 template <class Rec>
-RT_HD bool fwd_intersect(const Rec& d, const St& in, int n_fwd, St& out, NewtonSeq& q) {
+RT_HD bool fwd_intersect(const Rec& d, const St& in, int n_fwd, St& out, NewtonSeq& q,
+                         const NewtonSeq* given = nullptr) {
     const auto p = d.p;
     const int kind = d.ikind();
     out = in;
@@ -351,6 +355,20 @@ RT_HD bool fwd_intersect(const Rec& d, const St& in, int n_fwd, St& out, NewtonS
         if (sigma == sgnB) s = (qq != 0.0) ? C * drcp(qq) : (-B + sigma * sq) * drcp(2.0 * A);
         else s = qq * drcp(A);
         out.X = in.X + s * in.cX; out.Y = in.Y + s * in.cY; out.Z = in.Z + s * in.cZ;
+        return true;
+    }
+    if ((kind == AK_CONIC_NEWTON || kind == AK_TORIC_NEWTON) && given) {
+        q.s0 = -in.Z * drcp(in.cZ);
+        if (!(q.s0 >= 0) || n_fwd < 1 || n_fwd > kMaxNewtonAdj) return false;
+        double s1 = 0.0;
+RT_UNROLL
+        for (int k = 0; k < kMaxNewtonAdj; ++k) {
+            q.s[k] = given->s[k];
+            if (k == n_fwd - 1) s1 = q.s[k];
+        }
+        q.n = n_fwd;
+        const double u = s1 + q.s0;
+        out.X = in.X + u * in.cX; out.Y = in.Y + u * in.cY; out.Z = s1 * in.cZ;
         return true;
     }
     if (kind == AK_CONIC_NEWTON || kind == AK_TORIC_NEWTON) {
@@ -677,7 +695,8 @@ RT_HD bool adj_action(const Rec& d, const St& r, const Geom& g, double mu, doubl
 // of the state after the surface (in), of the entry state (out).  false when
 // the surface has no adjoint here (curved gratings) or the replayed ray is lost.
 template <class Rec>
-RT_HD bool surface_adjoint(const Rec& d, const St& entry, int n_fwd, double mu, double L, St& a, double* gp) {
+RT_HD bool surface_adjoint(const Rec& d, const St& entry, int n_fwd, double mu, double L, St& a, double* gp,
+                           const NewtonSeq* given = nullptr) {
     const auto p = d.p;
     const int type = d.type();
     St sT = entry;
@@ -691,7 +710,7 @@ RT_HD bool surface_adjoint(const Rec& d, const St& entry, int n_fwd, double mu, 
     }
     St sI;
     NewtonSeq q;
-    if (!fwd_intersect(d, sN, n_fwd, sI, q)) return false;
+    if (!fwd_intersect(d, sN, n_fwd, sI, q, given)) return false;
     // the profile at the intersection, for curved mirrors and lenses
     Geom g{};
     if ((type == RT_MIRROR || type == RT_LENS) && d.cls() != RT_PLANE) g = geom_at(d, sI.X, sI.Y, sI.Z);
@@ -759,6 +778,9 @@ struct AdjArgs {
     // batched adjoint with a beam per instance: n_inst records of
     // RT_BEAM_STRIDE + 3 doubles (beam fields, then the uniform values), or null
     const double* inst_beams;
+    // rows of the tape per lane: kTape per surface, and kMaxNewtonAdj more per
+    // iterating intersection where the forward records its Newton iterates
+    uint64_t tape_rows;
 };
 
 // Column q of the current ray's per-wavelength table values (AdjArgs is the
@@ -826,15 +848,26 @@ __device__ __forceinline__ LaneRows lane_rows(double* base, uint64_t rows, uint6
 // forward through one surface: tape of the entry state (rows k0 .. k0+6),
 // the trace's surface step, the Newton count
 // (lv, col: the ray's per-wavelength table values, as surface_step takes them)
-// (DERIV: lv holds the derived table's columns, as surface_step takes them)
-template <bool DERIV = false, class SD>
+// (DERIV: lv holds the derived table's columns, as surface_step takes them;
+// ITER: an iterating intersection records its first kMaxNewtonAdj Newton
+// iterates in rows k0 + kTape ..)
+struct TapeIterSink {
+    const LaneRows* t;
+    int k0;
+    __device__ void operator()(unsigned k, double s) const {
+        if (k < (unsigned)kMaxNewtonAdj) t->st(k0 + kTape + (int)k, s);
+    }
+};
+
+template <bool DERIV = false, bool ITER = false, class SD>
 __device__ __forceinline__ void fwd_step(const SD& d, int s, Ray& r, bool& alive, TraceCtx& c, const LaneRows& t,
                                          int k0, const double* lv = nullptr, int col = -1) {
     if (alive) {
         t.st(k0 + 0, r.X); t.st(k0 + 1, r.Y); t.st(k0 + 2, r.Z);
         t.st(k0 + 3, r.cX); t.st(k0 + 4, r.cY); t.st(k0 + 5, r.cZ);
     }
-    surface_step<SD, 0, DERIV>(d, s, r, alive, c, lv, col);
+    if constexpr (ITER) surface_step<SD, 0, DERIV>(d, s, r, alive, c, lv, col, TapeIterSink{&t, k0});
+    else surface_step<SD, 0, DERIV>(d, s, r, alive, c, lv, col);
     // the Newton count (row 6) only where the intersection iterates
     if ((d.ikind() == IK_CONIC_NEWTON || d.ikind() == IK_TORIC_NEWTON) && alive) t.st(k0 + 6, (double)c.last_it);
 }
@@ -845,7 +878,18 @@ __device__ __forceinline__ void fwd_step(const SD& d, int s, Ray& r, bool& alive
 // returns, so the caller can reduce gp across the wave.  With lv (the ray's
 // per-wavelength table values), n1 and n2 are lv[n1col] and lv[n2col], or
 // the surface's constants where a column is -1.
+// the Newton iterates the forward step recorded (fwd_step with ITER)
 template <class SD>
+__device__ __forceinline__ NewtonSeq taped_iterates(const SD&, const LaneRows& t, int k0, int nit) {
+    NewtonSeq q;
+    q.n = nit;
+    q.s0 = 0.0;
+RT_UNROLL
+    for (int k = 0; k < kMaxNewtonAdj; ++k) q.s[k] = k < nit ? t.ld(k0 + kTape + k) : 0.0;
+    return q;
+}
+
+template <bool ITER = false, class SD>
 __device__ __forceinline__ bool bwd_step(const SD& d, const Ray& r, St& ad, const TraceCtx& c, const LaneRows& t,
                                          int k0, double* gp, const double* lv = nullptr, int n1col = -1,
                                          int n2col = -1) {
@@ -870,12 +914,18 @@ RT_UNROLL
         mu = n1 * in2;
         if (d.type() == RT_GRATING) L = grating_order(d, r.lam) * r.lam * in2 * p[D_INV_D];
     }
+    if constexpr (ITER) {
+        if (d.ikind() == IK_CONIC_NEWTON || d.ikind() == IK_TORIC_NEWTON) {
+            const NewtonSeq q = taped_iterates(d, t, k0, nit);
+            return surface_adjoint(d, entry, nit, mu, L, ad, gp, &q);
+        }
+    }
     return surface_adjoint(d, entry, nit, mu, L, ad, gp);
 }
 
 // bwd_step for kernels reading the derived table: mu = lv[col] and, for a
 // grating, L = lv[col + 1], the values the forward step read
-template <class SD>
+template <bool ITER = false, class SD>
 __device__ __forceinline__ bool bwd_step_d(const SD& d, const Ray& r, St& ad, const TraceCtx& c, const LaneRows& t,
                                            int k0, double* gp, const double* lv, int col) {
 RT_UNROLL
@@ -886,6 +936,12 @@ RT_UNROLL
     if (d.type() == RT_LENS || d.type() == RT_GRATING) {
         mu = lv[col];
         if (d.type() == RT_GRATING) L = lv[col + 1];
+    }
+    if constexpr (ITER) {
+        if (d.ikind() == IK_CONIC_NEWTON || d.ikind() == IK_TORIC_NEWTON) {
+            const NewtonSeq q = taped_iterates(d, t, k0, nit);
+            return surface_adjoint(d, entry, nit, mu, L, ad, gp, &q);
+        }
     }
     return surface_adjoint(d, entry, nit, mu, L, ad, gp);
 }
@@ -1034,7 +1090,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs& a, Fwd forward, Bwd 
         gen_ray<BEAM>(bm, a.seed, i, r, jl);
         c.lk = (uint32_t)jl;
         bool alive = in;
-        forward(r, alive, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane));
+        forward(r, alive, c, lane_rows(a.tape, a.tape_rows, G, lane));
         unsigned long long good = __ballot(alive), all = __ballot(in);
         if (lw == 0) {
             atomicAdd(&s_stats[RT_ST_RAYS], (unsigned)__popcll(all));
@@ -1044,7 +1100,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs& a, Fwd forward, Bwd 
         const uint64_t g = i % a.n_groups;
         const double bx = a.seeds[3 * g], by = a.seeds[3 * g + 1], b = a.seeds[3 * g + 2];
         St ad = {bx + b * r.X, by + b * r.Y, 0.0, 0.0, 0.0, 0.0};
-        const bool ok = backward(r, ad, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane), acc, alive);
+        const bool ok = backward(r, ad, c, lane_rows(a.tape, a.tape_rows, G, lane), acc, alive);
         if (alive && !ok) ++skipped;
     }
     acc.end(a, wave, W, lw);
@@ -1113,7 +1169,7 @@ __device__ __forceinline__ void adjoint_batch_body(const AdjArgs& a, Fwd forward
         gen_ray<BEAM>(bm, ka->seed + inst, r, ray, jl);
         c.lk = (uint32_t)jl;
         bool alive = in;
-        forward(ray, alive, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane));
+        forward(ray, alive, c, lane_rows(a.tape, a.tape_rows, G, lane));
         unsigned long long good = __ballot(alive), all = __ballot(in);
         if (lw == 0) {
             if (all) atomicAdd(&s_stats[RT_ST_RAYS], (unsigned)__popcll(all));
@@ -1124,7 +1180,7 @@ __device__ __forceinline__ void adjoint_batch_body(const AdjArgs& a, Fwd forward
         const double bx = sd[0], by = sd[1], b = sd[2];
         St ad = {bx + b * ray.X, by + b * ray.Y, 0.0, 0.0, 0.0, 0.0};
         acc.tile_begin(a, t * wpg + wib, lw, a.n_tiles * wpg);
-        const bool ok = backward(ray, ad, c, lane_rows(a.tape, (uint64_t)a.n_surf * kTape, G, lane), acc, alive);
+        const bool ok = backward(ray, ad, c, lane_rows(a.tape, a.tape_rows, G, lane), acc, alive);
         if (alive && !ok) ++skipped;
         acc.tile_end(a, t * wpg + wib, lw);
     }

@@ -339,9 +339,11 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     if (oe != hipSuccess || resident < 1) resident = 1;
     uint64_t want = (n_rays + kAdjBlock - 1) / kAdjBlock;
     uint64_t cap = (uint64_t)rt_device_cus() * resident * adjoint_grid_mult(1);
-    // the tape (n_surf * kTape rows) and the accumulators (n_params rows) of
-    // G lanes are addressed as buffer resources of at most 2^31 - 1 bytes
-    const uint64_t rows_max = std::max<uint64_t>((uint64_t)n_surf * kTape, (uint64_t)std::max(n_params, 1));
+    // the tape (tape_rows rows: n_surf * kTape, plus the Newton iterates the
+    // generated kernel records) and the accumulators (n_params rows) of G
+    // lanes are addressed as buffer resources of at most 2^31 - 1 bytes
+    a.tape_rows = fn ? (uint64_t)rt_jit_adjoint_tape_rows(rt_scheme_signature(sch)) : (uint64_t)n_surf * kTape;
+    const uint64_t rows_max = std::max<uint64_t>(a.tape_rows, (uint64_t)std::max(n_params, 1));
     const uint64_t lanes_max = 0x7fffffffull / (rows_max * sizeof(double));
     if (cap > lanes_max / kAdjBlock) cap = lanes_max / kAdjBlock;
     if (cap < 1) return rt_set_error(RT_ERR_BAD_VALUE, "adjoint: too many parameters");
@@ -349,7 +351,7 @@ extern "C" int rt_hip_adjoint(const rt_hip_scheme* sch, const double* beam_desc,
     const uint64_t G = (uint64_t)grid * kAdjBlock;
 
     AdjBufs B;
-    rc = adj_buffers(rt_scheme_work(sch), (size_t)n_surf * kTape * G, (size_t)std::max(n_params, 1) * (G / 64),
+    rc = adj_buffers(rt_scheme_work(sch), (size_t)a.tape_rows * G, (size_t)std::max(n_params, 1) * (G / 64),
                      (size_t)std::max(n_params, 1) * kReduceChunks, host, !fn, B);
     if (rc != RT_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
@@ -443,7 +445,8 @@ extern "C" int rt_hip_batch_adjoint(const rt_hip_batch* b, const double* beam_de
                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, kAdjBlock, lds);
     if (oe != hipSuccess || resident < 1) resident = 1;
     uint64_t cap = (uint64_t)rt_device_cus() * resident * adjoint_grid_mult(16);
-    const uint64_t lanes_max = 0x7fffffffull / ((uint64_t)v.n_surf * kTape * sizeof(double));
+    a.tape_rows = fn ? (uint64_t)rt_jit_adjoint_tape_rows(*v.signature) : (uint64_t)v.n_surf * kTape;
+    const uint64_t lanes_max = 0x7fffffffull / (a.tape_rows * sizeof(double));
     if (cap > lanes_max / kAdjBlock) cap = lanes_max / kAdjBlock;
     const unsigned grid = (unsigned)(v.n_tiles < cap ? v.n_tiles : cap);
     const uint64_t G = (uint64_t)grid * kAdjBlock, wpg = kAdjBlock / 64;
@@ -453,7 +456,7 @@ extern "C" int rt_hip_batch_adjoint(const rt_hip_batch* b, const double* beam_de
                             (unsigned long long)v.n_tiles, n_params);
 
     AdjBufs B;
-    rc = adj_buffers(rt_batch_work(b), (size_t)v.n_surf * kTape * G, part_len, 0, host, !fn, B);
+    rc = adj_buffers(rt_batch_work(b), (size_t)a.tape_rows * G, part_len, 0, host, !fn, B);
     if (rc != RT_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
     a.prow = B.params;

@@ -569,12 +569,19 @@ __device__ __forceinline__ void toric_derivs(const SD& d, double X, double Y, do
     dFdZ = 1.0 - cR * Z;
 }
 
+// Receiver of the Newton iterates of an intersection, sink(k, s^(k)) at the
+// start of iteration k: none in the traces; the adjoint's forward records
+// them on its tape.
+struct NoIterSink {
+    __device__ void operator()(unsigned, double) const {}
+};
+
 // Intersection with the surface (engine_intersection_cycle dispatch,
 // surface.cu:593-636).  Returns false for a lost ray; `iters` counts Newton
 // iterations.  A lost Newton ray keeps its last iterate, as the reference
 // writes Xi,Yi,Zi before flagging (surface.cu:319-326).
-template <class SD>
-__device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& iters) {
+template <class SD, class Sink = NoIterSink>
+__device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& iters, const Sink& sink = Sink{}) {
     const cptr p = d.p;
     const int ik = d.ikind();
     const double c = p[RT_S_C];
@@ -627,6 +634,7 @@ __device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& ite
         double s1, s2 = 0.0, Xi = 0, Yi = 0, Zi = 0;
         do {
             s1 = s2;
+            sink(it, s1);
             Xi = X0 + (s1 + s0) * cX;
             Yi = Y0 + (s1 + s0) * cY;
             Zi = s1 * cZ;
@@ -664,6 +672,7 @@ __device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& ite
         double s1, s2 = 0.0, Xi = 0, Yi = 0, Zi = 0;
         do {
             s1 = s2;
+            sink(it, s1);
             Xi = X0 + (s1 + s0) * cX;
             Yi = Y0 + (s1 + s0) * cY;
             Zi = s1 * cZ;
@@ -1042,9 +1051,9 @@ __device__ __forceinline__ void ext_add(unsigned long long* e, int q, bool on, d
 // evaluates the splines here.  DERIV true: the derived table's columns
 // (dtab_body) give mu, then the grating term, or the detector's flux weight,
 // kOutOfTable where a function's wavelength is outside its table.
-template <class SD, int MODE = 0, bool DERIV = false>
+template <class SD, int MODE = 0, bool DERIV = false, class Sink = NoIterSink>
 __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& alive, TraceCtx& c,
-                                             const double* lv = nullptr, int col = -1) {
+                                             const double* lv = nullptr, int col = -1, const Sink& sink = Sink{}) {
     // the record's loads are issued at this surface, not hoisted ahead of
     // the previous surfaces' work (that would keep every surface's
     // parameters live in SGPRs at once)
@@ -1062,7 +1071,7 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
     unsigned it = 0;
     bool lost_int = false, lost_mask = false, lost_act = false;
     if (alive) {
-        alive = intersect_ray(d, r, it);                         // :1002
+        alive = intersect_ray(d, r, it, sink);                   // :1002
         lost_int = !alive;
     }
     c.last_it = it;
@@ -1324,33 +1333,42 @@ __device__ __forceinline__ void dtab_body(const DtabArgs& a) {
 
 // Group spot partials of one tile of blockDim.x consecutive rays, the first
 // of index r0 (ray r in group r mod G): per group, the moments (n, mean x,
-// mean y, M2 x, M2 y) of the tile's rays alive at the end, in ray order
-// (Welford's update, as rt_group_moments_kernel forms them), written
-// group-major to part[(g * tiles + tile) * 5 ..] (a group's rows are
-// consecutive for the merge that reads them).  Called by every thread of the
-// workgroup.
+// mean y, M2 x, M2 y) of the tile's rays alive at the end (two passes over
+// them: sums, then squared deviations from the mean), written group-major to
+// part[(g * tiles + tile) * 5 ..] (a group's rows are consecutive for the
+// merge that reads them).  Called by every thread of the workgroup, with
+// `buf` alternating between consecutive calls (the LDS copies are double
+// buffered, so one barrier per tile suffices).
 __device__ __forceinline__ void tile_group_moments(double* part, uint64_t tile, uint64_t tiles, uint64_t r0,
-                                                   uint64_t G, bool alive, double x, double y) {
-    __shared__ double s_gx[kBlock], s_gy[kBlock];
-    __shared__ unsigned char s_ga[kBlock];
+                                                   uint64_t G, bool alive, double x, double y, unsigned buf) {
+    __shared__ double s_gx[2][kBlock], s_gy[2][kBlock];
+    __shared__ unsigned char s_ga[2][kBlock];
     const int B = blockDim.x;                         // <= kBlock
-    __syncthreads();                                  // the previous tile's readers are done
-    s_gx[threadIdx.x] = x;
-    s_gy[threadIdx.x] = y;
-    s_ga[threadIdx.x] = alive ? 1 : 0;
+    const unsigned h = buf & 1u;
+    s_gx[h][threadIdx.x] = x;
+    s_gy[h][threadIdx.x] = y;
+    s_ga[h][threadIdx.x] = alive ? 1 : 0;
     __syncthreads();
     const uint64_t shift = r0 % G;
     for (uint64_t g = threadIdx.x; g < G; g += (uint64_t)B) {
-        double n = 0.0, mx = 0.0, my = 0.0, qx = 0.0, qy = 0.0;
-        for (uint64_t q = (g + G - shift) % G; q < (uint64_t)B; q += G) {
-            if (!s_ga[q]) continue;
-            const double vx = s_gx[q], vy = s_gy[q];
-            n += 1.0;
-            const double dx = vx - mx, dy = vy - my;
-            mx += dx / n;
-            my += dy / n;
-            qx += dx * (vx - mx);
-            qy += dy * (vy - my);
+        const uint64_t q0 = (g + G - shift) % G;
+        double n = 0.0, sx = 0.0, sy = 0.0;
+        for (uint64_t q = q0; q < (uint64_t)B; q += G)
+            if (s_ga[h][q]) {
+                n += 1.0;
+                sx += s_gx[h][q];
+                sy += s_gy[h][q];
+            }
+        double mx = 0.0, my = 0.0, qx = 0.0, qy = 0.0;
+        if (n > 0.0) {
+            mx = sx / n;
+            my = sy / n;
+            for (uint64_t q = q0; q < (uint64_t)B; q += G)
+                if (s_ga[h][q]) {
+                    const double dx = s_gx[h][q] - mx, dy = s_gy[h][q] - my;
+                    qx += dx * dx;
+                    qy += dy * dy;
+                }
         }
         double* o = part + (g * tiles + tile) * 5;
         o[0] = n; o[1] = mx; o[2] = my; o[3] = qx; o[4] = qy;
@@ -1394,6 +1412,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
     // are formed by the whole workgroup).  Other kernels let each lane leave
     // after its own last ray.
     constexpr bool kTiles = (MODE & kRecSpots) != 0;
+    unsigned tile_iter = 0;                           // the workgroup's tiles so far (kTiles)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
          MODE == 0 ? i < a.n : (kTiles ? i - threadIdx.x : i - __lane_id()) < a.n; i += stride) {
         const bool valid = MODE == 0 || i < a.n;
@@ -1450,7 +1469,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, Surfaces surfaces
         if constexpr (kTiles)
             tile_group_moments(kernel_args()->group_part, (i - threadIdx.x) / blockDim.x,
                                (a.n + blockDim.x - 1) / blockDim.x, a.ray_begin + (i - threadIdx.x), a.n_groups,
-                               alive, r.X, r.Y);
+                               alive, r.X, r.Y, tile_iter++);
     }
     lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
     lds_add(&s_stats[RT_ST_CCD_HITS], c.n_hit);
@@ -1571,6 +1590,7 @@ __device__ __forceinline__ void trace_batch_body(const TraceArgs& a, Surfaces su
     c.n_prev = 0.0;
     const bool store = (a.flags & RT_TRACE_STORE) != 0;
 
+    unsigned tile_iter = 0;                           // the workgroup's tiles so far
     for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
         // instance of tile t, from the per-tile map (one scalar load; uniform)
         const cargs ka = kernel_args();
@@ -1631,7 +1651,7 @@ __device__ __forceinline__ void trace_batch_body(const TraceArgs& a, Surfaces su
         // group spot partials of the tile (asked for)
         if (a.group_part)
             tile_group_moments(a.group_part, t, a.n_tiles, (t - ts[inst]) * blockDim.x, a.n_groups, alive, ray.X,
-                               ray.Y);
+                               ray.Y, tile_iter++);
     }
     lds_add(&s_stats[RT_ST_SPLINE_OOB], c.n_oob);
     __syncthreads();

@@ -252,6 +252,25 @@ static int sig_arg(const std::string& t, int n) {
     return q == std::string::npos ? 0 : atoi(t.c_str() + q);
 }
 
+// Tape rows of one surface in the generated adjoints: kTape, plus the Newton
+// iterates the forward records where the intersection iterates (StatSurf's
+// IK, its 7th template argument)
+static int adjoint_tape_rows_of(const std::string& type) {
+    const int ik = sig_arg(type, 6);
+    return rt::adj::kTape + ((ik == rt::IK_CONIC_NEWTON || ik == rt::IK_TORIC_NEWTON) ? rt::adj::kMaxNewtonAdj : 0);
+}
+
+int rt_jit_adjoint_tape_rows(const std::string& sig) {
+    int rows = 0;
+    size_t pos = 0;
+    while (pos < sig.size()) {
+        const size_t end = sig.find(';', pos);
+        rows += adjoint_tape_rows_of(sig.substr(pos, end - pos));
+        pos = end + 1;
+    }
+    return rows;
+}
+
 static std::string generate_adjoint_source(const std::string& sig, int beam_type, const std::vector<int>& rows,
                                            int n_surf, int block, int waves = 0, bool batch = false,
                                            bool ltab = false) {
@@ -329,6 +348,10 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
        << ", rt::adj::GroupAcc<" << n_reg << ", RtAdjMap>>(a,\n"
        << "    [&](rt::Ray& r, bool& alive, rt::TraceCtx& c, const rt::adj::LaneRows& t) {\n";
     if (use_tab) os << "      double lv[" << nc << "];\n";
+    // tape rows of each surface: the entry state and Newton count (kTape),
+    // and the Newton iterates where the intersection iterates
+    std::vector<int> k0(n_surf, 0);
+    for (int s = 1; s < n_surf; ++s) k0[s] = k0[s - 1] + adjoint_tape_rows_of(types[s - 1]);
     for (int s = 0; s < n_surf; ++s) {
         // refractive-index columns only: the adjoint accumulates no flux, so
         // the detector's QE and spectrum are never read
@@ -339,8 +362,8 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
                     os << "      rt::adj::load_lval_adj(lv, c, " << q << ");\n";
                     tab_s = true;
                 }
-        os << "      rt::adj::fwd_step" << (derived && use_tab ? "<true>" : "") << "(" << types[s] << "{c.table + "
-           << s << " * rt::DEV_STRIDE}, " << s << ", r, alive, c, t, " << s * rt::adj::kTape;
+        os << "      rt::adj::fwd_step<" << (derived && use_tab ? "true" : "false") << ", true>(" << types[s]
+           << "{c.table + " << s << " * rt::DEV_STRIDE}, " << s << ", r, alive, c, t, " << k0[s];
         if (tab_s) os << ", lv, " << col_base[(size_t)s];
         os << ");\n";
     }
@@ -358,8 +381,8 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
                     os << " rt::adj::load_lval_adj(lb, c, " << q << ");";
                     tab_s = true;
                 }
-            os << " double gp[rt::adj::PG_N]; const bool use = rt::adj::bwd_step_d(" << types[s] << "{c.table + " << s
-               << " * rt::DEV_STRIDE}, r, ad, c, t, " << s * rt::adj::kTape << ", gp, lb, "
+            os << " double gp[rt::adj::PG_N]; const bool use = rt::adj::bwd_step_d<true>(" << types[s]
+               << "{c.table + " << s << " * rt::DEV_STRIDE}, r, ad, c, t, " << k0[s] << ", gp, lb, "
                << (tab_s ? col_base[(size_t)s] : -1) << ") && ok;";
             for (size_t gi = 0; gi < groups.size(); ++gi) {
                 if (groups[gi].surf != s) continue;
@@ -381,8 +404,8 @@ static std::string generate_adjoint_source(const std::string& sig, int beam_type
             if (n2col[(size_t)s] >= 0 && n2col[(size_t)s] != n1col[(size_t)s])
                 os << " rt::adj::load_lval_adj(lb, c, " << n2col[(size_t)s] << ");";
         }
-        os << " double gp[rt::adj::PG_N]; const bool use = rt::adj::bwd_step(" << types[s] << "{c.table + " << s
-           << " * rt::DEV_STRIDE}, r, ad, c, t, " << s * rt::adj::kTape << ", gp";
+        os << " double gp[rt::adj::PG_N]; const bool use = rt::adj::bwd_step<true>(" << types[s] << "{c.table + "
+           << s << " * rt::DEV_STRIDE}, r, ad, c, t, " << k0[s] << ", gp";
         if (tab_s) os << ", lb, " << n1col[(size_t)s] << ", " << n2col[(size_t)s];
         else if (use_tab) os << ", lb, -1, -1";
         os << ") && ok;";

@@ -50,6 +50,11 @@ hipFunction_t rt_jit_kernel(const std::string& signature, int beam_mode, int var
 // n_surf x PG_N entries, the caller's parameter index of each (surface, slot)
 // or -1.  Compiled on first use and cached; nullptr on failure (the caller
 // launches the generic adjoint kernel).
+// Tape rows per lane of the generated adjoint kernels of a signature (entry
+// state and Newton count per surface, the Newton iterates where the
+// intersection iterates); the generic adjoint kernel uses n_surf x kTape.
+int rt_jit_adjoint_tape_rows(const std::string& signature);
+
 hipFunction_t rt_jit_adjoint_kernel(const std::string& signature, int beam_type, const std::vector<int>& rows,
                                     int n_surf, int block, int waves, int device, bool batch,
                                     bool ltab = false);

@@ -230,34 +230,37 @@ __global__ __launch_bounds__(256) void rt_group_spots_kernel(const double* part,
 // Spots from group-major partial rows (the traces' tile partials,
 // TraceArgs::group_part): one workgroup per (instance, group) = blockIdx.x,
 // the group's rows [row_start[i], row_start[i + 1]) of n_rows per group
-// (row_start null: one instance, every row).  Thread t merges rows t,
-// t + 256, ... in order, loading four ahead (consecutive threads read
-// consecutive rows), then a fixed tree over the threads.
-__global__ __launch_bounds__(256) void rt_spots_rows_kernel(const double* part, uint64_t n_rows,
-                                                            const uint64_t* row_start, uint64_t G, double* spots) {
-    __shared__ Mom s[256];
+// (row_start null: one instance, every row).  Thread t of NT merges rows t,
+// t + NT, ... in order, loading four ahead (consecutive threads read
+// consecutive rows), then a fixed tree over the threads.  NT: 1024 for one
+// scheme (thousands of rows per group), 256 for a batch's instances.
+template <int NT>
+__global__ __launch_bounds__(NT) void rt_spots_rows_kernel(const double* part, uint64_t n_rows,
+                                                           const uint64_t* row_start, uint64_t G, double* spots) {
+    constexpr int kRowThreads = NT;
+    __shared__ Mom s[kRowThreads];
     const uint64_t ig = blockIdx.x, i = ig / G, g = ig - i * G;
     const uint64_t r0 = row_start ? row_start[i] : 0, r1 = row_start ? row_start[i + 1] : n_rows;
     const double* base = part + g * n_rows * 5;
     Mom m{0.0, 0.0, 0.0, 0.0, 0.0};
     uint64_t b = r0 + threadIdx.x;
-    for (; b + 3 * 256 < r1; b += 4 * 256) {
+    for (; b + 3 * kRowThreads < r1; b += 4 * kRowThreads) {
         Mom q[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const double* o = base + (b + k * 256) * 5;
+            const double* o = base + (b + k * kRowThreads) * 5;
             q[k] = Mom{o[0], o[1], o[2], o[3], o[4]};
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) m = mom_merge(m, q[k]);
     }
-    for (; b < r1; b += 256) {
+    for (; b < r1; b += kRowThreads) {
         const double* o = base + b * 5;
         m = mom_merge(m, Mom{o[0], o[1], o[2], o[3], o[4]});
     }
     s[threadIdx.x] = m;
     __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
+    for (int k = kRowThreads / 2; k > 0; k >>= 1) {
         if ((int)threadIdx.x < k) s[threadIdx.x] = mom_merge(s[threadIdx.x], s[threadIdx.x + k]);
         __syncthreads();
     }
@@ -1049,7 +1052,7 @@ extern "C" int rt_hip_trace_spots(const rt_hip_scheme* sch, const double* beam_d
         if (rc != RT_OK) return rc;
         rows = (n_rays + (uint64_t)block - 1) / (uint64_t)block;
     }
-    hipLaunchKernelGGL(rt_spots_rows_kernel, dim3((unsigned)G), dim3(256), 0, st, (const double*)part, rows,
+    hipLaunchKernelGGL(rt_spots_rows_kernel<1024>, dim3((unsigned)G), dim3(1024), 0, st, (const double*)part, rows,
                        (const uint64_t*)nullptr, G, spots);
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     return RT_OK;
@@ -1689,7 +1692,8 @@ extern "C" int rt_hip_batch_trace_group_spots(const rt_hip_batch* bc, const doub
                      nullptr, nullptr, nullptr, stats, stream, part, G);
     if (rc != RT_OK) return rc;
     // an instance with no rays has no rows: its spots stay empty
-    hipLaunchKernelGGL(rt_spots_rows_kernel, dim3((unsigned)(b->n_inst * G)), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(rt_spots_rows_kernel<256>, dim3((unsigned)(b->n_inst * G)), dim3(256), 0,
+                       (hipStream_t)stream,
                        (const double*)part, (uint64_t)b->n_tiles, (const uint64_t*)b->tile_start_dev, G, spots);
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     return RT_OK;

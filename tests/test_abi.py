@@ -131,8 +131,13 @@ def test_generated_adjoint_kernel_compiles():
     assert rc == 0, text[-2000:]
     assert size > 10000
     assert "rt_adjoint_jit" in text
-    assert text.count("rt::adj::fwd_step(") == sch.n_surf
-    assert text.count("rt::adj::bwd_step(") == sch.n_surf
+    # the forward records the Newton iterates of the four iterating
+    # intersections (tape offsets 7 rows per surface, 8 more where it iterates)
+    assert text.count("rt::adj::fwd_step<false, true>(") == sch.n_surf
+    assert text.count("rt::adj::bwd_step<true>(") == sch.n_surf
+    import re
+    offsets = [int(k) for k in re.findall(r"r, alive, c, t, (\d+)", text)]
+    assert offsets == [0, 7, 14, 21, 28, 35, 50, 57, 72, 87, 102, 109]
     # reading a per-wavelength table (0x400).  One scheme: the derived
     # table's mu and grating columns (2 + 2 for the gratings, 6 for the lens
     # surfaces; not the detector's flux weight: no flux in the adjoint), read
@@ -141,8 +146,8 @@ def test_generated_adjoint_kernel_compiles():
     assert rc == 0, text[-2000:]
     assert text.count("rt::adj::load_lval_adj(lv, c, ") == 10
     assert text.count("rt::adj::load_lval_adj(lb, c, ") == 10
-    assert text.count("rt::adj::fwd_step<true>(") == sch.n_surf
-    assert text.count("rt::adj::bwd_step_d(") == sch.n_surf
+    assert text.count("rt::adj::fwd_step<true, true>(") == sch.n_surf
+    assert text.count("rt::adj::bwd_step_d<true>(") == sch.n_surf
     # a batch (0x100): the three glass columns in the forward (not the
     # detector QE) and n1/n2 of the six lens surfaces in the backward, where a
     # back surface's n1 is its front surface's n2 column
