@@ -46,6 +46,10 @@ enum DevField {
     D_K1C2 = RT_SURF_STRIDE + 4,       // (k+1) c c, the product the reference forms per ray
     D_INV_XPIX = RT_SURF_STRIDE + 5,   // 1/CCD x pixel size
     D_INV_YPIX = RT_SURF_STRIDE + 6,   // 1/CCD y pixel size
+    // aperture mask constants (mask_ray): circle, size0^2; rectangle of a
+    // Shade, -size0/2, size0/2, -size1/2, size1/2; other rectangle, its edges
+    // cen0 -+ size0/2, cen1 -+ size1/2
+    D_MASK = RT_SURF_STRIDE + 18,
     D_IA = RT_SURF_STRIDE + 8,         // i a_i, i = 1..RT_MAX_POLY (aspheric derivative terms)
     DEV_STRIDE = RT_SURF_STRIDE + 24
 };
@@ -697,19 +701,21 @@ __device__ __forceinline__ bool intersect_ray(const SD& d, Ray& r, unsigned& ite
     return true;   // AuxClass
 }
 
-// Surface::ApplyConstrains (surface.cpp:264-321), Shade::ApplyConstrains (:652-699)
+// Surface::ApplyConstrains (surface.cpp:264-321), Shade::ApplyConstrains
+// (:652-699).  The bounds (s0 * s0, -+s0/2, c0 -+ s0/2, ...) are formed once
+// on the host (D_MASK), the same IEEE values the reference forms per ray.
 template <class SD>
 __device__ __forceinline__ bool mask_ray(const SD& d, double X, double Y) {
     const cptr p = d.p;
     const int type = d.type(), shape = d.shape();
-    const double s0 = p[RT_S_SIZE0], s1 = p[RT_S_SIZE1], c0 = p[RT_S_CEN0], c1 = p[RT_S_CEN1];
+    const double c0 = p[RT_S_CEN0], c1 = p[RT_S_CEN1];
     double xr = X - c0, yr = Y - c1;
     if (type == RT_SHADE) {
-        if (shape == RT_CIRCLE) return !(xr * xr + yr * yr <= s0 * s0);
-        return !((xr >= -s0 / 2.0) && (xr <= s0 / 2.0) && (yr >= -s1 / 2.0) && (yr <= s1 / 2.0));
+        if (shape == RT_CIRCLE) return !(xr * xr + yr * yr <= p[D_MASK]);
+        return !((xr >= p[D_MASK]) && (xr <= p[D_MASK + 1]) && (yr >= p[D_MASK + 2]) && (yr <= p[D_MASK + 3]));
     }
-    if (shape == RT_CIRCLE) return !(xr * xr + yr * yr > s0 * s0);
-    return !((X < (c0 - s0 / 2.0)) || (X > (c0 + s0 / 2.0)) || (Y < (c1 - s1 / 2.0)) || (Y > (c1 + s1 / 2.0)));
+    if (shape == RT_CIRCLE) return !(xr * xr + yr * yr > p[D_MASK]);
+    return !((X < p[D_MASK]) || (X > p[D_MASK + 1]) || (Y < p[D_MASK + 2]) || (Y > p[D_MASK + 3]));
 }
 
 // surface_reflection kernels (surface.cu:771-827)
@@ -1062,7 +1068,11 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
     const cptr p = d.p;
     if (alive) {
         transform_ray(d, r);                                     // scheme.cpp:990
-        normalize_ray(r.cX, r.cY, r.cZ);                         // :993
+        // :993.  A frame change without rotation leaves the direction the
+        // previous surface's last normalisation produced; normalising it again
+        // (as the reference does) changes a component by at most one ulp, so
+        // only the first surface and rotated frames normalise here
+        if (s == 0 || d.rot(0) || d.rot(1) || d.rot(2)) normalize_ray(r.cX, r.cY, r.cZ);
     }
     if constexpr ((MODE & kRecExt) != 0) ext_add(c.ext + s * kExtPerSurf, 0, alive, r.cZ);   // :995
     if (d.type() == RT_AUXTYPE) return;                          // :1000
@@ -1125,7 +1135,11 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
                     }
                 }
             }
-            if (alive) normalize_ray(r.cX, r.cY, r.cZ);          // :1040
+            // :1040.  Only an action changes the direction since the entry
+            // normalisation: a stop, a Shade or a detector keeps it (the
+            // reference's renormalisation would move it by one ulp at most)
+            if (alive && (d.type() == RT_MIRROR || d.type() == RT_LENS || d.type() == RT_GRATING))
+                normalize_ray(r.cX, r.cY, r.cZ);
             lost_act = !alive;
         }
     }

@@ -633,6 +633,16 @@ void rt_device_record(const double* p, double* d) {
     d[rt::D_HALF_Y] = p[RT_S_CCD_YSIZE] / 2.0;
     d[rt::D_INV_XPIX] = p[RT_S_CCD_XPIX] != 0.0 ? 1.0 / p[RT_S_CCD_XPIX] : 0.0;
     d[rt::D_INV_YPIX] = p[RT_S_CCD_YPIX] != 0.0 ? 1.0 / p[RT_S_CCD_YPIX] : 0.0;
+    // mask bounds as the reference forms them per ray (surface.cpp:264-321, 652-699)
+    const double s0 = p[RT_S_SIZE0], s1 = p[RT_S_SIZE1], c0 = p[RT_S_CEN0], c1 = p[RT_S_CEN1];
+    double* m = d + rt::D_MASK;
+    if ((int)p[RT_S_SHAPE] == RT_CIRCLE) {
+        m[0] = s0 * s0;
+    } else if ((int)p[RT_S_TYPE] == RT_SHADE) {
+        m[0] = -s0 / 2.0; m[1] = s0 / 2.0; m[2] = -s1 / 2.0; m[3] = s1 / 2.0;
+    } else {
+        m[0] = c0 - s0 / 2.0; m[1] = c0 + s0 / 2.0; m[2] = c1 - s1 / 2.0; m[3] = c1 + s1 / 2.0;
+    }
     d[rt::D_K1C2] = (p[RT_S_K] + 1) * p[RT_S_C] * p[RT_S_C];
     for (int i = 1; i <= RT_MAX_POLY; ++i) d[rt::D_IA + i - 1] = (double)i * p[RT_S_A + i - 1];
 }
