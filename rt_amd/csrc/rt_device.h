@@ -442,6 +442,11 @@ __device__ __forceinline__ void gen_ray(cptr B, uint64_t seed, uint64_t r, Ray& 
 // bisection, exact knots included (a or b is then exactly 0 and 1).
 // err: 1 argument out of range, 2 zero interval.
 __device__ __forceinline__ double spline_eval(const double* s, double xv, int& err) {
+    // no multiply-add contraction: the kernels that evaluate a table (the
+    // per-ray trace, the per-wavelength table kernels) must form the same
+    // value bit for bit, and which products the compiler would fuse depends
+    // on the code around each copy
+#pragma clang fp contract(off)
     const int K = (int)s[0];
     const double* x = s + 3;
     const double* y = x + K;
@@ -1086,6 +1091,7 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
     }
     c.last_it = it;
     const bool met = alive;           // alive after the intersection
+    const unsigned long long met_mask = __ballot(met);
     bool passed = false;              // alive after the aperture mask
     if (alive) {
         alive = mask_ray(d, r.X, r.Y);                           // :1014
@@ -1159,7 +1165,9 @@ __device__ __forceinline__ void surface_step(const SD& d0, int s, Ray& r, bool& 
         if (lost_act) c.loss = (unsigned char)(3 + 3 * s);
     }
     if (d.ikind() == IK_CONIC_NEWTON || d.ikind() == IK_TORIC_NEWTON) lds_add(&c.st[RT_ST_SURF + 4 * s + 1], it);
-    unsigned long long li = __ballot(lost_int), la = __ballot(lost_mask || lost_act);
+    // lost at the intersection: entered, not met; at the mask or the action:
+    // met, not alive at the end (one ballot each of the alive flag)
+    const unsigned long long li = entering & ~met_mask, la = met_mask & ~__ballot(alive);
     if (__lane_id() == 0) {
         if (li) atomicAdd(&c.st[RT_ST_SURF + 4 * s + 2], (unsigned)__popcll(li));
         if (la) atomicAdd(&c.st[RT_ST_SURF + 4 * s + 3], (unsigned)__popcll(la));

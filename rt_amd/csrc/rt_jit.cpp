@@ -437,10 +437,14 @@ static bool compile_code(const std::string& src, std::vector<char>& code, std::s
     // the atomic optimizer is off: it wraps every statistics add (already made
     // by one lane of a ballot count) in a second lane election and a multiply
     // by the active-lane count, about 130 VALU instructions per pass over the
-    // echelle's surfaces; the counts are integer sums, the same either way
+    // echelle's surfaces; the counts are integer sums, the same either way.
+    // Multiply-adds are fused within a source expression only (-ffp-contract=on),
+    // so every kernel built from the stage code forms the same values: fused
+    // across statements, which products become FMAs depends on the code
+    // around them
     std::vector<const char*> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-munsafe-fp-atomics",
                                      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-                                     "-mllvm", "-amdgpu-atomic-optimizer-strategy=None"};
+                                     "-mllvm", "-amdgpu-atomic-optimizer-strategy=None", "-ffp-contract=on"};
     hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
     size_t log_size = 0;
     hiprtcGetProgramLogSize(prog, &log_size);
