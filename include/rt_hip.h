@@ -336,15 +336,18 @@ int rt_hip_spline_eval(const double *spline, size_t n, const double *x, double *
 int rt_hip_set_jit(int mode);
 
 /* For beams of one wavelength or a composite uniform grid (at most 16384
-   wavelengths), the generated traces read the scheme's spline-valued
-   functions (refractive indices, detector QE and spectrum) from a table of
-   their values at the grid's wavelengths, filled by a small kernel on the
-   trace's stream before each trace, with the same spline code and wavelength
-   formula as the per-ray evaluation (the results are the same bit for bit).
-   Mode 0 evaluates the splines per ray.  Returns the previous mode.  The
-   environment variable RT_HIP_LTAB=0 sets mode 0 at start-up.  The table is
-   per scheme (or batch): traces of one scheme with different wavelength grids
-   must be ordered on one stream.  No counterpart in the reference. */
+   wavelengths), the generated traces read per-wavelength values from a table
+   formed by a small kernel with the same formulas, spline code and wavelength
+   formula as the per-ray evaluation (the results are the same bit for bit):
+   for one scheme n1/n2 of each lens and grating, the grating term and the
+   CCD flux weight (refilled when the wavelength grid or the stream changes,
+   and on every trace once one was captured in a graph); for a batch its
+   spline-valued functions (refractive indices, detector QE and spectrum),
+   filled before each trace.  Mode 0 evaluates everything per ray.  Returns
+   the previous mode.  The environment variable RT_HIP_LTAB=0 sets mode 0 at
+   start-up.  The table is per scheme (or batch): traces of one scheme with
+   different wavelength grids must not run on two streams at once.  No
+   counterpart in the reference. */
 int rt_hip_set_ltab(int mode);
 
 /* Launch of a scheme's generated kernel: lanes per workgroup (64..1024,
