@@ -194,6 +194,7 @@ def main():
                    "flop_per_step_splines_counted": fl_splines,
                    "frac_splines_counted": fl_splines / (kernel_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
     }
+    out["detail"]["stages_separately"] = stages_separately(sch, det, rays, flag, stream, kernel_ms)
     ds.close()
     del rays, flag, hist, flux
 
@@ -225,6 +226,96 @@ def main():
                                          f"{cpu_model()}",
                                "pass_seconds": times}
     print(json.dumps(out))
+
+
+def stages_separately(sch, det, rays, flag, stream, fused_ms, reps=20):
+    """Beam generation and the detector stage timed on their own, with the
+    per-stage entry points, beside the fused trace that does both with the
+    surfaces in between: rt_hip_beam_positions, _directions and _wavelengths
+    write the 7 FP64 rows of every ray of the workload (beam_random,
+    beam_gauss_cosins, uniform_range_beam); the detector stage runs on the
+    rays the last step delivered to the detector, as ccd_frame reads them
+    from a saved beam: the hit counts are rt_hip_ccd_coordinates +
+    rt_hip_ccd_accumulate (ccd_coordinates, compute_ccd_image), the flux
+    image rt_hip_ccd_flux (pixel, QE spline, weight and atomic add in one
+    kernel).  Bytes per ray are the algorithmic ones (rows read and written,
+    one atomic word); the rate is reported against the HBM peak, and
+    the beam's FP64 generation work (Philox, logarithm, sine and cosine,
+    square roots) is beside it.  Median of `reps` HIP-event timings on the
+    launch stream."""
+    import torch
+    from rt_amd import lib
+    from rt_amd import scheme as S
+    L = lib.lib()
+    n = rays.shape[1]
+    st = stream.cuda_stream
+    desc = np.ascontiguousarray(sch.beam.pack())
+    gen = torch.empty((7, n), dtype=torch.float64, device="cuda")
+    P = [gen[k].data_ptr() for k in range(7)]
+    conic = sch.beam.type == S.RT_BEAM_CONIC
+    c = det.ccd
+    xdim, ydim = int(c["xdim"]), int(c["ydim"])
+    xsize, ysize = c.get("xsize", xdim * c["xpix"]), c.get("ysize", ydim * c["ypix"])
+    px = torch.empty(n, dtype=torch.int32, device="cuda")
+    py = torch.empty(n, dtype=torch.int32, device="cuda")
+    hist = torch.zeros(xdim * ydim, dtype=torch.int32, device="cuda")
+    image = torch.zeros(xdim * ydim, dtype=torch.float64, device="cuda")
+    counts = torch.zeros(3, dtype=torch.int64, device="cuda")
+    off = torch.zeros(1, dtype=torch.int64, device="cuda")
+    qe = det.qe
+    qe_rec = None
+    if isinstance(qe, S.Spline):
+        qe_rec = np.concatenate([[len(qe.x)], qe.x, qe.y, qe.y2]).astype(np.float64)
+    alive = flag != 0
+    hits = [rays[k][alive].contiguous() for k in (0, 1, 6)]
+    nh = hits[0].numel()
+    X, Y, lam = (h.data_ptr() for h in hits)
+
+    def beam():
+        lib.check(L.rt_hip_beam_positions(lib._dptr(desc), 7, 0, n, P[0], P[1], P[2], st), "rt_hip_beam_positions")
+        lib.check(L.rt_hip_beam_directions(lib._dptr(desc), 7, 0, n, P[0] if conic else None,
+                                           P[1] if conic else None, P[3], P[4], P[5], st), "rt_hip_beam_directions")
+        lib.check(L.rt_hip_beam_wavelengths(lib._dptr(desc), 7, 0, n, P[6], st), "rt_hip_beam_wavelengths")
+
+    def ccd_hist():
+        lib.check(L.rt_hip_ccd_coordinates(nh, X, Y, xsize, ysize, c["xpix"], c["ypix"], px.data_ptr(),
+                                           py.data_ptr(), st), "rt_hip_ccd_coordinates")
+        lib.check(L.rt_hip_ccd_accumulate(nh, px.data_ptr(), py.data_ptr(), None, xdim, ydim, None,
+                                          hist.data_ptr(), off.data_ptr(), st), "rt_hip_ccd_accumulate")
+
+    def ccd_flux():
+        lib.check(L.rt_hip_ccd_flux(nh, X, Y, lam, lib._dptr(qe_rec) if qe_rec is not None else None,
+                                    0.0 if qe_rec is not None else float(qe), None, 1.0, xsize, ysize,
+                                    c["xpix"], c["ypix"], xdim, ydim, image.data_ptr(), counts.data_ptr(), st),
+                  "rt_hip_ccd_flux")
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        return float(np.median(ms))
+
+    res = {}
+    for name, fn, bytes_per_ray, m, what in (
+            ("beam_generation", beam, 56, n, "7 FP64 rows written"),
+            ("ccd_hit_counts", ccd_hist, 36, nh, "X, Y read; 2 u32 pixel rows written and read; one u32 atomic"),
+            ("ccd_flux", ccd_flux, 32, nh, "X, Y, lambda read; one FP64 atomic")):
+        ms = timed(fn)
+        gbs = bytes_per_ray * m / (ms * 1e-3) / 1e9
+        res[name] = {"ms": ms, "rays": m, "algorithmic_bytes_per_ray": bytes_per_ray, "bytes": what,
+                     "achieved_GBs": gbs, "frac_of_hbm_peak": gbs / PEAK_HBM_GBS}
+    res["fused_trace_ms"] = fused_ms
+    res["ccd_rays_on_frame"] = int(counts[0].item()) // (reps + 1)
+    del gen, px, py, hist, image, hits
+    return res
 
 
 def cpu_model():

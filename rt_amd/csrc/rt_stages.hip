@@ -42,9 +42,25 @@ __device__ __forceinline__ cptr kernarg_record() {
     return (cptr)__builtin_amdgcn_kernarg_segment_ptr();   // StageArgs::rec is at offset 0
 }
 
+// Adds v summed over the workgroup into *dst with one atomic (none for a
+// zero sum).  Every thread of the workgroup calls it, after its ray loop.
+__device__ void block_add(unsigned long long* dst, unsigned long long v) {
+    __shared__ unsigned long long part[kThreads / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (__lane_id() == 0) part[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (unsigned w = 0; w < blockDim.x / 64; ++w) t += part[w];
+        if (t) atomicAdd(dst, t);
+    }
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(kThreads) void rt_stage_kernel(StageArgs a) {
     const rt::DynSurf d{kernarg_record()};
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long it_sum = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
         Ray r;
         r.X = a.X ? a.X[i] : 0.0; r.Y = a.Y ? a.Y[i] : 0.0; r.Z = a.Z ? a.Z[i] : 0.0;
@@ -85,8 +101,9 @@ __global__ __launch_bounds__(kThreads) void rt_stage_kernel(StageArgs a) {
         if (lost) a.flag[i] = 0;
         unsigned long long b = __ballot(lost);
         if (__lane_id() == 0 && b && a.n_bad) atomicAdd(a.n_bad, (unsigned long long)__popcll(b));
-        if (a.iters && it) atomicAdd(a.iters, (unsigned long long)it);
+        it_sum += it;
     }
+    if (a.iters) block_add(a.iters, it_sum);
 }
 
 enum BeamOp { GEN_POSITIONS, GEN_DIRECTIONS, GEN_WAVELENGTHS };
@@ -521,9 +538,9 @@ __global__ __launch_bounds__(kThreads) void rt_ccd_flux_kernel(FluxArgs a) {
         unsafeAtomicAdd(&a.image[(uint64_t)py * a.xdim + px], s / rt::kHC * q * a.area);
     }
     if (a.counts) {
-        if (on_n) atomicAdd(&a.counts[0], (unsigned long long)on_n);
-        if (off_n) atomicAdd(&a.counts[1], (unsigned long long)off_n);
-        if (oob_n) atomicAdd(&a.counts[2], (unsigned long long)oob_n);
+        block_add(&a.counts[0], on_n);
+        block_add(&a.counts[1], off_n);
+        block_add(&a.counts[2], oob_n);
     }
 }
 

@@ -93,6 +93,8 @@ def lib():
         L.rt_hip_beam_wavelengths.argtypes = [_dp, u64, n, n, _vp, _vp]
         L.rt_hip_ccd_coordinates.argtypes = [n, _vp, _vp, dbl, dbl, dbl, dbl, _vp, _vp, _vp]
         L.rt_hip_ccd_accumulate.argtypes = [n, _vp, _vp, _vp, n, n, _vp, _vp, _vp, _vp]
+        L.rt_hip_ccd_flux.argtypes = [n, _vp, _vp, _vp, _dp, dbl, _dp, dbl, dbl, dbl, dbl, dbl, n, n,
+                                      _vp, _vp, _vp]
         L.rt_hip_scale.argtypes = [n, _vp, _vp, _vp]
         L.rt_hip_grating_energy.argtypes = [n, _vp, lng, dbl, dbl, dbl, dbl, dbl, dbl, _vp, _vp]
         L.rt_hip_spline_init.argtypes = [n, _dp, _dp, _dp]
