@@ -400,7 +400,8 @@ def bench_batch(args):
     base = schemes.echelle()
     inst, counts = schemes.tolerance_instances(base, args.batch, seed=1, min_rays=10_000, max_rays=1_000_000)
     db = lib.DeviceBatch(inst, counts)
-    db.set_launch(256, 6, 2)
+    launch = tuple(int(v) for v in args.launch.split(",")) if args.launch else (256, 7, 16)
+    db.set_launch(*launch)
     spots = torch.zeros((len(inst), lib.SPOT_STRIDE), dtype=torch.float64, device="cuda")
     stats = torch.zeros(S.STATS_LEN, dtype=torch.int64, device="cuda")
     stream = torch.cuda.current_stream()
@@ -429,6 +430,7 @@ def bench_batch(args):
         "config": {"workload": "echelle-tolerance-batch", "instances": len(inst), "rays": int(sum(counts)),
                    "min_rays": int(min(counts)), "max_rays": int(max(counts))},
         "detail": {"kernel_ms": ev0.elapsed_time(ev1) / args.steps,
+                   "launch": {"block": launch[0], "waves_per_simd": launch[1], "grid_mult": launch[2]},
                    "ray_surface_per_step": ray_surf // args.steps,
                    "median_rms_spot_um": float(np.median(sp[:, 5]) * 1e3),
                    "rays_per_s": int(sum(counts)) * args.steps / wall}}))

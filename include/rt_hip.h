@@ -353,8 +353,9 @@ int rt_hip_set_ltab(int mode);
 /* Launch of a scheme's generated kernel: lanes per workgroup (64..1024,
    powers of two), the minimum number of waves per SIMD the compiler must fit
    its registers to (0: its own choice), and persistent workgroups per CU as a
-   multiple of those that are resident at once.  Default 256, 0, 2.  No
-   counterpart in the reference (its launches are fixed per kernel). */
+   multiple of those that are resident at once.  Default 256, 7, 16 (for
+   schemes and batches).  No counterpart in the reference (its launches are
+   fixed per kernel). */
 int rt_hip_scheme_set_launch(rt_hip_scheme *scheme, int block, int waves, int grid_mult);
 int rt_hip_scheme_get_launch(const rt_hip_scheme *scheme, int *block, int *waves, int *grid_mult);
 

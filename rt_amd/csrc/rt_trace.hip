@@ -403,28 +403,49 @@ __global__ void rt_batch_descend_kernel(double* tables, uint64_t n_inst, int n_s
 // dloss/dX_r = ax + b X_r for the group's rays: b = 1 / (n rms),
 // ax = -b cx + 2 w (cx - tx) / n, likewise for y.  Thread t sums groups
 // t, t + 256, ... in order, then a fixed tree: bit-reproducible.
-__global__ __launch_bounds__(256) void rt_spot_loss_kernel(const double* spots, uint64_t G, const double* target,
-                                                           double w, double* seeds, double* loss) {
-    __shared__ double s[256];
-    double acc = 0.0;
-    for (uint64_t g = threadIdx.x; g < G; g += 256) {
-        const double* sp = spots + g * RT_SPOT_STRIDE;
-        const double n = sp[0], cx = sp[1], cy = sp[2], rms = sp[5];
-        const double tx = target ? target[2 * g] : 0.0, ty = target ? target[2 * g + 1] : 0.0;
-        double ax = 0.0, ay = 0.0, b = 0.0;
-        if (n > 0.0) {
-            acc += rms + w * ((cx - tx) * (cx - tx) + (cy - ty) * (cy - ty));
-            b = rms > 0.0 ? 1.0 / (n * rms) : 0.0;
-            ax = -b * cx + 2.0 * w * (cx - tx) / n;
-            ay = -b * cy + 2.0 * w * (cy - ty) / n;
-        }
-        seeds[3 * g] = ax;
-        seeds[3 * g + 1] = ay;
-        seeds[3 * g + 2] = b;
+// Loss and adjoint seeds of every group, one workgroup: each lane takes the
+// groups lane, lane + kLossThreads, ... four at a time (their records loaded
+// together, so a lane has four loads in flight), adds its groups' loss terms
+// in that order, and the workgroup sums the lanes' terms by a fixed tree.
+constexpr int kLossThreads = 1024;
+
+__device__ __forceinline__ double spot_loss_group(const double* sp, const double* target, uint64_t g, double w,
+                                                  double* seeds) {
+    const double n = sp[0], cx = sp[1], cy = sp[2], rms = sp[5];
+    const double tx = target ? target[2 * g] : 0.0, ty = target ? target[2 * g + 1] : 0.0;
+    double ax = 0.0, ay = 0.0, b = 0.0, term = 0.0;
+    if (n > 0.0) {
+        term = rms + w * ((cx - tx) * (cx - tx) + (cy - ty) * (cy - ty));
+        b = rms > 0.0 ? 1.0 / (n * rms) : 0.0;
+        ax = -b * cx + 2.0 * w * (cx - tx) / n;
+        ay = -b * cy + 2.0 * w * (cy - ty) / n;
     }
+    seeds[3 * g] = ax;
+    seeds[3 * g + 1] = ay;
+    seeds[3 * g + 2] = b;
+    return term;
+}
+
+__global__ __launch_bounds__(kLossThreads) void rt_spot_loss_kernel(const double* spots, uint64_t G,
+                                                                    const double* target, double w, double* seeds,
+                                                                    double* loss) {
+    __shared__ double s[kLossThreads];
+    constexpr uint64_t T = kLossThreads;
+    double acc = 0.0;
+    uint64_t g = threadIdx.x;
+    for (; g + 3 * T < G; g += 4 * T) {
+        double rec[4][6];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) rec[k][q] = spots[(g + k * T) * RT_SPOT_STRIDE + q];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc += spot_loss_group(rec[k], target, g + k * T, w, seeds);
+    }
+    for (; g < G; g += T) acc += spot_loss_group(spots + g * RT_SPOT_STRIDE, target, g, w, seeds);
     s[threadIdx.x] = acc;
     __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
+    for (int k = kLossThreads / 2; k > 0; k >>= 1) {
         if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
         __syncthreads();
     }
@@ -488,9 +509,13 @@ struct rt_hip_scheme {
     std::vector<double> dtable;      // device records
     std::vector<double> dpool;       // device spline records
     // launch of the generated kernel (rt_hip_scheme_set_launch, rt_hip_autotune)
+    // (default 256 lanes, 7 waves/SIMD, 16 workgroups per resident slot: the
+    // autotuner's usual choice on the echelle, and the fastest of a sweep of
+    // the batched trace over 1024 instances of uneven size, 21.3 ms against
+    // 23.7-24.0 ms at a grid multiple of 2)
     int block = rt::kBlock;          // lanes per workgroup
-    int waves = 0;                   // minimum waves per SIMD asked of the compiler (0: its choice)
-    int grid_mult = 2;               // persistent workgroups per CU = grid_mult x resident ones
+    int waves = 7;                   // minimum waves per SIMD asked of the compiler (0: its choice)
+    int grid_mult = 16;              // persistent workgroups per CU = grid_mult x resident ones
     // surface unroll depth: 1, the whole chain unrolled (the generated
     // kernel); 0, a runtime loop over the records (the generic kernel)
     int unroll = 1;
@@ -1318,7 +1343,7 @@ struct rt_hip_batch {
     double* tables_dev = nullptr;
     double* pool_dev = nullptr;
     uint64_t* ray_start_dev = nullptr;
-    int block = rt::kBlock, waves = 0, grid_mult = 2;
+    int block = rt::kBlock, waves = 7, grid_mult = 16;   // as for one scheme
     int tiles_block = 0;             // workgroup size the tile arrays below are built for
     uint64_t n_tiles = 0;
     uint64_t* tile_start_dev = nullptr;
@@ -1966,8 +1991,8 @@ extern "C" int rt_hip_spot_loss(const double* spots, size_t n_groups, const doub
     if (!spots || !seeds || !loss || n_groups == 0) return set_error(RT_ERR_BAD_VALUE, "NULL argument or no groups");
     int rc = ensure_device();
     if (rc != RT_OK) return rc;
-    hipLaunchKernelGGL(rt_spot_loss_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, spots, (uint64_t)n_groups,
-                       target, weight, seeds, loss);
+    hipLaunchKernelGGL(rt_spot_loss_kernel, dim3(1), dim3(kLossThreads), 0, (hipStream_t)stream, spots,
+                       (uint64_t)n_groups, target, weight, seeds, loss);
     HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     return RT_OK;
 }

@@ -239,7 +239,7 @@ class DeviceScheme:
                                    stream), "rt_hip_adjoint")
         return grad
 
-    def set_launch(self, block: int = 256, waves: int = 0, grid_mult: int = 2):
+    def set_launch(self, block: int = 256, waves: int = 7, grid_mult: int = 16):
         """Launch of the generated kernel (rt_hip_scheme_set_launch)."""
         check(lib().rt_hip_scheme_set_launch(self.handle, block, waves, grid_mult), "rt_hip_scheme_set_launch")
 
@@ -414,7 +414,7 @@ class DeviceBatch:
         self._t, self._p = t, p
         check(lib().rt_hip_batch_update(self.handle, _dptr(t), _dptr(p), len(p)), "rt_hip_batch_update")
 
-    def set_launch(self, block=256, waves=0, grid_mult=2):
+    def set_launch(self, block=256, waves=7, grid_mult=16):
         check(lib().rt_hip_batch_set_launch(self.handle, block, waves, grid_mult), "rt_hip_batch_set_launch")
 
     def info(self):
