@@ -79,6 +79,9 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="measure the adjoint instead: gradient of the spot loss over the echelle's 48 design "
                          "parameters at 1e6 rays over 64 wavelengths, as a time ratio to the plain forward trace")
+    ap.add_argument("--scheme-run", action="store_true",
+                    help="measure the drop-in Scheme path instead: the fixed-order echelle written as the "
+                         "reference's XML and run by rt_scheme_run_file (files and PCIe included)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -109,6 +112,8 @@ def main():
         return bench_batch_adjoint(args)
     if args.design:
         return bench_design(args)
+    if args.scheme_run:
+        return bench_scheme_run(args)
     sch = schemes.echelle(args.n_per_lambda, args.nlambda)
     n = sch.beam.n_rays
     ds = lib.DeviceScheme(sch)
@@ -316,6 +321,61 @@ def stages_separately(sch, det, rays, flag, stream, fused_ms, reps=20):
     res["ccd_rays_on_frame"] = int(counts[0].item()) // (reps + 1)
     del gen, px, py, hist, image, hits
     return res
+
+
+def bench_scheme_run(args):
+    """The drop-in Scheme path end to end, as ray-tracing/main.cpp runs it on
+    this library: the echelle in one fixed order (4,883 rays x 2,048
+    wavelengths by default) written as the reference's XML scheme file and run
+    by rt_scheme_run_file: Scheme::load_from_file, the input beam generated
+    and saved, one recording trace of the range (loss codes, logged extents),
+    the Rearrange order and the renormalised lost rays formed on the device,
+    the result file written and the log.  Wall time per call, with the two
+    .dat files (57 B per ray each) and the PCIe copies inside it; the same
+    scheme through the reference's own per-stage host loop on the plugin takes
+    seconds (oracle/_ref/ref_pipeline_hip).  At most 5 timed calls after one
+    warm-up."""
+    import re
+    import shutil
+    import tempfile
+    from rt_amd import lib, scheme_xml, schemes
+    from rt_amd import scheme as S
+    sc = schemes.echelle_fixed_order(args.n_per_lambda, args.nlambda)
+    steps, warmup = max(1, min(args.steps, 5)), min(args.warmup, 1)
+    d = tempfile.mkdtemp(prefix="rt_scheme_run_")
+    try:
+        path = scheme_xml.write_scheme(sc, d)
+        lst, log = os.path.join(d, "list.txt"), os.path.join(d, "log.txt")
+        times = []
+        for k in range(warmup + steps):
+            t0 = time.perf_counter()
+            rc = lib.scheme_run_file(path, lst, log)
+            dt = time.perf_counter() - t0
+            if rc != 0:
+                raise RuntimeError("rt_scheme_run_file returned %d: %s" % (rc, open(log).read()[-2000:]))
+            if k >= warmup:
+                times.append(dt)
+            print(f"[scheme run] call {k}: {dt:.3f} s", file=sys.stderr, flush=True)
+        stages = 3 * sum(1 for s in sc.surfaces if s.type != S.RT_AUXTYPE)
+        good = [int(v) for v in re.findall(r"number of good rays: (\d+)\)", open(log).read())][-stages:]
+        n = sc.beam.n_rays
+        # rays entering each surface's intersection: all rays, then the good
+        # rays after the previous surface's action
+        entering = [n] + good[2:-1:3]
+        ray_surf = sum(entering)
+        files = [l.strip() for l in open(lst) if l.strip()]
+        file_bytes = sum(os.path.getsize(f) for f in files if os.path.exists(f))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    sec = float(np.median(times))
+    print(json.dumps({
+        "metric": "drop-in Scheme::run end to end (fixed-order echelle, files and PCIe included)",
+        "value": ray_surf / sec, "unit": "ray*surface/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic (Philox beam)",
+        "config": {"workload": "echelle-fixed-order-scheme-run", "rays": n, "n_per_lambda": args.n_per_lambda,
+                   "nlambda": args.nlambda, "surfaces": len(sc.surfaces)},
+        "detail": {"seconds_per_call": times, "median_s": sec, "ray_surface_per_call": ray_surf,
+                   "good_rays_at_end": good[-1] if good else None, "result_file_bytes": file_bytes}}))
 
 
 def cpu_model():

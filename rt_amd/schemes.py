@@ -199,6 +199,24 @@ def echelle(n_per_lambda: int = 4883, nlambda: int = 2048) -> Scheme:
     return Scheme(s, beam, name="echelle")
 
 
+def echelle_fixed_order(n_per_lambda: int = 4883, nlambda: int = 2048) -> Scheme:
+    """The echelle with its echelle grating in one fixed order, as the
+    reference's scheme file states a grating (a list of fixed orders,
+    surface.cpp:520-590), the beam inside that order's free spectral range,
+    and a detector spectrum table, so the scheme can be written as the
+    reference's XML and run by Scheme::run."""
+    from .scheme import RT_ORDER_FIXED, Spline
+    base = echelle(n_per_lambda=n_per_lambda, nlambda=nlambda)
+    surfs = list(base.surfaces)
+    g = surfs[3]
+    m = int(round(g.blaze_k / 0.6))
+    surfs[3] = dataclasses.replace(g, order_mode=RT_ORDER_FIXED, order=m)
+    surfs[-1] = dataclasses.replace(surfs[-1], spectrum=Spline(np.linspace(0.55, 0.65, 11),
+                                                                1.0 + 0.5 * np.linspace(0, 1, 11) ** 2))
+    beam = dataclasses.replace(base.beam, range=(0.597, 0.603))
+    return Scheme(surfs, beam, name="echelle-fixed-order")
+
+
 def tolerance_instances(base: Scheme, n_inst: int, seed: int = 0, decenter_mm: float = 0.020,
                         tilt_arcsec: float = 30.0, curvature_rel: float = 1e-4,
                         min_rays: int = 10_000, max_rays: int = 1_000_000, first: int = None):

@@ -34,7 +34,7 @@ import numpy as np
 import pytest
 
 import refpipe as R
-import scheme_xml as X
+from rt_amd import scheme_xml as X
 import test_saved_beam as SB
 from rt_amd import scheme as S
 from rt_amd import schemes

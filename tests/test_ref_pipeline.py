@@ -29,19 +29,10 @@ RTOL = 1e-12          # positions, cosines: relative to max(1, |value|)
 
 
 def echelle_fixed_order(n_per_lambda=1000, nlambda=16):
-    """The echelle spectrograph with its echelle in one fixed order (the
-    reference's Grating takes a list of fixed orders, surface.cpp:520-590)
-    and the beam inside that order's free spectral range."""
-    base = schemes.echelle(n_per_lambda=n_per_lambda, nlambda=nlambda)
-    surfs = list(base.surfaces)
-    g = surfs[3]
-    m = int(round(g.blaze_k / 0.6))
-    surfs[3] = dataclasses.replace(g, order_mode=S.RT_ORDER_FIXED, order=m)
-    surfs[-1] = dataclasses.replace(surfs[-1], spectrum=S.Spline(np.linspace(0.55, 0.65, 11),
-                                                                  1.0 + 0.5 * np.linspace(0, 1, 11) ** 2))
-    beam = dataclasses.replace(base.beam, range=(0.597, 0.603))
+    """The echelle in one fixed order (schemes.echelle_fixed_order) and the
+    efficiency rules of its gratings."""
     rules = {3: (63.4, 63.4, 0.0), 4: (12.0, 0.0, 0.0)}
-    return S.Scheme(surfs, beam, name="echelle-fixed-order"), rules
+    return schemes.echelle_fixed_order(n_per_lambda=n_per_lambda, nlambda=nlambda), rules
 
 
 def mixed(n=20000):

@@ -24,7 +24,7 @@ import numpy as np
 import pytest
 
 import branch_schemes as B
-import scheme_xml as X
+from rt_amd import scheme_xml as X
 from rt_amd import lib
 from rt_amd import scheme as S
 from rt_amd import schemes
