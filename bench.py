@@ -367,6 +367,7 @@ def bench_scheme_run(args):
         file_bytes = sum(os.path.getsize(f) for f in files if os.path.exists(f))
     finally:
         shutil.rmtree(d, ignore_errors=True)
+    device = scheme_path_device_ms(sc)
     sec = float(np.median(times))
     print(json.dumps({
         "metric": "drop-in Scheme::run end to end (fixed-order echelle, files and PCIe included)",
@@ -375,7 +376,58 @@ def bench_scheme_run(args):
         "config": {"workload": "echelle-fixed-order-scheme-run", "rays": n, "n_per_lambda": args.n_per_lambda,
                    "nlambda": args.nlambda, "surfaces": len(sc.surfaces)},
         "detail": {"seconds_per_call": times, "median_s": sec, "ray_surface_per_call": ray_surf,
-                   "good_rays_at_end": good[-1] if good else None, "result_file_bytes": file_bytes}}))
+                   "good_rays_at_end": good[-1] if good else None, "result_file_bytes": file_bytes,
+                   "device_ms": device}}))
+
+
+def scheme_path_device_ms(sc, reps=5):
+    """Device time of one range of Scheme::run through the C ABI, median of
+    `reps` HIP-event timings each: the recording trace (beam generated, every
+    stage, loss codes, logged extents), the Rearrange order
+    (rt_hip_rearrange, counts to the host) and the file-order arrays with the
+    lost rays renormalised (rt_hip_beam_order, with the later-Normalize counts
+    Scheme::run forms: 1 + 2 per later surface, 1 per later AUX surface)."""
+    import torch
+    from rt_amd import lib
+    from rt_amd import scheme as S
+    n = sc.beam.n_rays
+    ds = lib.DeviceScheme(sc)
+    rays = torch.empty((7, n), dtype=torch.float64, device="cuda")
+    flag = torch.empty(n, dtype=torch.int8, device="cuda")
+    loss = torch.empty(n, dtype=torch.uint8, device="cuda")
+    ext = torch.empty(sc.n_surf * lib.EXT_PER_SURF, dtype=torch.float64, device="cuda")
+    renorm = np.zeros(256, dtype=np.uint8)
+    for s in range(sc.n_surf):
+        later = sum(1 if t.type == S.RT_AUXTYPE else 2 for t in sc.surfaces[s + 1:])
+        for k in range(3):
+            renorm[3 * s + k + 1] = min(255, 1 + later)
+    stream = torch.cuda.current_stream()
+    state = {}
+
+    def record():
+        ds.trace_record(rays, flag, loss, seed=1, extents=ext)
+
+    def rearrange():
+        state["perm"], state["counts"] = lib.rearrange(loss)
+
+    def order():
+        state["out"] = lib.beam_order(state["perm"], loss, renorm, rays)
+
+    out = {}
+    for name, fn in (("recording_trace", record), ("rearrange", rearrange), ("beam_order", order)):
+        fn()
+        ms = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        out[name] = float(np.median(ms))
+    out["good_rays"] = int(state["counts"][0])
+    ds.close()
+    return out
 
 
 def cpu_model():
