@@ -39,6 +39,14 @@
 namespace rt {
 namespace adj {
 
+// The derivative formulas below run only in the reverse pass (the forward
+// pass is the trace's own stage code, compiled with multiply-adds fused
+// within a source expression only): here the compiler may fuse a multiply
+// and an add across statements too.
+#if defined(__clang__)
+#pragma clang fp contract(fast)
+#endif
+
 // reciprocal and reciprocal square root: the refined hardware estimates of
 // rt_device.h on the device, IEEE on the host
 RT_HD double drcp(double x) {
@@ -730,6 +738,10 @@ RT_HD bool surface_adjoint(const Rec& d, const St& entry, int n_fwd, double mu, 
     a = aI;
     return true;
 }
+
+#if defined(__clang__)
+#pragma clang fp contract(on)
+#endif
 
 }  // namespace adj
 }  // namespace rt
