@@ -363,16 +363,27 @@ __device__ __forceinline__ void gen_direction(cptr B, uint64_t seed, uint64_t r,
         sincos_bounded(kTwoPi * u4, sb, cb);
         double tX = tg * (rad * cb);
         double tY = tg * (rad * sb);
-        // cos = 1/sqrt(1+t^2); sin = sign(t)/sqrt(1+1/t^2) = t * cos
-        double cX, cY, cZ = 1.0, cs, sn;
-        cs = rsqrt(1.0 + tX * tX);
-        sn = tX * cs;
+        // cos = 1/sqrt(1+t^2); sin = sign(t)/sqrt(1+1/t^2) = t * cos.  For a
+        // narrow beam, tan sigma < 1e-5, every |t| is below 1e-4 (the
+        // Box-Muller radius is at most 8.6), and 1 - x/2 + 3x^2/8 in x = t^2
+        // is within 1e-24 of 1/sqrt(1+x): no reciprocal square root.  The
+        // beam's uniform value picks the branch for the whole wave.
+        double csx, csy;
+        if (tg < 1e-5) {
+            const double x = tX * tX, y = tY * tY;
+            csx = fma(x, fma(0.375, x, -0.5), 1.0);
+            csy = fma(y, fma(0.375, y, -0.5), 1.0);
+        } else {
+            csx = rsqrt(1.0 + tX * tX);
+            csy = rsqrt(1.0 + tY * tY);
+        }
+        double cX, cY, cZ = 1.0, sn;
+        sn = tX * csx;
         cY = -sn * cZ;
-        cZ = cs * cZ;
-        cs = rsqrt(1.0 + tY * tY);
-        sn = tY * cs;
+        cZ = csx * cZ;
+        sn = tY * csy;
         cX = -sn * cZ;
-        cZ = cs * cZ;
+        cZ = csy * cZ;
         ray.cX = cX; ray.cY = cY; ray.cZ = cZ;
     }
 }
