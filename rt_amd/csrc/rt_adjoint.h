@@ -765,7 +765,7 @@ namespace adj {
 constexpr int kTape = 7;                 // X Y Z cX cY cZ, Newton iterations
 
 struct AdjArgs {
-    double beam[RT_BEAM_STRIDE + 3];     // beam fields + BeamUniform (first: kernarg offset 0)
+    double beam[kBeamRec];               // beam fields + BeamUniform (first: kernarg offset 0)
     const double* table;                 // device records
     const double* pool;
     int n_surf, pool_len, n_par, pad;
@@ -788,7 +788,7 @@ struct AdjArgs {
     const double* ltab;
     uint64_t lt_nk;
     // batched adjoint with a beam per instance: n_inst records of
-    // RT_BEAM_STRIDE + 3 doubles (beam fields, then the uniform values), or null
+    // kBeamRec doubles (beam fields, then the uniform values), or null
     const double* inst_beams;
     // rows of the tape per lane: kTape per surface, and kMaxNewtonAdj more per
     // iterating intersection where the forward records its Newton iterates
@@ -1176,7 +1176,7 @@ __device__ __forceinline__ void adjoint_batch_body(const AdjArgs& a, Fwd forward
         cptr tbl = (cptr)ka->table + (uint32_t)inst * (uint32_t)ka->inst_stride;
         c.table = tbl;
         Ray ray;
-        cptr bm = ka->inst_beams ? (cptr)ka->inst_beams + inst * (uint64_t)(RT_BEAM_STRIDE + 3) : (cptr)ka;
+        cptr bm = ka->inst_beams ? (cptr)ka->inst_beams + inst * (uint64_t)kBeamRec : (cptr)ka;
         uint64_t jl;
         gen_ray<BEAM>(bm, ka->seed + inst, r, ray, jl);
         c.lk = (uint32_t)jl;

@@ -262,6 +262,7 @@ void fill_beam(AdjArgs& a, const double* beam_desc) {
     a.beam[rt::BU_CONE_COS] = cos(ang / 2.0);                           // beam.cu:593-594
     a.beam[rt::BU_CONE_SIN] = sin(ang / 2.0);
     a.beam[rt::BU_GAUSS_TG] = tan(a.beam[RT_B_ANGPAR] / 2.35482 * rt::kPi / 180 / 3600);   // beam.cu:1034,1099
+    a.beam[rt::BU_LAM_STEP] = rt::beam_lam_step(a.beam);
 }
 
 // minimum waves per SIMD asked of the generated kernels' register allocation:

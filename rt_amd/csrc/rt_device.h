@@ -336,10 +336,26 @@ __device__ __forceinline__ void gen_position(cptr B, uint64_t seed, uint64_t r, 
 // the host as the reference does: the cone half-angle sin/cos of
 // conic_cosins_kernel (beam.cu:592-598) and tan(sigma) of gauss_cosins_cycle
 // (beam.cu:1034).
+// lam_step: the spacing (range[1] - range[0]) / (N - 1) of a uniform
+// wavelength grid of N > 1 wavelengths (N: the composite beam's NLAMBDA, else
+// the ray count), the IEEE quotient formed once on the host (0 otherwise).
 struct BeamUniform {
-    double cone_cos, cone_sin, gauss_tg;
+    double cone_cos, cone_sin, gauss_tg, lam_step;
 };
-enum { BU_CONE_COS = RT_BEAM_STRIDE, BU_CONE_SIN = RT_BEAM_STRIDE + 1, BU_GAUSS_TG = RT_BEAM_STRIDE + 2 };
+enum {
+    BU_CONE_COS = RT_BEAM_STRIDE, BU_CONE_SIN = RT_BEAM_STRIDE + 1, BU_GAUSS_TG = RT_BEAM_STRIDE + 2,
+    BU_LAM_STEP = RT_BEAM_STRIDE + 3
+};
+
+// the uniform wavelength grid's size and spacing of a beam description (host
+// and device; the division is IEEE, so the step equals grid_lambda's)
+__host__ __device__ inline uint64_t beam_grid_size(const double* B) {
+    return B[RT_B_COMPOSITE] != 0.0 ? (uint64_t)B[RT_B_NLAMBDA] : (uint64_t)B[RT_B_NPER];
+}
+__host__ __device__ inline double beam_lam_step(const double* B) {
+    const uint64_t nl = beam_grid_size(B);
+    return nl > 1 ? (B[RT_B_RANGE + 1] - B[RT_B_RANGE + 0]) / (double)(nl - 1) : 0.0;
+}
 
 template <int TYPE>
 __device__ __forceinline__ void gen_direction(cptr B, uint64_t seed, uint64_t r, Ray& ray) {
@@ -427,7 +443,8 @@ __device__ __forceinline__ double gen_lambda(cptr B, uint64_t seed, uint64_t r, 
         else j = r % nl;
     }
     if (distr == RT_RANGE_MONO) { j = 0; return mono_lambda(B); }
-    if (distr == RT_RANGE_UNIFORM) return grid_lambda(B, nl, j);
+    // grid_lambda with the host's step: r0 + j * ((r1 - r0) / (nl - 1))
+    if (distr == RT_RANGE_UNIFORM) return nl <= 1 ? r0 : r0 + (double)j * B[BU_LAM_STEP];
     uint32_t o[4];
     philox_stream(seed, j, 2u, o);
     return (r1 - r0) * u01(o[0], o[1]) + r0;

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kThreads) void rt_stage_kernel(StageArgs a) {
 enum BeamOp { GEN_POSITIONS, GEN_DIRECTIONS, GEN_WAVELENGTHS };
 
 struct BeamGenArgs {
-    double beam[RT_BEAM_STRIDE + 3];   // beam fields + BeamUniform (first: kernarg offset 0)
+    double beam[rt::kBeamRec];         // beam fields + BeamUniform (first: kernarg offset 0)
     uint64_t seed, begin, n;
     int op, dir_type;
     double *X, *Y, *Z, *cX, *cY, *cZ, *L;
@@ -356,6 +356,7 @@ static int launch_beam(const double* beam_desc, uint64_t seed, size_t begin, siz
     a.beam[rt::BU_CONE_COS] = cos(ang / 2.0);                         // beam.cu:593-594
     a.beam[rt::BU_CONE_SIN] = sin(ang / 2.0);
     a.beam[rt::BU_GAUSS_TG] = tan(a.beam[RT_B_ANGPAR] / 2.35482 * rt::kPi / 180 / 3600);   // beam.cu:1034,1099
+    a.beam[rt::BU_LAM_STEP] = rt::beam_lam_step(a.beam);
     a.seed = seed; a.begin = begin; a.n = n; a.op = op; a.dir_type = (int)beam_desc[RT_B_TYPE];
     a.X = X; a.Y = Y; a.Z = Z; a.cX = cX; a.cY = cY; a.cZ = cZ; a.L = L;
     hipLaunchKernelGGL(rt_beam_kernel, dim3(grid_for(n)), dim3(kThreads), 0, (hipStream_t)stream, a);

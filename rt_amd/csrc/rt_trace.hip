@@ -937,6 +937,7 @@ static void beam_uniforms(TraceArgs& a) {
     a.beam.u.cone_sin = sin(ang / 2.0);
     double stddev = a.beam.v[RT_B_ANGPAR] / 2.35482 * rt::kPi / 180 / 3600;
     a.beam.u.gauss_tg = tan(stddev);
+    a.beam.u.lam_step = rt::beam_lam_step(a.beam.v);
 }
 
 namespace {
