@@ -366,13 +366,17 @@ def test_spline_scale_and_grating_energy(hip):
         assert float(np.max(err)) <= 1e-10 * peak, (order, float(np.max(err)), peak)
 
 
-@pytest.mark.parametrize("btype,ang", [(S.RT_BEAM_PARALLEL, 0.0), (S.RT_BEAM_GAUSS, 2.35482), (S.RT_BEAM_CONIC, 3.0)],
-                         ids=["parallel", "gauss", "conic"])
+@pytest.mark.parametrize("btype,ang", [(S.RT_BEAM_PARALLEL, 0.0), (S.RT_BEAM_GAUSS, 2.35482), (S.RT_BEAM_GAUSS, 3600.0),
+                                       (S.RT_BEAM_CONIC, 3.0)],
+                         ids=["parallel", "gauss", "gauss-wide", "conic"])
 def test_beam_angles_match_libm_within_ulps(hip, btype, ang):
     """The beam generators' sin/cos (disk angle, Box-Muller angle, conic
     azimuth) use a bounded-range reduction and polynomial of their own; the
     positions and direction cosines agree with the oracle's (C libm) within a
-    few ulps of their scale: |dX|, |dY| <= 1e-15 x outer radius, |dc| <= 1e-15."""
+    few ulps of their scale: |dX|, |dY| <= 1e-15 x outer radius, |dc| <= 1e-15.
+    The Gaussian cases take both forms of 1/sqrt(1 + t^2): the series of a
+    narrow beam (FWHM 2.35482 arcsec, tan sigma ~ 5e-6) and the reciprocal
+    square root of a wide one (FWHM 1 degree, tan sigma ~ 7e-3)."""
     sch = schemes.czerny_turner(40000, 5)
     sch.beam.type = btype
     p = list(sch.beam.params) + [0.0] * (4 - len(sch.beam.params))
