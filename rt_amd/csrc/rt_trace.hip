@@ -133,31 +133,6 @@ __global__ __launch_bounds__(kBlock) void rt_trace_batch_kernel(TraceArgs a) {
 // Per-instance spot from the tile moments, combined in tile order (Chan,
 // Golub and LeVeque's pairwise update): n, centroid x, y, variance x, y and
 // RMS radius sqrt(var x + var y).  One thread per instance.
-__global__ void rt_batch_spots_kernel(const double* tm, const uint64_t* tile_start, uint64_t n_inst,
-                                      double* spots) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_inst) return;
-    double n = 0.0, mx = 0.0, my = 0.0, qx = 0.0, qy = 0.0;
-    for (uint64_t t = tile_start[i]; t < tile_start[i + 1]; ++t) {
-        const double* o = tm + t * 5;
-        const double nb = o[0];
-        if (nb == 0.0) continue;
-        const double nn = n + nb, dx = o[1] - mx, dy = o[2] - my, f = nb / nn;
-        mx += dx * f;
-        my += dy * f;
-        qx += o[3] + dx * dx * n * f;
-        qy += o[4] + dy * dy * n * f;
-        n = nn;
-    }
-    double* out = spots + i * RT_SPOT_STRIDE;
-    out[0] = n;
-    out[1] = mx;
-    out[2] = my;
-    out[3] = n > 0.0 ? qx / n : 0.0;
-    out[4] = n > 0.0 ? qy / n : 0.0;
-    out[5] = n > 0.0 ? sqrt((qx + qy) / n) : 0.0;
-}
-
 // Spots per ray group (group of ray r = r mod G).  Moments (n, mean x,
 // mean y, M2 x, M2 y) are merged with Chan et al.'s pairwise update.
 struct Mom {
@@ -169,6 +144,41 @@ __device__ __forceinline__ Mom mom_merge(const Mom& a, const Mom& b) {
     if (a.n == 0.0) return b;
     const double nn = a.n + b.n, dx = b.mx - a.mx, dy = b.my - a.my, f = b.n / nn;
     return Mom{nn, a.mx + dx * f, a.my + dy * f, a.qx + b.qx + dx * dx * a.n * f, a.qy + b.qy + dy * dy * a.n * f};
+}
+
+// Per-instance spots of a batch from its tiles' moments (n, mean x, mean y,
+// M2 x, M2 y): one workgroup per instance; lane t merges tiles t, t + 256,
+// ... in ascending order, then the lanes' moments merge by a fixed tree, so
+// the result depends on the tiles alone.  (One lane walking an instance's
+// ~4,000 tiles serially took 2.2 ms for 1024 instances.)
+constexpr int kInstSpotThreads = 256;
+
+__global__ __launch_bounds__(kInstSpotThreads) void rt_batch_spots_kernel(const double* tm,
+                                                                         const uint64_t* tile_start,
+                                                                         uint64_t n_inst, double* spots) {
+    __shared__ Mom s[kInstSpotThreads];
+    const uint64_t i = blockIdx.x;
+    Mom m{0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint64_t t = tile_start[i] + threadIdx.x; t < tile_start[i + 1]; t += kInstSpotThreads) {
+        const double* o = tm + t * 5;
+        m = mom_merge(m, Mom{o[0], o[1], o[2], o[3], o[4]});
+    }
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int k = kInstSpotThreads / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) s[threadIdx.x] = mom_merge(s[threadIdx.x], s[threadIdx.x + k]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const Mom r = s[0];
+        double* out = spots + i * RT_SPOT_STRIDE;
+        out[0] = r.n;
+        out[1] = r.mx;
+        out[2] = r.my;
+        out[3] = r.n > 0.0 ? r.qx / r.n : 0.0;
+        out[4] = r.n > 0.0 ? r.qy / r.n : 0.0;
+        out[5] = r.n > 0.0 ? sqrt((r.qx + r.qy) / r.n) : 0.0;
+    }
 }
 
 constexpr int kSpotRays = 8;             // rays of one group per thread in stage 1
@@ -1689,9 +1699,9 @@ int batch_trace(const rt_hip_batch* bc, const double* beam_desc, uint64_t seed, 
         if (rc != RT_OK) return rc;
     }
     if (spots) {
-        unsigned g = (unsigned)((b->n_inst + 255) / 256);
-        hipLaunchKernelGGL(rt_batch_spots_kernel, dim3(g), dim3(256), 0, st, (const double*)b->moments_dev,
-                           (const uint64_t*)b->tile_start_dev, (uint64_t)b->n_inst, spots);
+        hipLaunchKernelGGL(rt_batch_spots_kernel, dim3((unsigned)b->n_inst), dim3(kInstSpotThreads), 0, st,
+                           (const double*)b->moments_dev, (const uint64_t*)b->tile_start_dev, (uint64_t)b->n_inst,
+                           spots);
         HIP_TRY(hipGetLastError(), RT_ERR_FAILED);
     }
     return RT_OK;
